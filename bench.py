@@ -1,0 +1,185 @@
+"""Throughput of the 2D-DIC hot path on one MI355X.
+
+Workload (one step = one frame of the hot path against a resident reference):
+2048x2048 u8 speckle, subset radius 15 (31x31), POIs on a 5 px grid with
+margin 17 (162,409 POIs), FFTCC integer seed from the global integer guess,
+then ICGN2D1 (conv 1e-3, at most 10 iterations).  The reference image, its
+gradients, the FFTCC reference spectra and the per-POI Hessians are prepared
+once before timing; the raw target is resident in HBM.  A step runs the target
+B-spline table, FFTCC and IC-GN and leaves the POIs in HBM.
+
+Prints one JSON line.  value = converged POIs per second (zncc finite >= 0).
+roofline: the IC-GN kernel, algorithmic bytes (2r+1)^2 * 76 B per executed
+POI-iteration over its HIP-event duration on the driver's stream.
+cpu_baseline: the CPU oracle (FP32 restatement of OpenCorr FFTCC2D + ICGN2D1,
+OpenMP) on a bounded sample of the same frame.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "converged POIs/sec (31x31 subset), learned vs FFTCC seed at p99 <=0.01 px"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_SAMPLE = 76  # 64 B B-spline texel + 4 B reference + 8 B gradients
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(name: str):
+    """HBM bytes per IC-GN launch from a committed rocprofv3 --pmc summary
+    (profiles/<name>.json), or None."""
+    p = ROOT / "profiles" / f"{name}.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_dispatch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(sc, res, sample_every: int):
+    """Time the CPU oracle on every sample_every-th POI: FFTCC (uncached, as
+    opencorr_cli.cpp times FFTCC2D::compute) + ICGN2D1 prepare (gradients of the
+    reference, B-spline table of the target: whole images) + compute.  Scale the
+    per-POI part to the full frame."""
+    from oracle import pyoracle
+    from tests.parity import compare
+
+    threads = min(16, os.cpu_count() or 1)
+    os.environ["ORACLE_THREADS"] = str(threads)
+    ref32, tar32 = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
+    pick = np.arange(0, len(sc.pois), sample_every)
+    q = sc.pois[pick].copy()
+    t0 = time.perf_counter()
+    pyoracle.fftcc(ref32, tar32, sc.radius, q)
+    t1 = time.perf_counter()
+    gx, gy = pyoracle.gradient(ref32)
+    coef = pyoracle.bspline(tar32)
+    t2 = time.perf_counter()
+    pyoracle.icgn(ref32, tar32, sc.radius, q, prepared=(gx, gy, coef))
+    t3 = time.perf_counter()
+    per_poi = ((t1 - t0) + (t3 - t2)) / len(pick)
+    t_full = (t2 - t1) + per_poi * len(sc.pois)
+    valid_frac = float(np.mean(np.isfinite(q["result"][:, 2]) & (q["result"][:, 2] >= 0)))
+    value = valid_frac * len(sc.pois) / t_full
+    parity = compare(q, res[pick])
+    return {
+        "value": value, "unit": "POIs/s", "cores": threads, "kind": "port",
+        "sample": (f"{len(pick)} of {len(sc.pois)} POIs (every {sample_every}th): FFTCC2D uncached "
+                   f"{t1 - t0:.2f} s + ICGN2D1 prepare {t2 - t1:.2f} s (whole images) + compute "
+                   f"{t3 - t2:.2f} s; per-POI part scaled to the frame"),
+    }, parity
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=2048)
+    ap.add_argument("--cpu-sample-every", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
+              "(the path shards over POIs; see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import fasticgn_amd as fa
+    from fasticgn_amd import synth
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None  # noqa: E731
+
+    t0 = time.perf_counter()
+    sc = synth.config1(size=args.size)
+    log(f"[bench] scene {sc.name}: {sc.ref.shape} {len(sc.pois)} POIs, guess {sc.meta['initial_guess']}"
+        f" ({time.perf_counter() - t0:.1f} s)")
+
+    drv = fa.FrameDriver(sc.radius, 0.001, 10)
+    drv.set_reference(fa.Image2D(sc.ref))
+    drv.set_pois(sc.pois)
+    t0 = time.perf_counter()
+    drv.upload_target(fa.Image2D(sc.tar))
+    upload_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    st0 = drv.run(policy=0)  # first run also prepares the reference side
+    cold_s = time.perf_counter() - t0
+    log(f"[bench] cold frame (reference prep + frame) {cold_s * 1e3:.1f} ms, upload {upload_s * 1e3:.1f} ms")
+    for _ in range(args.warmup):
+        drv.run(policy=0)
+    sync()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(drv.run(policy=0))
+    sync()
+    wall = time.perf_counter() - t0
+    res = drv.download()
+    log(f"[bench] {args.steps} steps in {wall:.3f} s")
+
+    ms_step = wall * 1e3 / args.steps
+    converged = float(np.mean([s["valid_count"] for s in stats]))
+    value = converged / (ms_step / 1e3)
+    icgn_ms = float(np.mean([s["icgn_ms"] for s in stats]))
+    fftcc_ms = float(np.mean([s["fftcc_ms"] for s in stats]))
+    bspline_ms = float(np.mean([s["bspline_ms"] for s in stats]))
+    dev_ms = float(np.mean([s["device_ms"] for s in stats]))
+    iters = float(np.mean([s["iteration_sum"] for s in stats]))
+    ns = (2 * sc.radius + 1) ** 2
+    alg_bytes = iters * ns * BYTES_PER_SAMPLE
+    achieved = alg_bytes / (icgn_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("icgn_config1"),
+                "kernel": "k_icgn<16,true>", "kernel_ms": round(icgn_ms, 4),
+                "algorithmic_bytes": int(alg_bytes)}
+
+    cpu = None
+    parity = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle) ...")
+        cpu, par = cpu_baseline(sc, res, args.cpu_sample_every)
+        parity = {k: par.get(k) for k in ("n", "status_mismatch", "median_du", "p99_du", "max_du",
+                                           "median_dv", "p99_dv", "max_dv", "pass")}
+        log(f"[bench] CPU baseline {cpu['value']:.0f} POIs/s; parity {parity}")
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "POIs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"2048x2048 u8 speckle (seeded), r=15 (31x31), grid 5, {len(sc.pois)} POIs, "
+                               "FFTCC seed from the global integer guess + ICGN2D1 conv 1e-3 max 10 it, "
+                               "reference resident",
+                   "seeder": "fftcc", "pois": len(sc.pois), "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": {"device_ms": round(dev_ms, 4), "bspline_ms": round(bspline_ms, 4),
+                   "fftcc_ms": round(fftcc_ms, 4), "icgn_ms": round(icgn_ms, 4),
+                   "iterations_per_poi": round(iters / len(sc.pois), 3),
+                   "converged_per_step": converged, "cold_frame_ms": round(cold_s * 1e3, 2),
+                   "target_upload_ms": round(upload_s * 1e3, 2),
+                   "pcie_inclusive_pois_per_s": round(converged / (ms_step / 1e3 + upload_s), 1),
+                   "parity_vs_cpu_oracle": parity},
+    }
+    print(json.dumps(out), flush=True)
+    drv.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

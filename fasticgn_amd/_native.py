@@ -1,0 +1,173 @@
+"""ctypes binding of libfasticgn.so, the C ABI declared in include/fasticgn.h.
+
+The library is built in-tree (``fasticgn_amd/libfasticgn.so``).  There is no
+fallback: if it is missing or does not load, every entry point of this package
+raises, so no result can silently come from anywhere but the gfx950 kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libfasticgn.so"
+
+FI_OK = 0
+FI_E_INVALID_ARGUMENT = -1
+FI_E_LOGIC = -2
+FI_E_OUT_OF_RANGE = -3
+FI_E_DEVICE = -4
+FI_E_UNSUPPORTED = -5
+FI_E_NO_MEMORY = -6
+
+FI_PIXEL_U8, FI_PIXEL_U16, FI_PIXEL_F32 = 0, 1, 2
+FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT, FI_DEPTH12_LEFT = 0, 1, 2
+FI_POLICY_SKIP_FFTCC = 1
+FI_POLICY_POI_FALLBACK = 2
+FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
+
+# opencorr::POI2D, src/oc_poi.h:102-108: 25 float32 = 100 bytes
+POI_DTYPE = np.dtype([
+    ("x", "<f4"), ("y", "<f4"),
+    ("deformation", "<f4", (12,)),
+    ("result", "<f4", (6,)),
+    ("strain", "<f4", (3,)),
+    ("subset_radius", "<f4", (2,)),
+])
+assert POI_DTYPE.itemsize == 100
+
+
+class FrameStats(ctypes.Structure):
+    """fi_frame_stats (include/fasticgn.h)."""
+    _fields_ = [
+        ("frame_index", ctypes.c_int32),
+        ("skip_fftcc_used", ctypes.c_int32),
+        ("fallback_used", ctypes.c_int32),
+        ("poi_fallback_count", ctypes.c_int32),
+        ("poi_fallback_recovered", ctypes.c_int32),
+        ("iteration_max", ctypes.c_int32),
+        ("bad_ratio", ctypes.c_float),
+        ("pre_repair_bad_ratio", ctypes.c_float),
+        ("baseline_bad_ratio", ctypes.c_float),
+        ("fallback_threshold", ctypes.c_float),
+        ("valid_ratio", ctypes.c_float),
+        ("iteration_mean", ctypes.c_float),
+        ("valid_count", ctypes.c_int64),
+        ("iteration_sum", ctypes.c_int64),
+        ("fftcc_pois", ctypes.c_int64),
+        ("device_ms", ctypes.c_double),
+        ("bspline_ms", ctypes.c_double),
+        ("fftcc_ms", ctypes.c_double),
+        ("icgn_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class LogicError(RuntimeError):
+    """std::logic_error of the reference."""
+
+
+class DeviceError(RuntimeError):
+    """A HIP runtime failure."""
+
+
+_EXC = {
+    FI_E_INVALID_ARGUMENT: ValueError,
+    FI_E_LOGIC: LogicError,
+    FI_E_OUT_OF_RANGE: IndexError,
+    FI_E_DEVICE: DeviceError,
+    FI_E_UNSUPPORTED: NotImplementedError,
+    FI_E_NO_MEMORY: MemoryError,
+}
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_S = ctypes.c_size_t
+_F = ctypes.c_float
+_U = ctypes.c_uint
+
+# name -> argtypes; every function returns int
+SIGNATURES = {
+    "fi_version": [ctypes.POINTER(_I)] * 3,
+    "fi_device_count": [ctypes.POINTER(_I)],
+    "fi_last_error": [ctypes.c_char_p, _S],
+    "fi_dft_twiddles": [_I, _P],
+    "fi_fftcc_create": [_I, _I, _I, ctypes.POINTER(_P)],
+    "fi_fftcc_destroy": [_P],
+    "fi_fftcc_set_ref": [_P, _P, _I, _I, _S, _I, _I],
+    "fi_fftcc_set_tar": [_P, _P, _I, _I, _S, _I, _I],
+    "fi_fftcc_prepare_ref_clear": [_P],
+    "fi_fftcc_prepare_ref": [_P, _P, _S],
+    "fi_fftcc_has_prepared_ref": [_P, ctypes.POINTER(_I)],
+    "fi_fftcc_compute": [_P, _P, _S],
+    "fi_fftcc_compute_indexed": [_P, _P, _S, _P],
+    "fi_icgn_create": [_I, _I, _F, _F, _I, ctypes.POINTER(_P)],
+    "fi_icgn_destroy": [_P],
+    "fi_icgn_set_iteration": [_P, _F, _F],
+    "fi_icgn_set_self_adaptive": [_P, _I],
+    "fi_icgn_set_ref": [_P, _P, _I, _I, _S, _I, _I],
+    "fi_icgn_set_tar": [_P, _P, _I, _I, _S, _I, _I],
+    "fi_icgn_prepare_ref": [_P],
+    "fi_icgn_prepare_tar": [_P],
+    "fi_icgn_prepare": [_P],
+    "fi_icgn_compute": [_P, _P, _S],
+    "fi_frame_create": [_I, _F, _F, _I, ctypes.POINTER(_P)],
+    "fi_frame_destroy": [_P],
+    "fi_frame_set_ref": [_P, _P, _I, _I, _S, _I, _I],
+    "fi_frame_set_pois": [_P, _P, _S],
+    "fi_frame_upload_target": [_P, _P, _I, _I, _S, _I, _I],
+    "fi_frame_run": [_P, _U, ctypes.POINTER(FrameStats)],
+    "fi_frame_download": [_P, _P, _S],
+    "fi_frame_reset_sequence": [_P],
+    "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libfasticgn.so once; raise if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("FASTICGN_LIB", LIB_PATH))
+    if not path.exists():
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() "
+                          "(make -C fasticgn_amd/csrc)")
+    lib = ctypes.CDLL(str(path))
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    lib = load()
+    buf = ctypes.create_string_buffer(1024)
+    lib.fi_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status: int) -> None:
+    if status != FI_OK:
+        raise _EXC.get(status, RuntimeError)(last_error())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    status = load().fi_device_count(ctypes.byref(n))
+    return n.value if status == FI_OK else 0
+
+
+def ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)

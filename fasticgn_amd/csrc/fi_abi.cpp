@@ -1,0 +1,272 @@
+// extern "C" boundary (include/fasticgn.h): every entry point catches
+// everything, records the message for fi_last_error and returns a status.
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "fi_engines.h"
+
+namespace fi {
+const std::string& last_error();
+}
+
+using namespace fi;
+
+namespace {
+
+template <typename F>
+int guarded(F&& f)
+{
+    try {
+        f();
+        return FI_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return FI_E_NO_MEMORY;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return FI_E_DEVICE;
+    } catch (...) {
+        set_last_error("unknown failure");
+        return FI_E_DEVICE;
+    }
+}
+
+template <typename H>
+void need(H* h)
+{
+    if (h == nullptr) fail(FI_E_INVALID_ARGUMENT, "null handle");
+}
+
+}  // namespace
+
+extern "C" {
+
+int fi_version(int* major, int* minor, int* patch)
+{
+    return guarded([&] {
+        if (major) *major = 0;
+        if (minor) *minor = 1;
+        if (patch) *patch = 0;
+    });
+}
+
+int fi_device_count(int* count)
+{
+    return guarded([&] {
+        if (!count) fail(FI_E_INVALID_ARGUMENT, "null output");
+        int c = 0;
+        FI_HIP(hipGetDeviceCount(&c));
+        *count = c;
+    });
+}
+
+int fi_last_error(char* buf, size_t len)
+{
+    const std::string& m = last_error();
+    if (buf && len > 0) {
+        const size_t k = m.size() < len - 1 ? m.size() : len - 1;
+        std::memcpy(buf, m.data(), k);
+        buf[k] = '\0';
+    }
+    return (int)m.size();
+}
+
+int fi_dft_twiddles(int n, float* out)
+{
+    return guarded([&] {
+        if (n < 1 || !out) fail(FI_E_INVALID_ARGUMENT, "bad twiddle request");
+        dft_twiddles(n, out);
+    });
+}
+
+// ---- FFTCC2D ----
+int fi_fftcc_create(int rx, int ry, int device, fi_fftcc** out)
+{
+    return guarded([&] {
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *out = nullptr;
+        *out = new fi_fftcc(rx, ry, device);
+    });
+}
+
+int fi_fftcc_destroy(fi_fftcc* h)
+{
+    return guarded([&] { delete h; });
+}
+
+int fi_fftcc_set_ref(fi_fftcc* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+{
+    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
+}
+
+int fi_fftcc_set_tar(fi_fftcc* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+{
+    return guarded([&] { need(h); h->set_tar(px, w, hh, stride, pixel, d12); });
+}
+
+int fi_fftcc_prepare_ref_clear(fi_fftcc* h)
+{
+    return guarded([&] { need(h); h->prepare_ref_clear(); });
+}
+
+int fi_fftcc_prepare_ref(fi_fftcc* h, const fi_poi* layout, size_t n)
+{
+    return guarded([&] { need(h); h->prepare_ref(layout, n); });
+}
+
+int fi_fftcc_has_prepared_ref(fi_fftcc* h, int* out)
+{
+    return guarded([&] {
+        need(h);
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *out = (h->ref.set && h->ready) ? 1 : 0;
+    });
+}
+
+int fi_fftcc_compute(fi_fftcc* h, fi_poi* pois, size_t n)
+{
+    return guarded([&] { need(h); h->compute(pois, n); });
+}
+
+int fi_fftcc_compute_indexed(fi_fftcc* h, fi_poi* pois, size_t n, const size_t* idx)
+{
+    return guarded([&] { need(h); h->compute_indexed(pois, n, idx); });
+}
+
+// ---- ICGN2D1 ----
+int fi_icgn_create(int rx, int ry, float conv, float stop, int device, fi_icgn** out)
+{
+    return guarded([&] {
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *out = nullptr;
+        *out = new fi_icgn(rx, ry, conv, stop, device);
+    });
+}
+
+int fi_icgn_destroy(fi_icgn* h)
+{
+    return guarded([&] { delete h; });
+}
+
+int fi_icgn_set_iteration(fi_icgn* h, float conv, float stop)
+{
+    return guarded([&] {
+        need(h);
+        h->conv = conv;
+        h->stop = stop;
+    });
+}
+
+int fi_icgn_set_self_adaptive(fi_icgn* h, int on)
+{
+    return guarded([&] {
+        need(h);
+        h->self_adaptive = on != 0;
+    });
+}
+
+int fi_icgn_set_ref(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+{
+    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
+}
+
+int fi_icgn_set_tar(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+{
+    return guarded([&] { need(h); h->set_tar(px, w, hh, stride, pixel, d12); });
+}
+
+int fi_icgn_prepare_ref(fi_icgn* h)
+{
+    return guarded([&] { need(h); h->prepare_ref(); });
+}
+
+int fi_icgn_prepare_tar(fi_icgn* h)
+{
+    return guarded([&] { need(h); h->prepare_tar(); });
+}
+
+int fi_icgn_prepare(fi_icgn* h)
+{
+    return guarded([&] {
+        need(h);
+        h->prepare_ref();
+        h->prepare_tar();
+    });
+}
+
+int fi_icgn_compute(fi_icgn* h, fi_poi* pois, size_t n)
+{
+    return guarded([&] { need(h); h->compute(pois, n); });
+}
+
+// ---- frame driver ----
+int fi_frame_create(int radius, float conv, float stop, int device, fi_frame** out)
+{
+    return guarded([&] {
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *out = nullptr;
+        *out = new fi_frame(radius, conv, stop, device);
+    });
+}
+
+int fi_frame_destroy(fi_frame* h)
+{
+    return guarded([&] { delete h; });
+}
+
+int fi_frame_set_ref(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+{
+    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
+}
+
+int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n)
+{
+    return guarded([&] { need(h); h->set_pois(pois, n); });
+}
+
+int fi_frame_upload_target(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+{
+    return guarded([&] { need(h); h->upload_target(px, w, hh, stride, pixel, d12); });
+}
+
+int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats)
+{
+    return guarded([&] { need(h); h->run(policy, stats); });
+}
+
+int fi_frame_download(fi_frame* h, fi_poi* out, size_t n)
+{
+    return guarded([&] { need(h); h->download(out, n); });
+}
+
+int fi_frame_reset_sequence(fi_frame* h)
+{
+    return guarded([&] { need(h); h->reset_sequence(); });
+}
+
+// ---- raw record, opencorr_cli_cuda_mvp.cpp:2498-2536 ----
+int fi_format_raw(const fi_poi* pois, size_t n, void* out, size_t out_len, size_t* written)
+{
+    return guarded([&] {
+        if (n > 0 && (!pois || !out)) fail(FI_E_INVALID_ARGUMENT, "null buffer");
+        if (out_len < n * 36) fail(FI_E_INVALID_ARGUMENT, "output buffer shorter than n * 36 bytes");
+        unsigned char* c = static_cast<unsigned char*>(out);
+        for (size_t i = 0; i < n; i++) {
+            const fi_poi& p = pois[i];
+            const int32_t x = (int32_t)p.x, y = (int32_t)p.y;
+            const float f[7] = { p.deformation[0], p.deformation[6], p.deformation[1], p.deformation[2],
+                                 p.deformation[7], p.deformation[8], p.result[2] };
+            std::memcpy(c, &x, 4);
+            std::memcpy(c + 4, &y, 4);
+            std::memcpy(c + 8, f, 28);
+            c += 36;
+        }
+        if (written) *written = n * 36;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,686 @@
+// Engine implementations: host orchestration of the gfx950 kernels.
+#include "fi_engines.h"
+
+#include <climits>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+
+using namespace fi;
+
+namespace {
+
+int safe_int(float v)
+{
+    // (int) of a non-finite or out-of-range float is undefined in C++; such a
+    // coordinate can never match a prepared layout entry.
+    if (!std::isfinite(v) || v >= 2147483520.f || v <= -2147483520.f) return INT_MIN;
+    return (int)v;
+}
+
+std::string size_str(int w, int h) { return std::to_string(w) + "x" + std::to_string(h); }
+
+size_t max_lds_per_block(int device)
+{
+    int v = 0;
+    FI_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+    return (size_t)v;
+}
+
+hipStream_t make_stream()
+{
+    hipStream_t s = nullptr;
+    FI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+}  // namespace
+
+// ===========================================================================
+// FFTCC2D
+// ===========================================================================
+fi_fftcc::fi_fftcc(int rx, int ry, int dev) : device(dev)
+{
+    // src/oc_fftcc.cpp:154-158 (and src/fftcc2d_cuda.h:85-109)
+    if (rx <= 0 || rx != ry || rx > 23170)
+        fail(FI_E_INVALID_ARGUMENT,
+             "FFTCC2D supports positive square radii only (1..23170); rectangular FFTs are unsupported");
+    if (rx > 32) fail(FI_E_UNSUPPORTED, "FFTCC on the device supports radii 1..32");
+    r = rx;
+    DeviceGuard g(device);
+    if (fftcc_lds_bytes(r, kFftccUncached) > max_lds_per_block(device))
+        fail(FI_E_UNSUPPORTED, "FFTCC window of radius " + std::to_string(r) + " exceeds the LDS per block");
+    stream = make_stream();
+    std::vector<float> t(2 * 2 * r);
+    dft_twiddles(2 * r, t.data());
+    tw.reserve(t.size());
+    FI_HIP(hipMemcpy(tw.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+fi_fftcc::~fi_fftcc()
+{
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void fi_fftcc::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+{
+    DeviceGuard g(device);
+    ref.upload(px, w, h, stride, pixel, depth12, stream);
+    ready = false;  // layout retained for a lazy rebuild (:188-195)
+}
+
+void fi_fftcc::set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+{
+    DeviceGuard g(device);
+    tar.upload(px, w, h, stride, pixel, depth12, stream);
+}
+
+void fi_fftcc::prepare_ref_clear()
+{
+    lay_x.clear();
+    lay_y.clear();
+    ready = false;
+}
+
+void fi_fftcc::build_cache(const fi_poi* layout_dev, size_t n)
+{
+    const size_t nh = (size_t)2 * r * (r + 1);
+    spectra.reserve(std::max<size_t>(1, n * nh * 2));
+    slots.reserve(std::max<size_t>(1, n));
+    FftccParams p{};
+    p.ref = ref.view;
+    p.tar = ref.view;
+    p.r = r;
+    p.mode = kFftccPrepare;
+    p.pois = const_cast<fi_poi*>(layout_dev);
+    p.n_pois = (int)n;
+    p.spectra = spectra.p;
+    p.slots = slots.p;
+    p.twiddles = tw.p;
+    launch_fftcc(p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
+    ready = true;
+}
+
+void fi_fftcc::prepare_ref(const fi_poi* layout, size_t n)
+{
+    if (!ref.set) fail(FI_E_LOGIC, "Reference image must be set before FFTCC2D::prepareRef");
+    if (n > (size_t)INT_MAX) fail(FI_E_INVALID_ARGUMENT, "queue too long");
+    if (n > 0 && layout == nullptr) fail(FI_E_INVALID_ARGUMENT, "layout pointer is null");
+    DeviceGuard g(device);
+    lay_x.resize(n);
+    lay_y.resize(n);
+    for (size_t i = 0; i < n; i++) {
+        lay_x[i] = safe_int(layout[i].x);
+        lay_y[i] = safe_int(layout[i].y);
+    }
+    ready = false;
+    if (n == 0) {
+        ready = true;
+        return;
+    }
+    d_pois.reserve(n);
+    FI_HIP(hipMemcpyAsync(d_pois.p, layout, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    build_cache(d_pois.p, n);
+}
+
+// ensureReferenceCache (:205-213): rebuild the spectra of the retained layout
+void fi_fftcc::ensure_cache()
+{
+    if (ready) return;
+    const size_t n = lay_x.size();
+    std::vector<fi_poi> layout(n);
+    for (size_t i = 0; i < n; i++) {
+        std::memset(&layout[i], 0, sizeof(fi_poi));
+        layout[i].x = (float)lay_x[i];
+        layout[i].y = (float)lay_y[i];
+    }
+    if (n == 0) {
+        ready = true;
+        return;
+    }
+    d_pois.reserve(n);
+    FI_HIP(hipMemcpyAsync(d_pois.p, layout.data(), n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    build_cache(d_pois.p, n);
+}
+
+bool fi_fftcc::layout_matches(const fi_poi* pois, size_t n) const
+{
+    if (lay_x.size() != n) return false;
+    for (size_t i = 0; i < n; i++)
+        if (lay_x[i] != safe_int(pois[i].x) || lay_y[i] != safe_int(pois[i].y)) return false;
+    return true;
+}
+
+void fi_fftcc::check_sizes() const
+{
+    if (!ref.set || !tar.set) fail(FI_E_LOGIC, "FFTCC2D images must be set before compute");
+    // src/fftcc2d_cuda.cu:694-701: a size mismatch is rejected, naming both sizes
+    if (ref.view.width != tar.view.width || ref.view.height != tar.view.height)
+        fail(FI_E_INVALID_ARGUMENT, "FFTCC2D reference " + size_str(ref.view.width, ref.view.height) +
+                                        " and target " + size_str(tar.view.width, tar.view.height) +
+                                        " sizes differ");
+}
+
+void fi_fftcc::run(fi_poi* pois, size_t n, int mode, const uint32_t* slot_host)
+{
+    h_pois.reserve(n);
+    std::memcpy(h_pois.p, pois, n * sizeof(fi_poi));
+    d_pois.reserve(n);
+    FI_HIP(hipMemcpyAsync(d_pois.p, h_pois.p, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    if (slot_host) {
+        d_slot.reserve(n);
+        FI_HIP(hipMemcpyAsync(d_slot.p, slot_host, n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    }
+    FftccParams p{};
+    p.ref = ref.view;
+    p.tar = tar.view;
+    p.r = r;
+    p.mode = mode;
+    p.pois = d_pois.p;
+    p.n_pois = (int)n;
+    p.slot = slot_host ? d_slot.p : nullptr;
+    p.spectra = spectra.p;
+    p.slots = slots.p;
+    p.twiddles = tw.p;
+    launch_fftcc(p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipMemcpyAsync(h_pois.p, d_pois.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipStreamSynchronize(stream));
+    std::memcpy(pois, h_pois.p, n * sizeof(fi_poi));
+}
+
+// compute(vector<POI2D>&), src/oc_fftcc.cpp:529-557
+void fi_fftcc::compute(fi_poi* pois, size_t n)
+{
+    check_sizes();
+    if (n == 0) return;
+    if (pois == nullptr) fail(FI_E_INVALID_ARGUMENT, "POI pointer is null");
+    if (n > (size_t)INT_MAX) fail(FI_E_INVALID_ARGUMENT, "queue too long");
+    DeviceGuard g(device);
+    if (layout_matches(pois, n)) {
+        ensure_cache();
+        run(pois, n, kFftccCached, nullptr);
+    } else {
+        run(pois, n, kFftccUncached, nullptr);
+    }
+}
+
+// compute(queue, cache_indices), src/oc_fftcc.cpp:498-527
+void fi_fftcc::compute_indexed(fi_poi* pois, size_t n, const size_t* idx)
+{
+    if (!ref.set || !tar.set) fail(FI_E_LOGIC, "FFTCC2D images must be set before compute");
+    if (n == 0) return;
+    if (pois == nullptr || idx == nullptr)
+        fail(FI_E_INVALID_ARGUMENT, "FFTCC2D::compute requires one cache index per POI");
+    if (lay_x.empty()) fail(FI_E_LOGIC, "FFTCC2D indexed compute requires a prepared reference layout");
+    std::vector<uint32_t> slot(n);
+    for (size_t i = 0; i < n; i++) {
+        const size_t k = idx[i];
+        if (k >= lay_x.size()) fail(FI_E_OUT_OF_RANGE, "FFTCC2D reference cache index out of range");
+        if (!std::isfinite(pois[i].x) || !std::isfinite(pois[i].y) || lay_x[k] != safe_int(pois[i].x) ||
+            lay_y[k] != safe_int(pois[i].y))
+            fail(FI_E_INVALID_ARGUMENT, "FFTCC2D reference cache index does not match POI coordinates");
+        slot[i] = (uint32_t)k;
+    }
+    check_sizes();
+    DeviceGuard g(device);
+    ensure_cache();
+    run(pois, n, kFftccCached, slot.data());
+}
+
+// ===========================================================================
+// ICGN2D1
+// ===========================================================================
+fi_icgn::fi_icgn(int rx_, int ry_, float conv_, float stop_, int dev)
+    : device(dev), rx(rx_), ry(ry_), conv(conv_), stop(stop_)
+{
+    if (rx < 1 || ry < 1) fail(FI_E_INVALID_ARGUMENT, "ICGN2D1 subset radii must be positive");
+    if (rx > kIcgnMaxRadius || ry > kIcgnMaxRadius)
+        fail(FI_E_UNSUPPORTED, "ICGN on the device supports radii 1.." + std::to_string(kIcgnMaxRadius));
+    DeviceGuard g(device);
+    stream = make_stream();
+}
+
+fi_icgn::~fi_icgn()
+{
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void fi_icgn::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+{
+    DeviceGuard g(device);
+    ref_raw.upload(px, w, h, stride, pixel, depth12, stream);
+    grads_ready = false;
+    ref_version++;
+}
+
+void fi_icgn::set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+{
+    DeviceGuard g(device);
+    tar_raw.upload(px, w, h, stride, pixel, depth12, stream);
+    coef_ready = false;
+}
+
+// prepareRef (:132-142): Gradient2D4 of the reference
+void fi_icgn::prepare_ref()
+{
+    if (!ref_raw.set) fail(FI_E_LOGIC, "ICGN2D1::prepareRef requires a reference image");
+    DeviceGuard g(device);
+    const size_t px = (size_t)ref_raw.view.width * ref_raw.view.height;
+    ref_f.reserve(px);
+    gx.reserve(px);
+    gy.reserve(px);
+    launch_to_float(ref_raw.view, ref_f.p, stream);
+    launch_gradient(ref_f.p, ref_raw.view.width, ref_raw.view.height, gx.p, gy.p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
+    grads_ready = true;
+    ref_version++;
+}
+
+// prepareTar (:144-153): BicubicBspline table of the target
+void fi_icgn::prepare_tar()
+{
+    if (!tar_raw.set) fail(FI_E_LOGIC, "ICGN2D1::prepareTar requires a target image");
+    DeviceGuard g(device);
+    coef.reserve((size_t)16 * tar_raw.view.width * tar_raw.view.height);
+    launch_bspline(tar_raw.view, coef.p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
+    coef_ready = true;
+}
+
+void fi_icgn::compute(fi_poi* pois, size_t n)
+{
+    if (!grads_ready || !coef_ready)
+        fail(FI_E_LOGIC, "ICGN2D1 compute requires prepareRef and prepareTar after the images are set");
+    if (ref_raw.view.width != tar_raw.view.width || ref_raw.view.height != tar_raw.view.height)
+        fail(FI_E_INVALID_ARGUMENT, "ICGN2D1 reference " + size_str(ref_raw.view.width, ref_raw.view.height) +
+                                        " and target " + size_str(tar_raw.view.width, tar_raw.view.height) +
+                                        " sizes differ");
+    if (n == 0) return;
+    if (pois == nullptr) fail(FI_E_INVALID_ARGUMENT, "POI pointer is null");
+    if (n > (size_t)INT_MAX) fail(FI_E_INVALID_ARGUMENT, "queue too long");
+    DeviceGuard g(device);
+
+    // radius groups (self_adaptive: per-POI subset_radius, src/oc_icgn.cpp:176-182)
+    std::map<std::pair<int, int>, std::vector<uint32_t>> groups;
+    std::vector<float> key(4 * n);
+    for (size_t i = 0; i < n; i++) {
+        int gx_r = rx, gy_r = ry;
+        if (self_adaptive) {
+            gx_r = safe_int(pois[i].subset_radius[0]);
+            gy_r = safe_int(pois[i].subset_radius[1]);
+            if (gx_r < 1 || gy_r < 1 || gx_r > kIcgnMaxRadius || gy_r > kIcgnMaxRadius)
+                fail(FI_E_UNSUPPORTED, "self-adaptive subset radius outside 1.." +
+                                           std::to_string(kIcgnMaxRadius) + " at POI " + std::to_string(i));
+            groups[{ gx_r, gy_r }].push_back((uint32_t)i);
+        }
+        key[4 * i] = pois[i].x;
+        key[4 * i + 1] = pois[i].y;
+        key[4 * i + 2] = (float)gx_r;
+        key[4 * i + 3] = (float)gy_r;
+    }
+    const bool prep_hit = prep_version == ref_version && prep_key.size() == key.size() &&
+                          std::memcmp(prep_key.data(), key.data(), key.size() * sizeof(float)) == 0;
+
+    h_pois.reserve(n);
+    std::memcpy(h_pois.p, pois, n * sizeof(fi_poi));
+    d_pois.reserve(n);
+    FI_HIP(hipMemcpyAsync(d_pois.p, h_pois.p, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    prep.reserve(n);
+
+    IcgnParams p{};
+    p.ref = ref_f.p;
+    p.gx = gx.p;
+    p.gy = gy.p;
+    p.coef = coef.p;
+    p.w = ref_raw.view.width;
+    p.h = ref_raw.view.height;
+    p.pois = d_pois.p;
+    p.prep = prep.p;
+    p.conv = conv;
+    p.stop = stop;
+    if (!self_adaptive) {
+        p.index = nullptr;
+        p.n = (int)n;
+        p.rx = rx;
+        p.ry = ry;
+        if (!prep_hit) launch_icgn_prep(p, stream);
+        launch_icgn(p, stream);
+        FI_HIP(hipGetLastError());
+    } else {
+        size_t total = 0;
+        for (auto& kv : groups) total += kv.second.size();
+        d_index.reserve(total);
+        std::vector<uint32_t> flat;
+        flat.reserve(total);
+        std::vector<std::pair<size_t, size_t>> spans;
+        for (auto& kv : groups) {
+            spans.push_back({ flat.size(), kv.second.size() });
+            flat.insert(flat.end(), kv.second.begin(), kv.second.end());
+        }
+        FI_HIP(hipMemcpyAsync(d_index.p, flat.data(), flat.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, stream));
+        p.self_adaptive = 1;
+        size_t gi = 0;
+        for (auto& kv : groups) {
+            p.rx = kv.first.first;
+            p.ry = kv.first.second;
+            p.index = d_index.p + spans[gi].first;
+            p.n = (int)spans[gi].second;
+            if (!prep_hit) launch_icgn_prep(p, stream);
+            launch_icgn(p, stream);
+            FI_HIP(hipGetLastError());
+            gi++;
+        }
+        // the staged index list must outlive the async copy
+        FI_HIP(hipStreamSynchronize(stream));
+    }
+    FI_HIP(hipMemcpyAsync(h_pois.p, d_pois.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipStreamSynchronize(stream));
+    std::memcpy(pois, h_pois.p, n * sizeof(fi_poi));
+    prep_key.swap(key);
+    prep_version = ref_version;
+}
+
+// ===========================================================================
+// Multi-frame driver
+// ===========================================================================
+namespace {
+constexpr float kBadRatioMargin = 0.03f;  // kSkipFftccBadRatioMargin, cli:480
+}
+
+fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
+    : device(dev), r(radius), conv(conv_), stop(stop_)
+{
+    if (radius < 1 || radius > 32)
+        fail(radius < 1 ? FI_E_INVALID_ARGUMENT : FI_E_UNSUPPORTED,
+             "frame driver radius must be 1..32 (FFTCC) and 1.." + std::to_string(kIcgnMaxRadius) + " (ICGN)");
+    if (radius > kIcgnMaxRadius) fail(FI_E_UNSUPPORTED, "ICGN on the device supports radii 1..31");
+    DeviceGuard g(device);
+    if (fftcc_lds_bytes(r, kFftccCached) > max_lds_per_block(device))
+        fail(FI_E_UNSUPPORTED, "FFTCC window exceeds the LDS per block");
+    stream = make_stream();
+    for (auto& e : ev) FI_HIP(hipEventCreate(&e));
+    std::vector<float> t(4 * r);
+    dft_twiddles(2 * r, t.data());
+    tw.reserve(t.size());
+    FI_HIP(hipMemcpy(tw.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    counters.reserve(1);
+    h_counters.reserve(1);
+    iter_total.reserve(1);
+    h_iter_total.reserve(1);
+}
+
+fi_frame::~fi_frame()
+{
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void fi_frame::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+{
+    DeviceGuard g(device);
+    ref_raw.upload(px, w, h, stride, pixel, depth12, stream);
+    ref_set = true;
+    prepared = false;
+    reset_sequence();
+}
+
+void fi_frame::set_pois(const fi_poi* pois, size_t count)
+{
+    if (count > 0 && pois == nullptr) fail(FI_E_INVALID_ARGUMENT, "POI pointer is null");
+    if (count > (size_t)INT_MAX) fail(FI_E_INVALID_ARGUMENT, "queue too long");
+    tmpl_host.assign(pois, pois + count);
+    n = count;
+    pois_set = true;
+    prepared = false;
+    reset_sequence();
+}
+
+void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+{
+    DeviceGuard g(device);
+    if (ref_set && (w != ref_raw.view.width || h != ref_raw.view.height))
+        fail(FI_E_INVALID_ARGUMENT, "reference " + size_str(ref_raw.view.width, ref_raw.view.height) +
+                                        " and target " + size_str(w, h) + " sizes differ");
+    tar.upload(px, w, h, stride, pixel, depth12, stream);
+    tar_ready = true;
+}
+
+void fi_frame::reset_sequence()
+{
+    have_prev = false;
+    baseline_ready = false;
+    baseline_bad = NAN;
+    frame_index = 0;
+}
+
+// prepareRef of both engines for the template (cli:4095-4104): reference f32,
+// gradients, FFTCC spectra, per-POI Hessian records.
+void fi_frame::prepare()
+{
+    if (!ref_set) fail(FI_E_LOGIC, "frame driver needs a reference image");
+    if (!pois_set) fail(FI_E_LOGIC, "frame driver needs a POI template");
+    const int w = ref_raw.view.width, h = ref_raw.view.height;
+    const size_t px = (size_t)w * h;
+    ref_f.reserve(px);
+    gx.reserve(px);
+    gy.reserve(px);
+    launch_to_float(ref_raw.view, ref_f.p, stream);
+    launch_gradient(ref_f.p, w, h, gx.p, gy.p, stream);
+    const size_t cap = std::max<size_t>(n, 1);
+    tmpl.reserve(cap);
+    cur.reserve(cap);
+    prev.reserve(cap);
+    save.reserve(cap);
+    repair.reserve(cap);
+    cand.reserve(cap);
+    prep.reserve(cap);
+    slots.reserve(cap);
+    spectra.reserve(cap * (size_t)2 * r * (r + 1) * 2);
+    if (n > 0) {
+        FI_HIP(hipMemcpyAsync(tmpl.p, tmpl_host.data(), n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+        FftccParams fp{};
+        fp.ref = ref_raw.view;
+        fp.tar = ref_raw.view;
+        fp.r = r;
+        fp.mode = kFftccPrepare;
+        fp.pois = tmpl.p;
+        fp.n_pois = (int)n;
+        fp.spectra = spectra.p;
+        fp.slots = slots.p;
+        fp.twiddles = tw.p;
+        launch_fftcc(fp, stream);
+        IcgnParams ip{};
+        ip.ref = ref_f.p;
+        ip.gx = gx.p;
+        ip.gy = gy.p;
+        ip.w = w;
+        ip.h = h;
+        ip.pois = tmpl.p;
+        ip.n = (int)n;
+        ip.prep = prep.p;
+        ip.rx = r;
+        ip.ry = r;
+        launch_icgn_prep(ip, stream);
+    }
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
+    prepared = true;
+}
+
+void fi_frame::fftcc_pass(fi_poi* queue, int count, const uint32_t* slot_dev)
+{
+    FftccParams fp{};
+    fp.ref = ref_raw.view;
+    fp.tar = tar.view;
+    fp.r = r;
+    fp.mode = kFftccCached;
+    fp.pois = queue;
+    fp.n_pois = count;
+    fp.slot = slot_dev;
+    fp.spectra = spectra.p;
+    fp.slots = slots.p;
+    fp.twiddles = tw.p;
+    launch_fftcc(fp, stream);
+}
+
+void fi_frame::icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index)
+{
+    IcgnParams ip{};
+    ip.ref = ref_f.p;
+    ip.gx = gx.p;
+    ip.gy = gy.p;
+    ip.coef = coef.p;
+    ip.w = ref_raw.view.width;
+    ip.h = ref_raw.view.height;
+    ip.pois = queue;
+    ip.n = count;
+    ip.prep = prep.p;
+    ip.prep_index = prep_index;
+    ip.rx = r;
+    ip.ry = r;
+    ip.conv = conv;
+    ip.stop = stop;
+    ip.iter_total = iter_total.p;
+    launch_icgn(ip, stream);
+}
+
+void fi_frame::zero_counters()
+{
+    FI_HIP(hipMemsetAsync(counters.p, 0, sizeof(FrameCounters), stream));
+}
+
+FrameCounters fi_frame::read_counters()
+{
+    FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, sizeof(FrameCounters), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipStreamSynchronize(stream));
+    return *h_counters.p;
+}
+
+// processFrame, opencorr_cli_cuda_mvp.cpp:3024-3210
+void fi_frame::run(unsigned policy, fi_frame_stats* stats)
+{
+    DeviceGuard g(device);
+    if (!prepared) prepare();
+    if (!tar_ready) fail(FI_E_LOGIC, "frame driver needs an uploaded target");
+    const int w = ref_raw.view.width, h = ref_raw.view.height;
+    const int cnt = (int)n;
+    fi_frame_stats st{};
+    st.frame_index = frame_index;
+    st.baseline_bad_ratio = NAN;
+    st.fallback_threshold = NAN;
+
+    FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
+    FI_HIP(hipEventRecord(ev[0], stream));
+    coef.reserve((size_t)16 * w * h);
+    launch_bspline(tar.view, coef.p, stream);
+    FI_HIP(hipEventRecord(ev[1], stream));
+
+    const bool skip = (policy & FI_POLICY_SKIP_FFTCC) && baseline_ready && have_prev;
+    float bad_ratio = 0.f;
+    double fftcc_ms = 0.0, icgn_ms = 0.0;
+    if (!skip) {
+        FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+        fftcc_pass(cur.p, cnt, nullptr);
+        FI_HIP(hipEventRecord(ev[2], stream));
+        icgn_pass(cur.p, cnt, nullptr);
+        FI_HIP(hipEventRecord(ev[3], stream));
+        FI_HIP(hipGetLastError());
+        st.fftcc_pois = cnt;
+    } else {
+        launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);
+        FI_HIP(hipEventRecord(ev[2], stream));
+        icgn_pass(cur.p, cnt, nullptr);
+        FI_HIP(hipEventRecord(ev[3], stream));
+        zero_counters();
+        const int collect = (policy & FI_POLICY_POI_FALLBACK) ? 1 : 0;
+        launch_scan(cur.p, prev.p, cnt, collect, counters.p, cand.p, stream);
+        FI_HIP(hipGetLastError());
+        const FrameCounters c = read_counters();
+        const float ratio = cnt ? (float)c.bad / (float)cnt : 0.f;
+        st.skip_fftcc_used = 1;
+        st.pre_repair_bad_ratio = ratio;
+        const float threshold = baseline_bad + kBadRatioMargin;
+        if (ratio > threshold) {
+            // whole-frame fallback with per-POI merge (:3115-3152)
+            st.fallback_used = 1;
+            FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+            FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+            fftcc_pass(cur.p, cnt, nullptr);
+            icgn_pass(cur.p, cnt, nullptr);
+            launch_merge_frame_fallback(save.p, cur.p, cnt, stream);
+            FI_HIP(hipGetLastError());
+            st.fftcc_pois = cnt;
+        } else if (collect && c.candidates > 0) {
+            // POI-level repair on cached spectra and resident Hessians (:2955-3022)
+            const int m = (int)c.candidates;
+            launch_gather(tmpl.p, cand.p, m, repair.p, stream);
+            fftcc_pass(repair.p, m, cand.p);
+            icgn_pass(repair.p, m, cand.p);
+            zero_counters();
+            launch_merge_repair(cur.p, repair.p, cand.p, m, counters.p, stream);
+            FI_HIP(hipGetLastError());
+            const FrameCounters rc = read_counters();
+            st.poi_fallback_count = m;
+            st.poi_fallback_recovered = (int32_t)rc.recovered;
+            st.fftcc_pois = m;
+        }
+    }
+    FI_HIP(hipEventRecord(ev[4], stream));
+    zero_counters();
+    launch_qa(cur.p, cnt, counters.p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipMemcpyAsync(h_iter_total.p, iter_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
+    const FrameCounters q = read_counters();
+    bad_ratio = cnt ? (float)q.bad / (float)cnt : 0.f;
+    if (!skip) st.pre_repair_bad_ratio = bad_ratio;
+    if ((policy & FI_POLICY_SKIP_FFTCC) && !baseline_ready && !skip) {
+        baseline_bad = bad_ratio;
+        baseline_ready = true;
+    }
+    if (baseline_ready) {
+        st.baseline_bad_ratio = baseline_bad;
+        st.fallback_threshold = baseline_bad + kBadRatioMargin;
+    }
+    st.bad_ratio = bad_ratio;
+    st.valid_count = (int64_t)q.valid;
+    st.valid_ratio = cnt ? (float)q.valid / (float)cnt : 0.f;
+    st.iteration_sum = (int64_t)*h_iter_total.p;
+    st.iteration_mean = q.iter_finite ? (float)((double)q.iter_sum / (double)q.iter_finite) : 0.f;
+    st.iteration_max = (int32_t)q.iter_max;
+    float ms = 0.f;
+    FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
+    st.device_ms = ms;
+    FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    st.bspline_ms = ms;
+    FI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+    fftcc_ms = ms;
+    FI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+    icgn_ms = ms;
+    st.fftcc_ms = fftcc_ms;
+    st.icgn_ms = icgn_ms;
+
+    std::swap(cur.p, prev.p);
+    std::swap(cur.cap, prev.cap);
+    have_prev = true;
+    frame_index++;
+    if (stats) *stats = st;
+}
+
+void fi_frame::download(fi_poi* out, size_t count)
+{
+    if (!have_prev) fail(FI_E_LOGIC, "no frame has been run");
+    if (count != n) fail(FI_E_INVALID_ARGUMENT, "download length must equal the template length");
+    if (count == 0) return;
+    if (out == nullptr) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");
+    DeviceGuard g(device);
+    FI_HIP(hipMemcpyAsync(out, prev.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipStreamSynchronize(stream));
+}

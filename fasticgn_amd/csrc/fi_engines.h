@@ -1,0 +1,111 @@
+// Engines behind the C ABI.  Each mirrors one reference class's lifecycle:
+//   FftccEngine  <- opencorr::FFTCC2D       src/oc_fftcc.h:56-100
+//   IcgnEngine   <- opencorr::ICGN2D1       src/oc_icgn.h:45-77
+//   FrameEngine  <- the fixed-reference sequence driver,
+//                   opencorr_cli_cuda_mvp.cpp:3024-3210 and :3834-4509
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "fi_host.h"
+
+struct fi_fftcc {
+    int device = 0;
+    int r = 0;
+    hipStream_t stream = nullptr;
+    fi::DeviceImage ref, tar;
+    // prepared reference layout (prepared_ref_, src/oc_fftcc.h:63-70)
+    std::vector<int32_t> lay_x, lay_y;
+    bool ready = false;
+    fi::DevArray<float> spectra;
+    fi::DevArray<fi::FftccSlot> slots;
+    fi::DevArray<float> tw;
+    fi::DevArray<fi_poi> d_pois;
+    fi::DevArray<uint32_t> d_slot;
+    fi::PinnedArray<fi_poi> h_pois;
+
+    fi_fftcc(int rx, int ry, int dev);
+    ~fi_fftcc();
+    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void prepare_ref_clear();
+    void prepare_ref(const fi_poi* layout, size_t n);
+    void compute(fi_poi* pois, size_t n);
+    void compute_indexed(fi_poi* pois, size_t n, const size_t* idx);
+
+private:
+    void build_cache(const fi_poi* layout_dev, size_t n);
+    void ensure_cache();
+    bool layout_matches(const fi_poi* pois, size_t n) const;
+    void run(fi_poi* pois, size_t n, int mode, const uint32_t* slot_host);
+    void check_sizes() const;
+};
+
+struct fi_icgn {
+    int device = 0;
+    int rx = 0, ry = 0;
+    float conv = 0.001f, stop = 10.f;
+    bool self_adaptive = false;
+    hipStream_t stream = nullptr;
+    fi::DeviceImage ref_raw, tar_raw;
+    fi::DevArray<float> ref_f, gx, gy, coef;
+    bool grads_ready = false, coef_ready = false;
+    uint64_t ref_version = 0;
+    // per-POI reference records, keyed by (x, y, rx, ry) per POI
+    std::vector<float> prep_key;
+    uint64_t prep_version = ~0ull;
+    fi::DevArray<fi::PoiPrep> prep;
+    fi::DevArray<fi_poi> d_pois;
+    fi::DevArray<uint32_t> d_index;
+    fi::PinnedArray<fi_poi> h_pois;
+
+    fi_icgn(int rx, int ry, float conv, float stop, int dev);
+    ~fi_icgn();
+    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void prepare_ref();
+    void prepare_tar();
+    void compute(fi_poi* pois, size_t n);
+};
+
+struct fi_frame {
+    int device = 0;
+    int r = 0;
+    float conv = 0.001f, stop = 10.f;
+    hipStream_t stream = nullptr;
+    fi::DeviceImage ref_raw, tar;
+    fi::DevArray<float> ref_f, gx, gy, coef, tw, spectra;
+    fi::DevArray<fi::FftccSlot> slots;
+    fi::DevArray<fi::PoiPrep> prep;
+    std::vector<fi_poi> tmpl_host;
+    fi::DevArray<fi_poi> tmpl, cur, prev, save, repair;
+    fi::DevArray<uint32_t> cand;
+    fi::DevArray<fi::FrameCounters> counters;
+    fi::PinnedArray<fi::FrameCounters> h_counters;
+    fi::DevArray<unsigned long long> iter_total;
+    fi::PinnedArray<unsigned long long> h_iter_total;
+    size_t n = 0;
+    bool ref_set = false, pois_set = false, prepared = false, tar_ready = false;
+    bool have_prev = false, baseline_ready = false;
+    float baseline_bad = NAN;
+    int frame_index = 0;
+    hipEvent_t ev[8] = {};
+
+    fi_frame(int radius, float conv, float stop, int dev);
+    ~fi_frame();
+    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void set_pois(const fi_poi* pois, size_t n);
+    void upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void run(unsigned policy, fi_frame_stats* stats);
+    void download(fi_poi* out, size_t n);
+    void reset_sequence();
+
+private:
+    void prepare();
+    void fftcc_pass(fi_poi* queue, int count, const uint32_t* slot_dev);
+    void icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index);
+    fi::FrameCounters read_counters();
+    void zero_counters();
+};

@@ -1,0 +1,123 @@
+// Host-side runtime shared by the engines: error plumbing (no exception ever
+// crosses the C ABI), device buffers, pinned staging, device-resident images.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fi_launch.h"
+#include "fi_params.h"
+
+namespace fi {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+#define FI_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            ::fi::fail(FI_E_DEVICE, std::string(#call) + " failed: " + hipGetErrorString(e_)); \
+    } while (0)
+
+void set_last_error(const std::string& msg);
+
+// Device buffer that only grows.
+template <typename T>
+struct DevArray {
+    T* p = nullptr;
+    size_t cap = 0;
+    DevArray() = default;
+    DevArray(const DevArray&) = delete;
+    DevArray& operator=(const DevArray&) = delete;
+    ~DevArray() { release(); }
+    void reserve(size_t n)
+    {
+        if (n <= cap) return;
+        release();
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, n * sizeof(T));
+        if (e != hipSuccess)
+            fail(e == hipErrorOutOfMemory ? FI_E_NO_MEMORY : FI_E_DEVICE,
+                 "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+        p = static_cast<T*>(q);
+        cap = n;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Pinned host staging buffer that only grows.
+template <typename T>
+struct PinnedArray {
+    T* p = nullptr;
+    size_t cap = 0;
+    PinnedArray() = default;
+    PinnedArray(const PinnedArray&) = delete;
+    PinnedArray& operator=(const PinnedArray&) = delete;
+    ~PinnedArray() { release(); }
+    void reserve(size_t n)
+    {
+        if (n <= cap) return;
+        release();
+        void* q = nullptr;
+        FI_HIP(hipHostMalloc(&q, n * sizeof(T), hipHostMallocDefault));
+        p = static_cast<T*>(q);
+        cap = n;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// An image uploaded as its raw pixels (u8 / u16 / f32), converted on the fly
+// by the kernels that read it.  Upload validates the 12-bit right-aligned
+// range (src/oc_image.cpp:172-178).
+struct DeviceImage {
+    DevArray<uint8_t> raw;
+    DevImage view;
+    bool set = false;
+    void upload(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12,
+                hipStream_t s);
+};
+
+void check_image_args(const void* px, int width, int height, size_t stride_bytes, int pixel,
+                      int depth12);
+
+// Selects the handle's device for the duration of a call.
+struct DeviceGuard {
+    explicit DeviceGuard(int device) { FI_HIP(hipSetDevice(device)); }
+};
+
+// Twiddle table of the FFTCC DFT specification: (cos, -sin)(2 pi k / n),
+// each rounded from double.
+void dft_twiddles(int n, float* out);
+
+// Frame-policy kernels (k_policy.hip).
+void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, fi_poi* out, int n,
+                               hipStream_t s);
+void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
+                 FrameCounters* counters, uint32_t* candidates, hipStream_t s);
+void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s);
+void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s);
+void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
+                         FrameCounters* counters, hipStream_t s);
+void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s);
+
+}  // namespace fi

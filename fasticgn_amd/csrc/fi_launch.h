@@ -1,0 +1,34 @@
+// Host-side launchers for the gfx950 kernels (defined in k_*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fi_params.h"
+
+namespace fi {
+
+void launch_to_float(const DevImage& img, float* out, hipStream_t s);
+void launch_gradient(const float* img, int w, int h, float* gx, float* gy, hipStream_t s);
+void launch_bspline(const DevImage& img, float* coef, hipStream_t s);
+
+// Dynamic LDS bytes the FFTCC kernel needs for radius r in the given mode.
+size_t fftcc_lds_bytes(int r, int mode);
+void launch_fftcc(const FftccParams& p, hipStream_t s);
+
+// Largest radius the wave-per-POI IC-GN kernels handle.
+constexpr int kIcgnMaxRadius = 31;
+void launch_icgn_prep(const IcgnParams& p, hipStream_t s);
+void launch_icgn(const IcgnParams& p, hipStream_t s);
+
+// Frame-policy helpers (k_policy.hip).
+struct FrameCounters {
+    unsigned long long bad;          // isBadPoiForFallback
+    unsigned long long valid;        // finite zncc >= 0
+    unsigned long long iter_sum;     // sum of finite iteration counts
+    unsigned long long iter_finite;  // POIs with a finite iteration count
+    unsigned long long recovered;    // repaired POIs that are no longer bad
+    unsigned int iter_max;
+    unsigned int candidates;         // POI-level repair candidates written
+};
+
+}  // namespace fi

@@ -1,0 +1,73 @@
+// Kernel parameter blocks and device-resident record layouts, shared by the
+// host engines and the gfx950 kernels.  Plain structs, passed by value.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/fasticgn.h"
+
+namespace fi {
+
+enum PixelKind { kU8 = FI_PIXEL_U8, kU16 = FI_PIXEL_U16, kF32 = FI_PIXEL_F32 };
+
+// A raw image resident on the device (row-major, pitch in elements).
+struct DevImage {
+    const void* data = nullptr;
+    int width = 0, height = 0;
+    int pixel = kF32;
+    size_t pitch = 0;    // elements per row
+    float scale = 1.f;   // 1, or 1/16 for left-aligned 12-bit input
+};
+
+// Per-POI reference-side data kept resident across frames (the reference
+// recomputes it per POI per call on the CPU, src/oc_icgn.cpp:198-234, and
+// caches it per layout on its GPU).  160 bytes, float4-aligned.
+struct alignas(16) PoiPrep {
+    float mean;      // reference subset mean
+    float norm;      // sqrt(sum (r - mean)^2)
+    int32_t status;  // 1 prepared, 0 rejected by the entry gate
+    int32_t rx;      // radius the record was prepared for
+    float hinv[36];  // inverse Hessian, row-major (FP64 inversion, stored FP32)
+};
+static_assert(sizeof(PoiPrep) == 160, "PoiPrep layout");
+
+// FFTCC reference-spectrum cache slot metadata.
+struct FftccSlot {
+    int32_t x, y;    // integer centre the spectrum was taken at
+    float norm;      // sum of squares of the zero-mean window (not sqrt)
+    int32_t valid;
+};
+
+enum FftccMode { kFftccPrepare = 0, kFftccCached = 1, kFftccUncached = 2 };
+
+struct FftccParams {
+    DevImage ref, tar;
+    int r;                    // radius; window n = 2r
+    int mode;
+    fi_poi* pois;             // queue (prepare: the layout, read only)
+    int n_pois;
+    const uint32_t* slot;     // cached mode: slot per POI, nullptr = identity
+    float* spectra;           // slots * n * (r+1) complex, interleaved
+    FftccSlot* slots;
+    const float* twiddles;    // 2n floats
+};
+
+struct IcgnParams {
+    const float* ref;         // reference, f32 row-major
+    const float* gx;
+    const float* gy;
+    const float* coef;        // target B-spline table, 16 floats per texel
+    int w, h;
+    fi_poi* pois;
+    const uint32_t* index;    // POIs this launch processes (nullptr = all)
+    int n;                    // number of POIs this launch processes
+    PoiPrep* prep;            // per-POI reference records
+    const uint32_t* prep_index;  // record of POI i (nullptr = record i)
+    int rx, ry;               // uniform radius (self_adaptive = 0)
+    int self_adaptive;
+    float conv, stop;
+    unsigned long long* iter_total;  // optional: executed iterations, summed
+};
+
+}  // namespace fi

@@ -1,0 +1,77 @@
+// Host runtime: last-error storage, image upload, DFT twiddles.
+#include <cmath>
+#include <string>
+
+#include "fi_host.h"
+
+namespace fi {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+const std::string& last_error() { return g_last_error; }
+
+static size_t bytes_per_pixel(int pixel)
+{
+    return pixel == kU8 ? 1 : pixel == kU16 ? 2 : 4;
+}
+
+void check_image_args(const void* px, int width, int height, size_t stride_bytes, int pixel,
+                      int depth12)
+{
+    if (px == nullptr) fail(FI_E_INVALID_ARGUMENT, "image pointer is null");
+    if (width < 1 || height < 1)
+        fail(FI_E_INVALID_ARGUMENT, "image size must be positive, got " + std::to_string(width) +
+                                        "x" + std::to_string(height));
+    if (pixel != kU8 && pixel != kU16 && pixel != kF32)
+        fail(FI_E_INVALID_ARGUMENT, "unknown pixel format " + std::to_string(pixel));
+    if (depth12 != FI_DEPTH_NATIVE && depth12 != FI_DEPTH12_RIGHT && depth12 != FI_DEPTH12_LEFT)
+        fail(FI_E_INVALID_ARGUMENT, "unknown 12-bit alignment " + std::to_string(depth12));
+    if (depth12 != FI_DEPTH_NATIVE && pixel != kU16)
+        fail(FI_E_INVALID_ARGUMENT, "12-bit alignment applies to 16-bit pixels only");
+    if (stride_bytes < (size_t)width * bytes_per_pixel(pixel))
+        fail(FI_E_INVALID_ARGUMENT, "row stride " + std::to_string(stride_bytes) +
+                                        " is shorter than a row");
+}
+
+void DeviceImage::upload(const void* px, int width, int height, size_t stride_bytes, int pixel,
+                         int depth12, hipStream_t s)
+{
+    check_image_args(px, width, height, stride_bytes, pixel, depth12);
+    if (depth12 == FI_DEPTH12_RIGHT) {
+        // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
+        for (int y = 0; y < height; y++) {
+            const uint16_t* row = reinterpret_cast<const uint16_t*>(
+                static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes);
+            for (int x = 0; x < width; x++)
+                if (row[x] > 4095)
+                    fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
+        }
+    }
+    const size_t bpp = bytes_per_pixel(pixel);
+    const size_t row_bytes = (size_t)width * bpp;
+    raw.reserve(row_bytes * height);
+    FI_HIP(hipMemcpy2DAsync(raw.p, row_bytes, px, stride_bytes, row_bytes, height,
+                            hipMemcpyHostToDevice, s));
+    FI_HIP(hipStreamSynchronize(s));
+    view.data = raw.p;
+    view.width = width;
+    view.height = height;
+    view.pixel = pixel;
+    view.pitch = (size_t)width;
+    view.scale = (depth12 == FI_DEPTH12_LEFT) ? (1.f / 16.f) : 1.f;
+    set = true;
+}
+
+void dft_twiddles(int n, float* out)
+{
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int k = 0; k < n; k++) {
+        double a = two_pi * (double)k / (double)n;
+        out[2 * k] = (float)std::cos(a);
+        out[2 * k + 1] = (float)(-std::sin(a));
+    }
+}
+
+}  // namespace fi

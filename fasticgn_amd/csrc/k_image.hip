@@ -1,0 +1,154 @@
+// Whole-image kernels: raw -> f32 conversion, Gradient2D4 (4th-order central
+// difference) and the BicubicBspline coefficient table.  Each output element
+// performs the same float operations, in the same order, as the reference's
+// scalar loop (this file is compiled with -ffp-contract=off), so the device
+// results equal the CPU restatement bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "fi_device.h"
+#include "fi_launch.h"
+
+namespace fi {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_to_float(const T* __restrict__ raw, size_t pitch, int w,
+                                                  int h, float scale, float* __restrict__ out)
+{
+    const size_t total = (size_t)w * h;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const size_t y = i / w, x = i - y * w;
+        out[i] = load_px<T>(raw, y * pitch + x, scale);
+    }
+}
+
+// Gradient2D4::getGradientX/Y, src/oc_gradient.cpp:37-79: taps applied as
+// acc -= f(c+2)/12; acc += f(c+1)*2/3; acc -= f(c-1)*2/3; acc += f(c-2)/12,
+// zero within 2 px of the border in the derivative's direction.
+__global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ img, int w, int h,
+                                                  float* __restrict__ gx, float* __restrict__ gy)
+{
+    const float first = 1.f / 12.f;
+    const float second = 2.f / 3.f;
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= w || y >= h) return;
+    const size_t i = (size_t)y * w + x;
+    float ax = 0.f, ay = 0.f;
+    if (x >= 2 && x < w - 2) {
+        ax -= img[i + 2] * first;
+        ax += img[i + 1] * second;
+        ax -= img[i - 1] * second;
+        ax += img[i - 2] * first;
+    }
+    if (y >= 2 && y < h - 2) {
+        ay -= img[i + 2 * (size_t)w] * first;
+        ay += img[i + (size_t)w] * second;
+        ay -= img[i - (size_t)w] * second;
+        ay += img[i - 2 * (size_t)w] * first;
+    }
+    gx[i] = ax;
+    gy[i] = ay;
+}
+
+// BC = B * C, src/oc_cubic_bspline.h:52-58, as the float quotients the
+// reference's initialisers produce.
+__device__ __forceinline__ float bc(int i, int j)
+{
+    constexpr float m[4][4] = {
+        { -144.0f / 336.0f, 384.0f / 336.0f, -384.0f / 336.0f, 144.0f / 336.0f },
+        { 342.0f / 336.0f, -702.0f / 336.0f, 450.0f / 336.0f, -90.0f / 336.0f },
+        { -198.0f / 336.0f, -18.0f / 336.0f, 270.0f / 336.0f, -54.0f / 336.0f },
+        { 0.0f / 336.0f, 336.0f / 336.0f, 0.0f / 336.0f, 0.0f / 336.0f },
+    };
+    return m[i][j];
+}
+
+// BicubicBspline::prepare, src/oc_cubic_bspline.cpp:84-132.  One thread per
+// texel; texels outside 1 <= r < h-2, 1 <= c < w-2 are zero.  Output: 16
+// floats per texel, coef[k][l] = P[3-k][3-l],
+// P[k][l] = sum_m sum_n (BC[l][m] * BC[k][n]) * Q[n][m] in (m, n) order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_bspline(const T* __restrict__ raw, size_t pitch, int w,
+                                                 int h, float scale, float* __restrict__ coef)
+{
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (c >= w || r >= h) return;
+    float4* dst = reinterpret_cast<float4*>(coef + 16 * ((size_t)r * w + c));
+    if (r < 1 || r >= h - 2 || c < 1 || c >= w - 2) {
+        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+        dst[0] = z; dst[1] = z; dst[2] = z; dst[3] = z;
+        return;
+    }
+    float q[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            q[i][j] = load_px<T>(raw, (size_t)(r - 1 + i) * pitch + (c - 1 + j), scale);
+    float p[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+            float acc = 0.f;
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++) acc += bc(l, m) * bc(k, n) * q[n][m];
+            p[k][l] = acc;
+        }
+    dst[0] = make_float4(p[3][3], p[3][2], p[3][1], p[3][0]);
+    dst[1] = make_float4(p[2][3], p[2][2], p[2][1], p[2][0]);
+    dst[2] = make_float4(p[1][3], p[1][2], p[1][1], p[1][0]);
+    dst[3] = make_float4(p[0][3], p[0][2], p[0][1], p[0][0]);
+}
+
+static dim3 tile_grid(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
+
+void launch_to_float(const DevImage& img, float* out, hipStream_t s)
+{
+    const size_t total = (size_t)img.width * img.height;
+    const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+    switch (img.pixel) {
+    case kU8:
+        hipLaunchKernelGGL(k_to_float<uint8_t>, dim3(blocks), dim3(256), 0, s,
+                           (const uint8_t*)img.data, img.pitch, img.width, img.height, img.scale, out);
+        break;
+    case kU16:
+        hipLaunchKernelGGL(k_to_float<uint16_t>, dim3(blocks), dim3(256), 0, s,
+                           (const uint16_t*)img.data, img.pitch, img.width, img.height, img.scale, out);
+        break;
+    default:
+        hipLaunchKernelGGL(k_to_float<float>, dim3(blocks), dim3(256), 0, s,
+                           (const float*)img.data, img.pitch, img.width, img.height, img.scale, out);
+        break;
+    }
+}
+
+void launch_gradient(const float* img, int w, int h, float* gx, float* gy, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gradient, tile_grid(w, h), dim3(256), 0, s, img, w, h, gx, gy);
+}
+
+void launch_bspline(const DevImage& img, float* coef, hipStream_t s)
+{
+    const dim3 g = tile_grid(img.width, img.height);
+    switch (img.pixel) {
+    case kU8:
+        hipLaunchKernelGGL(k_bspline<uint8_t>, g, dim3(256), 0, s, (const uint8_t*)img.data,
+                           img.pitch, img.width, img.height, img.scale, coef);
+        break;
+    case kU16:
+        hipLaunchKernelGGL(k_bspline<uint16_t>, g, dim3(256), 0, s, (const uint16_t*)img.data,
+                           img.pitch, img.width, img.height, img.scale, coef);
+        break;
+    default:
+        hipLaunchKernelGGL(k_bspline<float>, g, dim3(256), 0, s, (const float*)img.data,
+                           img.pitch, img.width, img.height, img.scale, coef);
+        break;
+    }
+}
+
+}  // namespace fi

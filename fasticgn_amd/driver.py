@@ -1,0 +1,63 @@
+"""Multi-frame driver: a fixed reference, one target per frame.
+
+The reference, its gradients, the FFTCC reference spectra and the per-POI
+Hessian records stay resident on the GPU; each frame uploads one target, and
+the frame policy of opencorr_cli_cuda_mvp.cpp:3024-3210 (FFTCC + IC-GN on the
+first frame; later frames seeded from the previous result, bad-ratio guard,
+whole-frame fallback with per-POI merge, POI-level repair) runs on the device.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+from ._native import POI_DTYPE, FrameStats
+from .opencorr import Image2D, _check_queue, _Handle
+
+SKIP_FFTCC = nat.FI_POLICY_SKIP_FFTCC
+POI_FALLBACK = nat.FI_POLICY_POI_FALLBACK
+DEFAULT_POLICY = nat.FI_POLICY_DEFAULT
+
+
+class FrameDriver(_Handle):
+    _destroy = "fi_frame_destroy"
+
+    def __init__(self, radius: int, conv_criterion: float = 0.001, stop_condition: float = 10,
+                 device: int = 0):
+        super().__init__()
+        nat.call("fi_frame_create", int(radius), float(conv_criterion), float(stop_condition),
+                 int(device), ctypes.byref(self._h))
+        self.radius = int(radius)
+        self.n = 0
+
+    def set_reference(self, ref: Image2D) -> None:
+        nat.call("fi_frame_set_ref", self._h, *ref.args())
+
+    def set_pois(self, template: np.ndarray) -> None:
+        q = _check_queue(template)
+        nat.call("fi_frame_set_pois", self._h, nat.ptr(q), len(q))
+        self.n = len(q)
+
+    def upload_target(self, tar: Image2D) -> None:
+        nat.call("fi_frame_upload_target", self._h, *tar.args())
+
+    def run(self, policy: int = DEFAULT_POLICY) -> dict:
+        st = FrameStats()
+        nat.call("fi_frame_run", self._h, int(policy), ctypes.byref(st))
+        return st.as_dict()
+
+    def download(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=POI_DTYPE)
+        nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))
+        return out
+
+    def reset_sequence(self) -> None:
+        nat.call("fi_frame_reset_sequence", self._h)
+
+    def process(self, tar: Image2D, policy: int = DEFAULT_POLICY) -> tuple[np.ndarray, dict]:
+        """Upload one target, run the frame, return (results, frame QA)."""
+        self.upload_target(tar)
+        stats = self.run(policy)
+        return self.download(), stats

@@ -1,0 +1,226 @@
+"""OpenCorr-shaped Python API over the gfx950 C ABI.
+
+Names, argument meaning and error behaviour follow the reference's C++ classes
+so that code written against them reads the same:
+
+    FFTCC2D(rx, ry, ...)                   src/oc_fftcc.h:56-100, src/oc_fftcc.cpp:154-557
+    ICGN2D1(rx, ry, conv, stop, ...)       src/oc_icgn.h:45-77,   src/oc_icgn.cpp:95-397
+    POI2D queue                            src/oc_poi.h:102-136 (numpy structured array)
+
+Differences that the device path imposes are explicit: radius limits raise
+NotImplementedError (FI_E_UNSUPPORTED), and ICGN2D1.compute requires
+prepare() after the images change (the reference would read stale tables).
+A POI queue is a numpy array of POI_DTYPE; compute() updates it in place.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Iterable, Sequence
+
+import numpy as np
+
+from . import _native as nat
+from ._native import POI_DTYPE
+
+
+def poi_queue(points: Iterable[Sequence[float]] | np.ndarray) -> np.ndarray:
+    """POI2D(x, y) for each point, every other field zero (POI2D::clear)."""
+    pts = np.asarray(points, dtype=np.float32).reshape(-1, 2)
+    q = np.zeros(len(pts), dtype=POI_DTYPE)
+    q["x"] = pts[:, 0]
+    q["y"] = pts[:, 1]
+    return q
+
+
+def grid_queue(width: int, height: int, radius: int, grid: int, margin: int | None = None) -> np.ndarray:
+    """Integer POI grid of buildPoiQueue (opencorr_cli_cuda_mvp.cpp:2439-2472):
+    x, y from margin (default radius + 2) while < size - margin, step grid."""
+    m = radius + 2 if margin is None else margin
+    xs = np.arange(m, width - m, grid, dtype=np.float32)
+    ys = np.arange(m, height - m, grid, dtype=np.float32)
+    yy, xx = np.meshgrid(ys, xs, indexing="ij")
+    return poi_queue(np.stack([xx.ravel(), yy.ravel()], axis=1))
+
+
+def _check_queue(queue: np.ndarray) -> np.ndarray:
+    if not isinstance(queue, np.ndarray) or queue.dtype != POI_DTYPE:
+        raise TypeError("POI queue must be a numpy array of POI_DTYPE")
+    if not queue.flags.c_contiguous:
+        raise ValueError("POI queue must be C-contiguous")
+    return queue
+
+
+class Image2D:
+    """A grey image: row-major uint8, uint16 or float32 pixels.
+
+    depth12 selects the 12-bit handling of 16-bit input
+    (src/oc_image.cpp:163-185): 'native', 'right' (values must be <= 4095) or
+    'left' (divided by 16)."""
+
+    _DEPTH = {"native": nat.FI_DEPTH_NATIVE, "right": nat.FI_DEPTH12_RIGHT, "left": nat.FI_DEPTH12_LEFT}
+
+    def __init__(self, pixels: np.ndarray, depth12: str = "native"):
+        a = np.ascontiguousarray(pixels)
+        if a.ndim != 2:
+            raise ValueError("Image2D needs a 2-D array")
+        if a.dtype == np.uint8:
+            self.pixel = nat.FI_PIXEL_U8
+        elif a.dtype == np.uint16:
+            self.pixel = nat.FI_PIXEL_U16
+        elif a.dtype == np.float32:
+            self.pixel = nat.FI_PIXEL_F32
+        else:
+            raise TypeError(f"unsupported pixel type {a.dtype}")
+        if depth12 not in self._DEPTH:
+            raise ValueError(f"depth12 must be one of {sorted(self._DEPTH)}")
+        self.data = a
+        self.depth12 = self._DEPTH[depth12]
+        self.height, self.width = a.shape
+
+    def args(self):
+        return (nat.ptr(self.data), self.width, self.height, self.data.strides[0],
+                self.pixel, self.depth12)
+
+    def as_float(self) -> np.ndarray:
+        a = self.data.astype(np.float32)
+        if self.depth12 == nat.FI_DEPTH12_LEFT:
+            a *= np.float32(1.0 / 16.0)
+        return a
+
+
+class _Handle:
+    _destroy = ""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            nat.call(self._destroy, self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class FFTCC2D(_Handle):
+    """opencorr::FFTCC2D on the device (FFTCC2DCuda's device argument)."""
+
+    _destroy = "fi_fftcc_destroy"
+
+    def __init__(self, subset_radius_x: int, subset_radius_y: int, device: int = 0):
+        super().__init__()
+        nat.call("fi_fftcc_create", int(subset_radius_x), int(subset_radius_y), int(device),
+                 ctypes.byref(self._h))
+        self.subset_radius_x = int(subset_radius_x)
+        self.subset_radius_y = int(subset_radius_y)
+        self.ref_img: Image2D | None = None
+        self.tar_img: Image2D | None = None
+
+    def setImages(self, ref: Image2D, tar: Image2D) -> None:
+        self.setRefImage(ref)
+        self.setTarImage(tar)
+
+    def setRefImage(self, ref: Image2D) -> None:
+        nat.call("fi_fftcc_set_ref", self._h, *ref.args())
+        self.ref_img = ref
+
+    def setTarImage(self, tar: Image2D) -> None:
+        nat.call("fi_fftcc_set_tar", self._h, *tar.args())
+        self.tar_img = tar
+
+    def hasPreparedReference(self, ref: Image2D) -> bool:
+        out = ctypes.c_int(0)
+        nat.call("fi_fftcc_has_prepared_ref", self._h, ctypes.byref(out))
+        return bool(out.value) and ref is self.ref_img
+
+    def prepare(self) -> None:
+        self.prepareRef()
+        self.prepareTar()
+
+    def prepareRef(self, queue: np.ndarray | None = None) -> None:
+        if queue is None:
+            nat.call("fi_fftcc_prepare_ref_clear", self._h)
+            return
+        q = _check_queue(queue)
+        nat.call("fi_fftcc_prepare_ref", self._h, nat.ptr(q), len(q))
+
+    def prepareTar(self) -> None:
+        pass  # the target is read directly (src/oc_fftcc.cpp:315)
+
+    def compute(self, queue: np.ndarray, cache_indices: Sequence[int] | None = None) -> None:
+        q = _check_queue(queue)
+        if cache_indices is None:
+            nat.call("fi_fftcc_compute", self._h, nat.ptr(q), len(q))
+            return
+        idx = np.ascontiguousarray(cache_indices, dtype=np.uint64)
+        if len(idx) != len(q):
+            raise ValueError("FFTCC2D::compute requires one cache index per POI")
+        nat.call("fi_fftcc_compute_indexed", self._h, nat.ptr(q), len(q), nat.ptr(idx))
+
+
+class ICGN2D1(_Handle):
+    """opencorr::ICGN2D1 on the device."""
+
+    _destroy = "fi_icgn_destroy"
+
+    def __init__(self, subset_radius_x: int, subset_radius_y: int, conv_criterion: float,
+                 stop_condition: float, device: int = 0):
+        super().__init__()
+        nat.call("fi_icgn_create", int(subset_radius_x), int(subset_radius_y),
+                 float(conv_criterion), float(stop_condition), int(device), ctypes.byref(self._h))
+        self.subset_radius_x = int(subset_radius_x)
+        self.subset_radius_y = int(subset_radius_y)
+        self.conv_criterion = float(conv_criterion)
+        self.stop_condition = float(stop_condition)
+        self.self_adaptive = False
+
+    def setIteration(self, conv_criterion: float, stop_condition: float) -> None:
+        nat.call("fi_icgn_set_iteration", self._h, float(conv_criterion), float(stop_condition))
+        self.conv_criterion, self.stop_condition = float(conv_criterion), float(stop_condition)
+
+    def setSelfAdaptive(self, on: bool) -> None:
+        nat.call("fi_icgn_set_self_adaptive", self._h, int(bool(on)))
+        self.self_adaptive = bool(on)
+
+    def setImages(self, ref: Image2D, tar: Image2D) -> None:
+        self.setRefImage(ref)
+        self.setTarImage(tar)
+
+    def setRefImage(self, ref: Image2D) -> None:
+        nat.call("fi_icgn_set_ref", self._h, *ref.args())
+
+    def setTarImage(self, tar: Image2D) -> None:
+        nat.call("fi_icgn_set_tar", self._h, *tar.args())
+
+    def prepareRef(self) -> None:
+        nat.call("fi_icgn_prepare_ref", self._h)
+
+    def prepareTar(self) -> None:
+        nat.call("fi_icgn_prepare_tar", self._h)
+
+    def prepare(self) -> None:
+        nat.call("fi_icgn_prepare", self._h)
+
+    def compute(self, queue: np.ndarray) -> None:
+        q = _check_queue(queue)
+        nat.call("fi_icgn_compute", self._h, nat.ptr(q), len(q))
+
+
+def format_raw(queue: np.ndarray) -> bytes:
+    """36-byte records x,y (int32) u,v,ux,uy,vx,vy,zncc (float32), little endian
+    (formatRawBinary, opencorr_cli_cuda_mvp.cpp:2498-2536)."""
+    q = _check_queue(queue)
+    out = np.empty(len(q) * 36, dtype=np.uint8)
+    written = ctypes.c_size_t(0)
+    nat.call("fi_format_raw", nat.ptr(q), len(q), nat.ptr(out), out.nbytes, ctypes.byref(written))
+    return out[: written.value].tobytes()

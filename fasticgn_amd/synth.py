@@ -1,0 +1,227 @@
+"""Seeded synthetic speckle scenes for the 2D-DIC workloads.
+
+Every scene is generated from a seed with numpy's PCG64, so the same call
+returns the same bytes on any machine with the same numpy.  A scene holds the
+reference and target images, the POI queue and a truth function giving the
+exact displacement and gradients at a reference point.
+
+Speckle: Gaussian dots with centres uniform over the image, radius
+R ~ U[2, 4] px rendered with sigma = R / 2; intensity
+background + amplitude * min(1, sum of dot profiles), so overlapping dots
+saturate at background + amplitude; additive Gaussian noise; rounded and
+clipped to the pixel range.
+
+Target: the noisy reference resampled with Keys cubic convolution (a = -0.5,
+edge-clamped) at the back-projected position X(x) of every target pixel, where
+x = X + u(X) is the forward displacement map, then rounded and clipped.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable
+
+import numpy as np
+
+from ._native import POI_DTYPE
+from .opencorr import grid_queue
+
+
+@dataclass
+class Scene:
+    name: str
+    ref: np.ndarray
+    tar: np.ndarray
+    pois: np.ndarray
+    radius: int
+    truth: Callable[[np.ndarray, np.ndarray], tuple]  # (X, Y) -> (u, v, ux, uy, vx, vy)
+    meta: dict = field(default_factory=dict)
+    depth12: str = "native"
+
+
+def dot_field(width: int, height: int, n_dots: int, rng: np.random.Generator,
+              radius=(2.0, 4.0)):
+    cx = rng.uniform(0.0, width, n_dots)
+    cy = rng.uniform(0.0, height, n_dots)
+    sigma = rng.uniform(radius[0], radius[1], n_dots) / 2.0
+    return cx, cy, sigma
+
+
+def render_dots(cx, cy, sigma, width: int, height: int, background: float,
+                amplitude: float) -> np.ndarray:
+    """Noise-free speckle image (float64, height x width), pixels sampled at
+    integer coordinates."""
+    reach = int(math.ceil(4.0 * float(np.max(sigma)))) if len(sigma) else 0
+    acc = np.zeros(width * height, dtype=np.float64)
+    fx = np.floor(cx).astype(np.int64)
+    fy = np.floor(cy).astype(np.int64)
+    inv2s2 = 1.0 / (2.0 * sigma * sigma)
+    offsets = [(dx, dy) for dy in range(-reach, reach + 2) for dx in range(-reach, reach + 2)]
+    chunk = 16
+    for k in range(0, len(offsets), chunk):
+        idx_parts, w_parts = [], []
+        for dx, dy in offsets[k:k + chunk]:
+            ix = fx + dx
+            iy = fy + dy
+            ok = (ix >= 0) & (ix < width) & (iy >= 0) & (iy < height)
+            d2 = (ix - cx) ** 2 + (iy - cy) ** 2
+            w = np.exp(-d2 * inv2s2)
+            ok &= d2 <= (4.0 * sigma) ** 2
+            idx_parts.append((iy * width + ix)[ok])
+            w_parts.append(w[ok])
+        acc += np.bincount(np.concatenate(idx_parts), np.concatenate(w_parts), minlength=acc.size)
+    img = background + amplitude * np.minimum(1.0, acc)
+    return img.reshape(height, width)
+
+
+def quantize(img: np.ndarray, rng: np.random.Generator, noise: float, dtype, vmax: float) -> np.ndarray:
+    noisy = img + (rng.normal(0.0, noise, img.shape) if noise > 0 else 0.0)
+    return np.clip(np.rint(noisy), 0, vmax).astype(dtype)
+
+
+def _keys(t: np.ndarray):
+    a = -0.5
+    t1 = t + 1.0
+    t2 = 1.0 - t
+    t3 = 2.0 - t
+    w0 = ((a * t1 - 5 * a) * t1 + 8 * a) * t1 - 4 * a
+    w1 = ((a + 2) * t - (a + 3)) * t * t + 1
+    w2 = ((a + 2) * t2 - (a + 3)) * t2 * t2 + 1
+    w3 = ((a * t3 - 5 * a) * t3 + 8 * a) * t3 - 4 * a
+    return w0, w1, w2, w3
+
+
+def cubic_resample(img: np.ndarray, X: np.ndarray, Y: np.ndarray) -> np.ndarray:
+    """Keys cubic convolution of img at (X, Y), edge-clamped."""
+    h, w = img.shape
+    src = img.astype(np.float64)
+    ix = np.floor(X).astype(np.int64)
+    iy = np.floor(Y).astype(np.int64)
+    wx = _keys(X - ix)
+    wy = _keys(Y - iy)
+    out = np.zeros(X.shape, dtype=np.float64)
+    for j in range(4):
+        yy = np.clip(iy + j - 1, 0, h - 1)
+        row = np.zeros(X.shape, dtype=np.float64)
+        for i in range(4):
+            xx = np.clip(ix + i - 1, 0, w - 1)
+            row += wx[i] * src[yy, xx]
+        out += wy[j] * row
+    return out
+
+
+def warp_target(ref: np.ndarray, inverse_map, dtype, vmax: float, rows_per_chunk: int = 256) -> np.ndarray:
+    """Target pixel x takes the reference resampled at X = inverse_map(x)."""
+    h, w = ref.shape
+    out = np.empty((h, w), dtype=dtype)
+    xs = np.arange(w, dtype=np.float64)
+    for y0 in range(0, h, rows_per_chunk):
+        ys = np.arange(y0, min(h, y0 + rows_per_chunk), dtype=np.float64)
+        yy, xx = np.meshgrid(ys, xs, indexing="ij")
+        X, Y = inverse_map(xx, yy)
+        v = cubic_resample(ref, X, Y)
+        out[y0:y0 + len(ys)] = np.clip(np.rint(v), 0, vmax).astype(dtype)
+    return out
+
+
+def affine_scene(name: str, width: int, height: int, n_dots: int, seed: int, t=(0.0, 0.0),
+                 grad=(0.0, 0.0, 0.0, 0.0), radius: int = 15, grid: int = 10,
+                 background: float = 40.0, peak: float = 180.0, noise: float = 1.5,
+                 dtype=np.uint8, vmax: float = 255.0) -> Scene:
+    """x = c + A (X - c) + t, A = I + [[ux, uy], [vx, vy]], c the image centre."""
+    rng = np.random.default_rng(seed)
+    cx, cy, s = dot_field(width, height, n_dots, rng)
+    ref = quantize(render_dots(cx, cy, s, width, height, background, peak - background), rng,
+                   noise, dtype, vmax)
+    ux, uy, vx, vy = grad
+    A = np.array([[1.0 + ux, uy], [vx, 1.0 + vy]])
+    Ai = np.linalg.inv(A)
+    c = ((width - 1) / 2.0, (height - 1) / 2.0)
+
+    def inverse_map(x, y):
+        dx, dy = x - c[0] - t[0], y - c[1] - t[1]
+        return c[0] + Ai[0, 0] * dx + Ai[0, 1] * dy, c[1] + Ai[1, 0] * dx + Ai[1, 1] * dy
+
+    tar = warp_target(ref, inverse_map, dtype, vmax)
+
+    def truth(X, Y):
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        u = t[0] + ux * (X - c[0]) + uy * (Y - c[1])
+        v = t[1] + vx * (X - c[0]) + vy * (Y - c[1])
+        one = np.ones_like(X)
+        return u, v, ux * one, uy * one, vx * one, vy * one
+
+    pois = grid_queue(width, height, radius, grid)
+    meta = {"seed": seed, "n_dots": n_dots, "translation": list(t), "gradients": list(grad),
+            "grid": grid, "margin": radius + 2, "noise_sigma": noise, "background": background,
+            "peak": peak, "sigma_rule": "R/2, R~U[2,4]", "resample": "keys-cubic a=-0.5"}
+    return Scene(name, ref, tar, pois, radius, truth, meta)
+
+
+def config0(seed: int = 20260) -> Scene:
+    """512x512 u8, ~6000 dots, t = (2.35, -1.70), ux = vy = 1e-3, r = 15, grid 10."""
+    return affine_scene("config0", 512, 512, 6000, seed, t=(2.35, -1.70),
+                        grad=(1e-3, 0.0, 0.0, 1e-3), radius=15, grid=10)
+
+
+def config1(seed: int = 20261, size: int = 2048, grid: int = 5, radius: int = 15) -> Scene:
+    """size^2 u8 speckle at the dot density of config 0.  Displacement
+    u = U + 0.5 sin(2 pi X / 256), v = V + 0.5 cos(2 pi Y / 256), (U, V) drawn
+    from U[-20, 20].  Each POI starts from the global integer guess
+    (round(U), round(V)) so FFTCC's capture range (-(r-1)..r) holds the
+    residual."""
+    rng = np.random.default_rng(seed)
+    U, V = rng.uniform(-20.0, 20.0, 2)
+    n_dots = int(round(6000 * size * size / (512 * 512)))
+    cx, cy, s = dot_field(size, size, n_dots, rng)
+    ref = quantize(render_dots(cx, cy, s, size, size, 40.0, 140.0), rng, 1.5, np.uint8, 255.0)
+    k = 2.0 * math.pi / 256.0
+
+    def fwd(X, Y):
+        return U + 0.5 * np.sin(k * X), V + 0.5 * np.cos(k * Y)
+
+    def inverse_map(x, y):
+        X, Y = x - U, y - V
+        for _ in range(8):
+            u, v = fwd(X, Y)
+            X, Y = x - u, y - v
+        return X, Y
+
+    tar = warp_target(ref, inverse_map, np.uint8, 255.0)
+
+    def truth(X, Y):
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        u, v = fwd(X, Y)
+        z = np.zeros_like(X)
+        return u, v, 0.5 * k * np.cos(k * X), z, z, -0.5 * k * np.sin(k * Y)
+
+    pois = grid_queue(size, size, radius, grid)
+    pois["deformation"][:, 0] = np.float32(round(U))
+    pois["deformation"][:, 6] = np.float32(round(V))
+    meta = {"seed": seed, "n_dots": n_dots, "U": float(U), "V": float(V),
+            "initial_guess": [round(U), round(V)], "grid": grid, "margin": radius + 2,
+            "noise_sigma": 1.5, "sigma_rule": "R/2, R~U[2,4]", "resample": "keys-cubic a=-0.5"}
+    return Scene(f"config1_{size}", ref, tar, pois, radius, truth, meta)
+
+
+def lcg_pair(width: int = 192, height: int = 128, seed: int = 12345):
+    """The reference's FFTCC known-answer images
+    (smoke/library_boundary_probe.cpp:41-47): LCG pixels, target = reference
+    moved by (+2, -1) with fill 127."""
+    state = np.uint32(seed)
+    ref = np.empty((height, width), dtype=np.float32)
+    a, c = np.uint32(1664525), np.uint32(1013904223)
+    with np.errstate(over="ignore"):
+        for y in range(height):
+            for x in range(width):
+                state = np.uint32(a * state + c)
+                ref[y, x] = float(state >> np.uint32(24))
+    tar = np.full_like(ref, 127.0)
+    tar[: height - 1, 2:] = ref[1:, : width - 2]
+    pois = np.zeros(0, dtype=POI_DTYPE)
+    pts = [(x, y) for y in range(32, 96, 16) for x in range(32, 160, 16)]
+    from .opencorr import poi_queue
+    pois = poi_queue(pts)
+    return ref, tar, pois

@@ -1,0 +1,178 @@
+/*
+ * fasticgn.h -- C ABI of the MI355X (gfx950) 2D-DIC hot path.
+ *
+ * The reference exposes this path only as C++ classes (namespace opencorr):
+ *   FFTCC2D        src/oc_fftcc.h:56-100      (CPU)   FFTCC2DCuda  src/fftcc2d_cuda.h:57-114
+ *   ICGN2D1        src/oc_icgn.h:45-77        (CPU)   ICGN2D1CudaMvp src/icgn2d1_cuda_mvp.h:62-143
+ *   POI2D          src/oc_poi.h:102-136
+ * and the multi-frame driver inside its CLI (opencorr_cli_cuda_mvp.cpp:3024-3210,
+ * 3834-4509).  This header is the thin C boundary those classes sit on in this
+ * project (include/fasticgn_opencorr.hpp) and that Python reaches via ctypes.
+ *
+ * Conventions (mirroring the reference's library contract):
+ *   - Every function returns an int status: FI_OK (0) or a negative FI_E_* code.
+ *     Nothing throws across the ABI.  fi_last_error() returns the message of
+ *     the last failure on the calling host thread.
+ *   - Per-POI numerical failures are not errors: they are the reference's zncc
+ *     sentinels -3 (rejected at entry / out of interpolatable range), -4 (not
+ *     converged within the iteration cap), -5 (NaN), src/oc_dic.h:29-35.
+ *   - fi_poi is layout-identical to opencorr::POI2D (25 floats, 100 bytes), so
+ *     std::vector<POI2D>::data() and a numpy structured array pass through.
+ *   - Calls are synchronous: results are in caller memory when a call returns.
+ *     The caller owns every buffer it passes.  One handle is never used from
+ *     two host threads at once; distinct handles may be.
+ *   - Images are row-major.  pixel: FI_PIXEL_U8 / U16 / F32.  depth12:
+ *     FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT (values must be <= 4095) or
+ *     FI_DEPTH12_LEFT (divided by 16), src/oc_image.cpp:163-185.
+ */
+#ifndef FASTICGN_H
+#define FASTICGN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define FI_OK 0
+#define FI_E_INVALID_ARGUMENT (-1) /* std::invalid_argument in the reference */
+#define FI_E_LOGIC (-2)            /* std::logic_error */
+#define FI_E_OUT_OF_RANGE (-3)     /* std::out_of_range */
+#define FI_E_DEVICE (-4)           /* HIP runtime failure (std::runtime_error) */
+#define FI_E_UNSUPPORTED (-5)      /* valid for the reference, not for this device path */
+#define FI_E_NO_MEMORY (-6)
+
+/* ---- pixel formats ------------------------------------------------------ */
+#define FI_PIXEL_U8 0
+#define FI_PIXEL_U16 1
+#define FI_PIXEL_F32 2
+#define FI_DEPTH_NATIVE 0
+#define FI_DEPTH12_RIGHT 1
+#define FI_DEPTH12_LEFT 2
+
+/* opencorr::POI2D (src/oc_poi.h:102-108): Point2D x,y; DeformationVector2D
+ * p[12] = u ux uy uxx uxy uyy v vx vy vxx vxy vyy; Result2D r[6] = u0 v0 zncc
+ * iteration convergence feature; StrainVector2D e[3]; subset_radius x,y. */
+typedef struct fi_poi {
+    float x, y;
+    float deformation[12];
+    float result[6];
+    float strain[3];
+    float subset_radius[2];
+} fi_poi;
+
+/* Per-frame quality record of the raw manifest (opencorr_cli_cuda_mvp.cpp:654-723). */
+typedef struct fi_frame_stats {
+    int32_t frame_index;
+    int32_t skip_fftcc_used;      /* frame seeded from the previous result */
+    int32_t fallback_used;        /* whole-frame FFTCC re-run and per-POI merge */
+    int32_t poi_fallback_count;   /* POI-level repair candidates */
+    int32_t poi_fallback_recovered;
+    int32_t iteration_max;
+    float bad_ratio;              /* after repair / merge */
+    float pre_repair_bad_ratio;
+    float baseline_bad_ratio;     /* NaN until established */
+    float fallback_threshold;     /* NaN until established */
+    float valid_ratio;
+    float iteration_mean;
+    int64_t valid_count;
+    int64_t iteration_sum;        /* sum of executed IC-GN iterations */
+    int64_t fftcc_pois;           /* POIs run through FFTCC this frame */
+    double device_ms;             /* device time of the frame (HIP events) */
+    double bspline_ms;            /* target B-spline table */
+    double fftcc_ms;              /* FFTCC seeding (or temporal seed copy) */
+    double icgn_ms;               /* first IC-GN pass over the frame */
+} fi_frame_stats;
+
+typedef struct fi_fftcc fi_fftcc;
+typedef struct fi_icgn fi_icgn;
+typedef struct fi_frame fi_frame;
+
+/* ---- library ------------------------------------------------------------ */
+int fi_version(int* major, int* minor, int* patch);
+int fi_device_count(int* count);
+/* Copies the message of the calling thread's last failure (NUL-terminated,
+ * truncated to len).  Returns the length of the full message. */
+int fi_last_error(char* buf, size_t len);
+/* The FFTCC DFT twiddle table for length n (2n floats: cos, -sin), as the
+ * device kernels use it.  For tests of the DFT specification. */
+int fi_dft_twiddles(int n, float* out);
+
+/* ---- FFTCC2D -------------------------------------------------------------
+ * FFTCC2D(int rx, int ry, int threads)    src/oc_fftcc.cpp:154-170
+ * FFTCC2DCuda(int rx, int ry, int device) src/fftcc2d_cuda.h:85-109
+ * Square radii 1..23170 are valid (else FI_E_INVALID_ARGUMENT, as the
+ * reference throws invalid_argument); this device path runs 1..32 and returns
+ * FI_E_UNSUPPORTED above that. */
+int fi_fftcc_create(int rx, int ry, int device, fi_fftcc** out);
+int fi_fftcc_destroy(fi_fftcc* h);
+/* setRefImage / setTarImage (:188-202): content-change notifications. */
+int fi_fftcc_set_ref(fi_fftcc* h, const void* px, int width, int height,
+                     size_t stride_bytes, int pixel, int depth12);
+int fi_fftcc_set_tar(fi_fftcc* h, const void* px, int width, int height,
+                     size_t stride_bytes, int pixel, int depth12);
+/* prepareRef() (:231-236) clears the layout; prepareRef(queue) (:246-313)
+ * caches the reference spectra of that layout on the device. */
+int fi_fftcc_prepare_ref_clear(fi_fftcc* h);
+int fi_fftcc_prepare_ref(fi_fftcc* h, const fi_poi* layout, size_t n);
+int fi_fftcc_has_prepared_ref(fi_fftcc* h, int* out);
+/* compute(vector<POI2D>&) (:529-557): cached when the queue matches the
+ * prepared layout, uncached otherwise. */
+int fi_fftcc_compute(fi_fftcc* h, fi_poi* pois, size_t n);
+/* compute(queue, cache_indices) (:498-527): indexed against the cached layout;
+ * FI_E_INVALID_ARGUMENT / FI_E_LOGIC / FI_E_OUT_OF_RANGE as the reference throws. */
+int fi_fftcc_compute_indexed(fi_fftcc* h, fi_poi* pois, size_t n, const size_t* cache_indices);
+
+/* ---- ICGN2D1 -------------------------------------------------------------
+ * ICGN2D1(int rx, int ry, float conv, float stop, int threads) src/oc_icgn.cpp:95-112
+ * ICGN2D1CudaMvp(rx, ry, conv, int stop, device, dual_gpu)    src/icgn2d1_cuda_mvp.h:98-136
+ * This device path runs radii 1..31. */
+int fi_icgn_create(int rx, int ry, float conv, float stop, int device, fi_icgn** out);
+int fi_icgn_destroy(fi_icgn* h);
+int fi_icgn_set_iteration(fi_icgn* h, float conv, float stop);       /* :120-124 */
+int fi_icgn_set_self_adaptive(fi_icgn* h, int on);                   /* src/oc_dic.h:62 */
+int fi_icgn_set_ref(fi_icgn* h, const void* px, int width, int height,
+                    size_t stride_bytes, int pixel, int depth12);
+int fi_icgn_set_tar(fi_icgn* h, const void* px, int width, int height,
+                    size_t stride_bytes, int pixel, int depth12);
+int fi_icgn_prepare_ref(fi_icgn* h);   /* Gradient2D4 of the reference (:132-142) */
+int fi_icgn_prepare_tar(fi_icgn* h);   /* BicubicBspline table of the target (:144-153) */
+int fi_icgn_prepare(fi_icgn* h);       /* both (:155-159) */
+/* compute(vector<POI2D>&) (:389-397, per POI :168-387). */
+int fi_icgn_compute(fi_icgn* h, fi_poi* pois, size_t n);
+
+/* ---- multi-frame driver --------------------------------------------------
+ * The fixed-reference sequence of opencorr_cli_cuda_mvp.cpp:3834-4509 with the
+ * frame policy of processFrame (:3024-3210): the reference, its gradients,
+ * per-POI Hessians and FFTCC reference spectra stay resident; each frame
+ * uploads one target.  policy flags: */
+#define FI_POLICY_SKIP_FFTCC 1u   /* seed frames > 0 from the previous result (:560-624) */
+#define FI_POLICY_POI_FALLBACK 2u /* POI-level repair of bad POIs (:2955-3022) */
+#define FI_POLICY_DEFAULT (FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK)
+int fi_frame_create(int radius, float conv, float stop, int device, fi_frame** out);
+int fi_frame_destroy(fi_frame* h);
+int fi_frame_set_ref(fi_frame* h, const void* px, int width, int height,
+                     size_t stride_bytes, int pixel, int depth12);
+/* POI template (buildPoiQueue, :2439-2472, or any explicit list). */
+int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n);
+/* Upload one target (host memory) into the device staging buffer. */
+int fi_frame_upload_target(fi_frame* h, const void* px, int width, int height,
+                           size_t stride_bytes, int pixel, int depth12);
+/* Process the uploaded target with the given policy; results stay on the
+ * device (fi_frame_download copies them out).  stats may be NULL. */
+int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats);
+int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
+/* Forget the previous frames (next run seeds by FFTCC again). */
+int fi_frame_reset_sequence(fi_frame* h);
+
+/* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
+ * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */
+int fi_format_raw(const fi_poi* pois, size_t n, void* out, size_t out_len, size_t* written);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

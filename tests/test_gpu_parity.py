@@ -1,0 +1,318 @@
+"""Parity of the gfx950 path (through the C ABI) with the CPU oracle.
+
+Bars:
+  FFTCC  integer peaks, zncc and every other POI field bit-identical to the
+         oracle (both follow one DFT specification).
+  ICGN   validity flags (valid / -3 / -4 / -5) identical; u, v within the
+         reference's parity gate, per frame median <= 1e-3 px, nearest-rank
+         p99 <= 1e-2 px, max <= 0.5 px (smoke/check_cpu_gpu_parity.py:46-98).
+"""
+import numpy as np
+import pytest
+
+import fasticgn_amd as fa
+from fasticgn_amd import synth
+from fasticgn_amd.opencorr import poi_queue
+from tests.parity import compare, finite_valid
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c0():
+    return synth.config0()
+
+
+def fields_equal(a, b):
+    return a.tobytes() == b.tobytes()
+
+
+# ------------------------------------------------------------------ FFTCC --
+
+def test_fftcc_lcg_known_answer(oracle):
+    ref, tar, q = synth.lcg_pair()
+    say("fftcc KAT r=7 192x128", len(q), "POIs")
+    R, T = fa.Image2D(ref), fa.Image2D(tar)
+    with fa.FFTCC2D(7, 7) as f:
+        f.setImages(R, T)
+        f.prepareRef(q)
+        assert f.hasPreparedReference(R)
+        g = q.copy()
+        f.compute(g)
+    assert np.all(g["deformation"][:, 0] == 2) and np.all(g["deformation"][:, 6] == -1)
+    assert np.all(g["result"][:, 2] > 0.5)
+    o = q.copy()
+    oracle.PreparedRef(ref, 7, q).compute(tar, o)
+    assert fields_equal(g, o)
+
+
+@pytest.mark.parametrize("radius", [7, 10, 15, 20])
+def test_fftcc_bit_exact_cached_and_uncached(oracle, c0, radius):
+    say("fftcc config0 radius", radius)
+    ref, tar = c0.ref, c0.tar
+    q = fa.grid_queue(512, 512, radius, 9)
+    R, T = fa.Image2D(ref), fa.Image2D(tar)
+    with fa.FFTCC2D(radius, radius) as f:
+        f.setImages(R, T)
+        f.prepareRef(q)
+        g_cached = q.copy()
+        f.compute(g_cached)
+        moved = q.copy()
+        moved["x"] += 1.0  # layout differs -> uncached path
+        moved["deformation"][:, 0] = 1.0
+        g_unc = moved.copy()
+        f.compute(g_unc)
+    o_cached = q.copy()
+    oracle.PreparedRef(ref.astype(np.float32), radius, q).compute(tar.astype(np.float32), o_cached)
+    o_unc = moved.copy()
+    oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), radius, o_unc)
+    assert fields_equal(g_cached, o_cached)
+    assert fields_equal(g_unc, o_unc)
+    assert (g_cached["result"][:, 2] > 0).mean() > 0.9
+
+
+def test_fftcc_indexed_and_errors(oracle, c0):
+    q = fa.grid_queue(512, 512, 15, 20)
+    R, T = fa.Image2D(c0.ref), fa.Image2D(c0.tar)
+    with fa.FFTCC2D(15, 15) as f:
+        f.setImages(R, T)
+        with pytest.raises(fa.LogicError):
+            f.compute(q[:3].copy(), [0, 1, 2])  # no layout yet
+        f.prepareRef(q)
+        idx = np.array([5, 17, 3, 40])
+        sub = q[idx].copy()
+        f.compute(sub, idx)
+        with pytest.raises(ValueError):
+            f.compute(q[[1, 2]].copy(), [2, 1])  # coordinates do not match
+        with pytest.raises(IndexError):
+            f.compute(q[:1].copy(), [len(q)])
+        with pytest.raises(ValueError):
+            f.compute(q[:2].copy(), [0])
+        small = fa.Image2D(np.zeros((64, 96), np.uint8))
+        f.setTarImage(small)
+        before = q.copy()
+        with pytest.raises(ValueError, match="512x512.*96x64"):
+            f.compute(before)
+        assert fields_equal(before, q)
+    o = q[idx].copy()
+    oracle.PreparedRef(c0.ref.astype(np.float32), 15, q).compute(c0.tar.astype(np.float32), o, idx)
+    assert fields_equal(sub, o)
+
+
+def test_fftcc_empty_queue(c0):
+    with fa.FFTCC2D(15, 15) as f:
+        f.setImages(fa.Image2D(c0.ref), fa.Image2D(c0.tar))
+        q = poi_queue(np.zeros((0, 2)))
+        f.compute(q)
+        f.prepareRef(q)
+        f.compute(q)
+
+
+# ------------------------------------------------------------------- ICGN --
+
+def run_icgn_gpu(ref, tar, radius, q, conv=0.001, stop=10.0, self_adaptive=False):
+    g = q.copy()
+    with fa.ICGN2D1(radius, radius, conv, stop) as ic:
+        if self_adaptive:
+            ic.setSelfAdaptive(True)
+        ic.setImages(fa.Image2D(ref), fa.Image2D(tar))
+        ic.prepare()
+        ic.compute(g)
+    return g
+
+
+def report(tag, cmp):
+    keys = ["n", "cpu_valid", "gpu_valid", "status_mismatch", "median_du", "p99_du", "max_du",
+            "median_dv", "p99_dv", "max_dv"]
+    say(tag, {k: cmp.get(k) for k in keys})
+
+
+def test_icgn_config0_parity(oracle, c0):
+    say("icgn config0 r=15", len(c0.pois), "POIs")
+    ref, tar = c0.ref, c0.tar
+    seeds = c0.pois.copy()
+    oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), 15, seeds)
+    g = run_icgn_gpu(ref, tar, 15, seeds)
+    o = seeds.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o)
+    cmp = compare(o, g)
+    report("icgn config0 vs FP32 oracle", cmp)
+    assert cmp["pass"], cmp
+    o64 = seeds.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o64, f64=True)
+    c64 = compare(o64, g)
+    report("icgn config0 vs FP64 oracle", c64)
+    assert c64["pass"]
+    ok = finite_valid(g)
+    u, v, *_ = c0.truth(g["x"], g["y"])
+    err = np.hypot(g["deformation"][ok, 0] - u[ok], g["deformation"][ok, 6] - v[ok])
+    say("icgn config0 vs truth rms", float(np.sqrt(np.mean(err ** 2))))
+    assert np.sqrt(np.mean(err ** 2)) < 0.03
+    # untouched fields pass through
+    np.testing.assert_array_equal(g["strain"], seeds["strain"])
+    np.testing.assert_array_equal(g["deformation"][:, [3, 4, 5, 9, 10, 11]],
+                                  seeds["deformation"][:, [3, 4, 5, 9, 10, 11]])
+
+
+@pytest.mark.parametrize("radius", [7, 10, 20])
+def test_icgn_other_radii_parity(oracle, c0, radius):
+    say("icgn config0 radius", radius)
+    ref, tar = c0.ref, c0.tar
+    q = fa.grid_queue(512, 512, radius, 13)
+    oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), radius, q)
+    g = run_icgn_gpu(ref, tar, radius, q)
+    o = q.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), radius, o)
+    cmp = compare(o, g)
+    report(f"icgn r={radius}", cmp)
+    assert cmp["pass"], cmp
+
+
+def test_icgn_edge_cases(oracle, c0):
+    """OOB band, zero texture (-5), entry gate: statuses identical, and the
+    fields the reference leaves alone are left alone."""
+    ref = c0.ref.copy()
+    ref[100:200, 100:200] = 128
+    tar = np.roll(ref, (-2, 3), axis=(0, 1))
+    pts = [(x, y) for y in range(15, 498, 11) for x in range(15, 498, 11)]
+    q = poi_queue(pts)
+    q["deformation"][:, 0] = 3
+    q["deformation"][:, 6] = -2
+    q["result"][::7, 2] = -4.0
+    q["deformation"][3::11, 0] = np.nan
+    q["result"][5::13, 2] = np.nan
+    g = run_icgn_gpu(ref, tar, 15, q)
+    o = q.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o)
+    cmp = compare(o, g)
+    report("icgn edge cases", cmp)
+    assert cmp["status_mismatch"] == 0
+    assert cmp["pass"], cmp
+    zc = o["result"][:, 2]
+    assert (zc == -5).any() and (zc == -3).any() and (zc == -4).any()
+    bad = ~finite_valid(o)
+    # rejected POIs: every field but zncc equals the oracle's
+    for name in ("x", "y", "deformation", "strain"):
+        np.testing.assert_array_equal(g[name][bad], o[name][bad])
+
+
+def test_icgn_self_adaptive_mixed_radii(oracle, c0):
+    ref, tar = c0.ref, c0.tar
+    q = fa.grid_queue(512, 512, 20, 12)
+    rr = np.array([10, 15, 20], dtype=np.float32)[np.arange(len(q)) % 3]
+    q["subset_radius"][:, 0] = q["subset_radius"][:, 1] = rr
+    oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), 15, q)
+    g = run_icgn_gpu(ref, tar, 15, q, self_adaptive=True)
+    o = q.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o, self_adaptive=True)
+    cmp = compare(o, g)
+    report("icgn mixed radii", cmp)
+    assert cmp["pass"], cmp
+    np.testing.assert_array_equal(g["subset_radius"], o["subset_radius"])
+
+
+def test_icgn_iteration_cap_and_conv(oracle, c0):
+    """A tight criterion with a small cap produces -4 the same way."""
+    ref, tar = c0.ref, c0.tar
+    q = c0.pois[:600].copy()
+    oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), 15, q)
+    g = run_icgn_gpu(ref, tar, 15, q, conv=1e-6, stop=2)
+    o = q.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o, conv=1e-6, stop=2)
+    cmp = compare(o, g)
+    report("icgn cap", cmp)
+    assert (o["result"][:, 2] == -4).sum() > 100
+    assert cmp["status_mismatch"] == 0
+
+
+def test_icgn_deterministic(c0):
+    seeds = c0.pois.copy()
+    seeds["deformation"][:, 0] = 2.0
+    seeds["deformation"][:, 6] = -2.0
+    a = run_icgn_gpu(c0.ref, c0.tar, 15, seeds)
+    b = run_icgn_gpu(c0.ref, c0.tar, 15, seeds)
+    assert fields_equal(a, b)
+
+
+# ------------------------------------------------------------ frame driver --
+
+def test_frame_driver_matches_engines(oracle, c0):
+    say("frame driver config0 first frame")
+    with fa.FrameDriver(15) as d:
+        d.set_reference(fa.Image2D(c0.ref))
+        d.set_pois(c0.pois)
+        res, st = d.process(fa.Image2D(c0.tar), policy=0)
+    q = c0.pois.copy()
+    with fa.FFTCC2D(15, 15) as f:
+        f.setImages(fa.Image2D(c0.ref), fa.Image2D(c0.tar))
+        f.prepareRef(q)
+        f.compute(q)
+    g = run_icgn_gpu(c0.ref, c0.tar, 15, q)
+    assert fields_equal(res, g)
+    assert st["fftcc_pois"] == len(q)
+    assert st["valid_count"] == int(finite_valid(res).sum())
+    say("frame stats", st)
+
+
+def test_frame_driver_sequence(oracle, c0):
+    """Frames of growing translation against a fixed reference, with the
+    reference's default policy (skip-FFTCC seeding, bad-ratio guard, POI-level
+    repair); each frame is checked against the CPU restatement of the policy."""
+    from tests.policy import PolicyOracle
+    ref = c0.ref
+    steps = [(0.0, 0.0), (0.6, -0.4), (1.3, -0.9), (6.0, 4.0), (6.4, 4.3)]
+    po = PolicyOracle(oracle, ref, c0.pois, 15)
+    with fa.FrameDriver(15) as d:
+        d.set_reference(fa.Image2D(ref))
+        d.set_pois(c0.pois)
+        for k, t in enumerate(steps):
+            sc = synth.affine_scene("seq", 512, 512, 6000, 20260, t=t,
+                                    grad=(4e-4 * k, 0, 0, 4e-4 * k), radius=15, grid=10)
+            res, st = d.process(fa.Image2D(sc.tar))
+            o, ost = po.frame(sc.tar)
+            say("frame", k, {x: st[x] for x in ("skip_fftcc_used", "fallback_used", "poi_fallback_count",
+                                                  "bad_ratio", "valid_ratio", "iteration_mean")}, ost)
+            for key in ("skip_fftcc_used", "fallback_used", "poi_fallback_count"):
+                assert st[key] == ost[key], key
+            cmp = compare(o, res)
+            report(f"frame {k}", cmp)
+            assert cmp["pass"], cmp
+
+
+# ------------------------------------------------------------- full size --
+
+def test_config1_full_size_properties(oracle):
+    """2048^2, ~162k POIs through the frame driver; parity on a POI sample
+    against the oracle, accuracy against the known field on every POI."""
+    sc = synth.config1()
+    say("config1", sc.ref.shape, len(sc.pois), "POIs", sc.meta["initial_guess"])
+    with fa.FrameDriver(15) as d:
+        d.set_reference(fa.Image2D(sc.ref))
+        d.set_pois(sc.pois)
+        res, st = d.process(fa.Image2D(sc.tar), policy=0)
+    say("config1 stats", st)
+    ok = finite_valid(res)
+    assert ok.mean() > 0.99
+    u, v, *_ = sc.truth(res["x"], res["y"])
+    err = np.maximum(np.abs(res["deformation"][ok, 0] - u[ok]), np.abs(res["deformation"][ok, 6] - v[ok]))
+    say("config1 truth p99", float(np.percentile(err, 99)), "max", float(err.max()))
+    assert np.percentile(err, 99) < 0.1
+    rng = np.random.default_rng(0)
+    pick = np.sort(rng.choice(len(sc.pois), 3000, replace=False))
+    ref32, tar32 = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
+    # FFTCC at full size: bit-identical on the sample
+    o = sc.pois[pick].copy()
+    oracle.fftcc(ref32, tar32, 15, o)
+    g = sc.pois[pick].copy()
+    with fa.FFTCC2D(15, 15) as f:
+        f.setImages(fa.Image2D(sc.ref), fa.Image2D(sc.tar))
+        f.compute(g)
+    assert fields_equal(g, o)
+    oracle.icgn(ref32, tar32, 15, o)
+    cmp = compare(o, res[pick])
+    report("config1 sample", cmp)
+    assert cmp["pass"], cmp
