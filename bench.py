@@ -99,11 +99,7 @@ def main() -> int:
 
     import fasticgn_amd as fa
     from fasticgn_amd import synth
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
+    from fasticgn_amd._native import synchronize as sync
 
     t0 = time.perf_counter()
     sc = synth.config1(size=args.size)

@@ -94,6 +94,7 @@ _U = ctypes.c_uint
 SIGNATURES = {
     "fi_version": [ctypes.POINTER(_I)] * 3,
     "fi_device_count": [ctypes.POINTER(_I)],
+    "fi_device_synchronize": [_I],
     "fi_last_error": [ctypes.c_char_p, _S],
     "fi_dft_twiddles": [_I, _P],
     "fi_fftcc_create": [_I, _I, _I, ctypes.POINTER(_P)],
@@ -167,6 +168,10 @@ def device_count() -> int:
     n = ctypes.c_int(0)
     status = load().fi_device_count(ctypes.byref(n))
     return n.value if status == FI_OK else 0
+
+
+def synchronize(device: int = 0) -> None:
+    call("fi_device_synchronize", int(device))
 
 
 def ptr(a: np.ndarray) -> ctypes.c_void_p:

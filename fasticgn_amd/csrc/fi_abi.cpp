@@ -64,6 +64,14 @@ int fi_device_count(int* count)
     });
 }
 
+int fi_device_synchronize(int device)
+{
+    return guarded([&] {
+        DeviceGuard g(device);
+        FI_HIP(hipDeviceSynchronize());
+    });
+}
+
 int fi_last_error(char* buf, size_t len)
 {
     const std::string& m = last_error();

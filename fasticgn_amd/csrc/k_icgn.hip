@@ -325,10 +325,12 @@ __global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
                 st.add(t[k]);
             }
         }
-        if (wave_any(oob) || wave_any(nanc)) {
+        // ballots over the whole wave, taken before any lane-0 branch
+        const bool any_oob = wave_any(oob), any_nan = wave_any(nanc);
+        if (any_oob || any_nan) {
             // out of the interpolatable range takes precedence (:290-309)
             if (lane == 0) {
-                poi->result[2] = wave_any(oob) ? -3.f : -5.f;
+                poi->result[2] = any_oob ? -3.f : -5.f;
                 if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
             }
             return;

@@ -93,6 +93,8 @@ typedef struct fi_frame fi_frame;
 /* ---- library ------------------------------------------------------------ */
 int fi_version(int* major, int* minor, int* patch);
 int fi_device_count(int* count);
+/* hipDeviceSynchronize on the given device (timing brackets). */
+int fi_device_synchronize(int device);
 /* Copies the message of the calling thread's last failure (NUL-terminated,
  * truncated to len).  Returns the length of the full message. */
 int fi_last_error(char* buf, size_t len);
