@@ -83,6 +83,28 @@ def cpu_baseline(sc, res, sample_every: int):
     }, parity
 
 
+def copy_calibration(mib: int, repeats: int = 3) -> int:
+    """Launch the HIP runtime's device-to-device copy kernel on a buffer of
+    known size (reads mib MiB, writes mib MiB): the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run."""
+    import ctypes
+    import fasticgn_amd  # noqa: F401  (same HIP runtime as the library)
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = mib << 20
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(ctypes.byref(p), ctypes.c_size_t(n)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    for _ in range(repeats):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(n), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"copy_calibration_bytes": n, "repeats": repeats}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,7 +113,12 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-sample-every", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--shape", default="", help="IC-GN lane-to-POI shape WAVES,SPL")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
               "(the path shards over POIs; see DESIGN.md)", file=sys.stderr)
@@ -109,6 +136,8 @@ def main() -> int:
     drv = fa.FrameDriver(sc.radius, 0.001, 10)
     drv.set_reference(fa.Image2D(sc.ref))
     drv.set_pois(sc.pois)
+    if args.shape:
+        drv.set_shape(*[int(v) for v in args.shape.split(",")])
     t0 = time.perf_counter()
     drv.upload_target(fa.Image2D(sc.tar))
     upload_s = time.perf_counter() - t0
@@ -141,7 +170,7 @@ def main() -> int:
     achieved = alg_bytes / (icgn_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile("icgn_config1"),
-                "kernel": "k_icgn<16,true>", "kernel_ms": round(icgn_ms, 4),
+                "kernel": f"k_icgn<{args.shape or 'default'}>", "kernel_ms": round(icgn_ms, 4),
                 "algorithmic_bytes": int(alg_bytes)}
 
     cpu = None

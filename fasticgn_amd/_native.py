@@ -110,6 +110,7 @@ SIGNATURES = {
     "fi_icgn_destroy": [_P],
     "fi_icgn_set_iteration": [_P, _F, _F],
     "fi_icgn_set_self_adaptive": [_P, _I],
+    "fi_icgn_set_shape": [_P, _I, _I],
     "fi_icgn_set_ref": [_P, _P, _I, _I, _S, _I, _I],
     "fi_icgn_set_tar": [_P, _P, _I, _I, _S, _I, _I],
     "fi_icgn_prepare_ref": [_P],
@@ -124,6 +125,7 @@ SIGNATURES = {
     "fi_frame_run": [_P, _U, ctypes.POINTER(FrameStats)],
     "fi_frame_download": [_P, _P, _S],
     "fi_frame_reset_sequence": [_P],
+    "fi_frame_set_shape": [_P, _I, _I],
     "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
 }
 

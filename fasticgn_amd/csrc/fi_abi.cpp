@@ -177,6 +177,16 @@ int fi_icgn_set_self_adaptive(fi_icgn* h, int on)
     });
 }
 
+int fi_icgn_set_shape(fi_icgn* h, int waves, int spl)
+{
+    return guarded([&] {
+        need(h);
+        check_shape(waves, spl);
+        h->shape_waves = waves;
+        h->shape_spl = spl;
+    });
+}
+
 int fi_icgn_set_ref(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
 {
     return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
@@ -254,6 +264,16 @@ int fi_frame_download(fi_frame* h, fi_poi* out, size_t n)
 int fi_frame_reset_sequence(fi_frame* h)
 {
     return guarded([&] { need(h); h->reset_sequence(); });
+}
+
+int fi_frame_set_shape(fi_frame* h, int waves, int spl)
+{
+    return guarded([&] {
+        need(h);
+        check_shape(waves, spl);
+        h->shape_waves = waves;
+        h->shape_spl = spl;
+    });
 }
 
 // ---- raw record, opencorr_cli_cuda_mvp.cpp:2498-2536 ----

@@ -97,6 +97,15 @@ __device__ __forceinline__ float wave_sum_comp(CSum a)
     return tt + comp;
 }
 
+// Compensated wave sum of two per-lane partial sums: they are combined
+// exactly (TwoSum), then reduced across lanes with the TwoSum tree.
+__device__ __forceinline__ float wave_sum_pair(float a, float b)
+{
+    CSum c;
+    two_sum(a, b, c.s, c.c);
+    return wave_sum_comp(c);
+}
+
 __device__ __forceinline__ float wave_max(float v)
 {
     v = fmaxf(v, dpp_f<kDppQuadXor1>(v));

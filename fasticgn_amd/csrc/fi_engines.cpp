@@ -21,6 +21,20 @@ int safe_int(float v)
 
 std::string size_str(int w, int h) { return std::to_string(w) + "x" + std::to_string(h); }
 
+// IC-GN launch with an optional explicit lane-to-POI shape
+void icgn_launch(const IcgnParams& p, int waves, int spl, hipStream_t s)
+{
+    if (waves == 0) {
+        launch_icgn(p, s);
+        return;
+    }
+    const int ns = (2 * p.rx + 1) * (2 * p.ry + 1);
+    if (64 * waves * spl < ns)
+        fail(FI_E_INVALID_ARGUMENT, "IC-GN shape " + std::to_string(waves) + "x" + std::to_string(spl) +
+                                        " does not cover a subset of " + std::to_string(ns) + " samples");
+    launch_icgn_shape(p, waves, spl, s);
+}
+
 size_t max_lds_per_block(int device)
 {
     int v = 0;
@@ -36,6 +50,13 @@ hipStream_t make_stream()
 }
 
 }  // namespace
+
+void fi::check_shape(int waves, int spl)
+{
+    if (waves == 0 && spl == 0) return;
+    if (!icgn_shape_supported(waves, spl))
+        fail(FI_E_INVALID_ARGUMENT, "unsupported IC-GN shape " + std::to_string(waves) + "x" + std::to_string(spl));
+}
 
 // ===========================================================================
 // FFTCC2D
@@ -67,6 +88,10 @@ void fi_fftcc::set_ref(const void* px, int w, int h, size_t stride, int pixel, i
 {
     DeviceGuard g(device);
     ref.upload(px, w, h, stride, pixel, depth12, stream);
+    ref_f.reserve((size_t)w * h);
+    launch_to_float(ref.view, ref_f.p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
     ready = false;  // layout retained for a lazy rebuild (:188-195)
 }
 
@@ -74,6 +99,10 @@ void fi_fftcc::set_tar(const void* px, int w, int h, size_t stride, int pixel, i
 {
     DeviceGuard g(device);
     tar.upload(px, w, h, stride, pixel, depth12, stream);
+    tar_f.reserve((size_t)w * h);
+    launch_to_float(tar.view, tar_f.p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
 }
 
 void fi_fftcc::prepare_ref_clear()
@@ -89,8 +118,10 @@ void fi_fftcc::build_cache(const fi_poi* layout_dev, size_t n)
     spectra.reserve(std::max<size_t>(1, n * nh * 2));
     slots.reserve(std::max<size_t>(1, n));
     FftccParams p{};
-    p.ref = ref.view;
-    p.tar = ref.view;
+    p.ref = ref_f.p;
+    p.tar = ref_f.p;
+    p.w = ref.view.width;
+    p.h = ref.view.height;
     p.r = r;
     p.mode = kFftccPrepare;
     p.pois = const_cast<fi_poi*>(layout_dev);
@@ -175,8 +206,10 @@ void fi_fftcc::run(fi_poi* pois, size_t n, int mode, const uint32_t* slot_host)
         FI_HIP(hipMemcpyAsync(d_slot.p, slot_host, n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
     }
     FftccParams p{};
-    p.ref = ref.view;
-    p.tar = tar.view;
+    p.ref = ref_f.p;
+    p.tar = tar_f.p;
+    p.w = ref.view.width;
+    p.h = ref.view.height;
     p.r = r;
     p.mode = mode;
     p.pois = d_pois.p;
@@ -350,7 +383,7 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
         p.rx = rx;
         p.ry = ry;
         if (!prep_hit) launch_icgn_prep(p, stream);
-        launch_icgn(p, stream);
+        icgn_launch(p, shape_waves, shape_spl, stream);
         FI_HIP(hipGetLastError());
     } else {
         size_t total = 0;
@@ -373,7 +406,12 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
             p.index = d_index.p + spans[gi].first;
             p.n = (int)spans[gi].second;
             if (!prep_hit) launch_icgn_prep(p, stream);
-            launch_icgn(p, stream);
+            // an explicit shape applies only where it covers the group's subset
+            const int ns = (2 * p.rx + 1) * (2 * p.ry + 1);
+            if (shape_waves && 64 * shape_waves * shape_spl >= ns)
+                launch_icgn_shape(p, shape_waves, shape_spl, stream);
+            else
+                launch_icgn(p, stream);
             FI_HIP(hipGetLastError());
             gi++;
         }
@@ -487,8 +525,10 @@ void fi_frame::prepare()
     if (n > 0) {
         FI_HIP(hipMemcpyAsync(tmpl.p, tmpl_host.data(), n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
         FftccParams fp{};
-        fp.ref = ref_raw.view;
-        fp.tar = ref_raw.view;
+        fp.ref = ref_f.p;
+        fp.tar = ref_f.p;
+        fp.w = w;
+        fp.h = h;
         fp.r = r;
         fp.mode = kFftccPrepare;
         fp.pois = tmpl.p;
@@ -518,8 +558,10 @@ void fi_frame::prepare()
 void fi_frame::fftcc_pass(fi_poi* queue, int count, const uint32_t* slot_dev)
 {
     FftccParams fp{};
-    fp.ref = ref_raw.view;
-    fp.tar = tar.view;
+    fp.ref = ref_f.p;
+    fp.tar = tar_f.p;
+    fp.w = ref_raw.view.width;
+    fp.h = ref_raw.view.height;
     fp.r = r;
     fp.mode = kFftccCached;
     fp.pois = queue;
@@ -549,7 +591,7 @@ void fi_frame::icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index)
     ip.conv = conv;
     ip.stop = stop;
     ip.iter_total = iter_total.p;
-    launch_icgn(ip, stream);
+    icgn_launch(ip, shape_waves, shape_spl, stream);
 }
 
 void fi_frame::zero_counters()
@@ -580,7 +622,9 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
     FI_HIP(hipEventRecord(ev[0], stream));
     coef.reserve((size_t)16 * w * h);
+    tar_f.reserve((size_t)w * h);
     launch_bspline(tar.view, coef.p, stream);
+    launch_to_float(tar.view, tar_f.p, stream);
     FI_HIP(hipEventRecord(ev[1], stream));
 
     const bool skip = (policy & FI_POLICY_SKIP_FFTCC) && baseline_ready && have_prev;

@@ -11,11 +11,17 @@
 
 #include "fi_host.h"
 
+namespace fi {
+// Validates an explicit IC-GN lane-to-POI shape (0, 0 = default).
+void check_shape(int waves, int spl);
+}
+
 struct fi_fftcc {
     int device = 0;
     int r = 0;
     hipStream_t stream = nullptr;
     fi::DeviceImage ref, tar;
+    fi::DevArray<float> ref_f, tar_f;  // f32 copies the kernels read
     // prepared reference layout (prepared_ref_, src/oc_fftcc.h:63-70)
     std::vector<int32_t> lay_x, lay_y;
     bool ready = false;
@@ -48,6 +54,7 @@ struct fi_icgn {
     int rx = 0, ry = 0;
     float conv = 0.001f, stop = 10.f;
     bool self_adaptive = false;
+    int shape_waves = 0, shape_spl = 0;
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw, tar_raw;
     fi::DevArray<float> ref_f, gx, gy, coef;
@@ -74,9 +81,10 @@ struct fi_frame {
     int device = 0;
     int r = 0;
     float conv = 0.001f, stop = 10.f;
+    int shape_waves = 0, shape_spl = 0;
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw, tar;
-    fi::DevArray<float> ref_f, gx, gy, coef, tw, spectra;
+    fi::DevArray<float> ref_f, gx, gy, coef, tw, spectra, tar_f;
     fi::DevArray<fi::FftccSlot> slots;
     fi::DevArray<fi::PoiPrep> prep;
     std::vector<fi_poi> tmpl_host;

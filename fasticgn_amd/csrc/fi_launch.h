@@ -11,7 +11,7 @@ void launch_to_float(const DevImage& img, float* out, hipStream_t s);
 void launch_gradient(const float* img, int w, int h, float* gx, float* gy, hipStream_t s);
 void launch_bspline(const DevImage& img, float* coef, hipStream_t s);
 
-// Dynamic LDS bytes the FFTCC kernel needs for radius r in the given mode.
+// LDS bytes of one FFTCC block for radius r (its static allocation).
 size_t fftcc_lds_bytes(int r, int mode);
 void launch_fftcc(const FftccParams& p, hipStream_t s);
 
@@ -19,6 +19,10 @@ void launch_fftcc(const FftccParams& p, hipStream_t s);
 constexpr int kIcgnMaxRadius = 31;
 void launch_icgn_prep(const IcgnParams& p, hipStream_t s);
 void launch_icgn(const IcgnParams& p, hipStream_t s);
+// Lane-to-POI shape: waves per POI and samples per lane (waves*64*spl >= ns).
+bool icgn_shape_supported(int waves, int spl);
+void icgn_default_shape(int ns, int& waves, int& spl);
+void launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s);
 
 // Frame-policy helpers (k_policy.hip).
 struct FrameCounters {

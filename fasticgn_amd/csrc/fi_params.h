@@ -42,7 +42,9 @@ struct FftccSlot {
 enum FftccMode { kFftccPrepare = 0, kFftccCached = 1, kFftccUncached = 2 };
 
 struct FftccParams {
-    DevImage ref, tar;
+    const float* ref;         // reference, f32 row-major, pitch = w
+    const float* tar;         // target, f32 row-major, pitch = w
+    int w, h;
     int r;                    // radius; window n = 2r
     int mode;
     fi_poi* pois;             // queue (prepare: the layout, read only)

@@ -213,13 +213,9 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
 
 // Bicubic B-spline sample from the 16-float texel (coef[i][j] multiplies
 // dy^i dx^j, src/oc_cubic_bspline.cpp:154-177), Horner in dx then dy.
-__device__ __forceinline__ float bspline_sample(const float* __restrict__ coef, int w, float gx,
-                                                float gy)
+__device__ __forceinline__ float bspline_eval(const float4 c0, const float4 c1, const float4 c2,
+                                              const float4 c3, float dx, float dy)
 {
-    const float fx = floorf(gx), fy = floorf(gy);
-    const float dx = gx - fx, dy = gy - fy;
-    const float4* c = reinterpret_cast<const float4*>(coef) + 4 * ((size_t)(int)fy * w + (int)fx);
-    const float4 c0 = c[0], c1 = c[1], c2 = c[2], c3 = c[3];
     const float r0 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c0.w, dx, c0.z), dx, c0.y), dx, c0.x);
     const float r1 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c1.w, dx, c1.z), dx, c1.y), dx, c1.x);
     const float r2 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c2.w, dx, c2.z), dx, c2.y), dx, c2.x);
@@ -227,13 +223,106 @@ __device__ __forceinline__ float bspline_sample(const float* __restrict__ coef, 
     return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(r3, dy, r2), dy, r1), dy, r0);
 }
 
-// ---------------------------------------------------------------------------
-// IC-GN iteration.
-// ---------------------------------------------------------------------------
-template <int SPL, bool CACHE>
-__global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
+// The coefficient table is read through a raw buffer resource: one 32-bit
+// byte offset per sample, and the hardware range check returns zeros for any
+// offset past the table, so a sample outside the image (whose value is
+// discarded) needs no clamp and can never fault.
+constexpr uint32_t kBufferWord3 = 0x00020000;  // gfx9-family raw buffer, dword data
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const float* coef, uint32_t bytes)
 {
-    const int lane = threadIdx.x;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(coef), (short)0, (int)bytes, (int)kBufferWord3);
+}
+
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+}
+
+// The table is four planes of float4 (plane k = coef[k][0..3] of every texel);
+// plane_bytes = 16 * w * h.  An offset past a plane's end reads the next plane
+// or, past the table, zeros: either way a value that is discarded.
+__device__ __forceinline__ float bspline_sample(__amdgpu_buffer_rsrc_t r, int w, uint32_t plane_bytes,
+                                                float gx, float gy)
+{
+    const float fx = floorf(gx), fy = floorf(gy);
+    const float dx = gx - fx, dy = gy - fy;
+    const uint32_t off = ((uint32_t)((int)fy * w + (int)fx)) << 4;
+    return bspline_eval(ld4(r, off, 0), ld4(r, off, plane_bytes), ld4(r, off, 2 * plane_bytes),
+                        ld4(r, off, 3 * plane_bytes), dx, dy);
+}
+
+// ---------------------------------------------------------------------------
+// IC-GN iteration.  One block of WAVES wavefronts per POI; thread t owns
+// subset samples t, t + 64*WAVES, ... (SPL of them), with the reference value,
+// gradients and local coordinates held in VGPRs for the whole solve.  Sums are
+// per-lane FMA chains, a compensated (TwoSum) DPP tree inside each wave and,
+// for WAVES > 1, a compensated combine of the per-wave results through LDS in a
+// fixed order, so every wave holds bit-identical totals and takes the same
+// branches.
+// ---------------------------------------------------------------------------
+
+// Minimum resident waves per SIMD the register allocation must allow (the
+// compiler otherwise trades occupancy for deeper load pipelining).
+#ifndef FI_ICGN_MIN_WAVES
+#define FI_ICGN_MIN_WAVES 3
+#endif
+
+template <int WAVES>
+struct BlockSum {
+    // per wave: NV compensated pairs
+    static constexpr int NV = 7;
+    float* lds;
+    __device__ __forceinline__ void reduce(float* vals, int nv, int wave, int lane)
+    {
+        // vals[i] arrive as per-lane partial sums
+        CSum part[NV];
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            if (i < nv) {
+                // wave_sum_comp returns s + c; keep both for the cross-wave step
+                CSum c;
+                c.s = vals[i];
+                c.c = 0.f;
+                part[i].s = wave_sum_comp(c);
+                part[i].c = 0.f;
+            }
+        }
+        if (WAVES == 1) {
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                if (i < nv) vals[i] = part[i].s;
+            return;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; i++)
+                if (i < nv) lds[wave * NV + i] = part[i].s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; i++) {
+            if (i < nv) {
+                CSum acc;
+#pragma unroll
+                for (int w = 0; w < WAVES; w++) acc.add(lds[w * NV + i]);
+                vals[i] = acc.s + acc.c;
+            }
+        }
+        __syncthreads();
+    }
+};
+
+template <int WAVES, int SPL>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
+void k_icgn(IcgnParams p)
+{
+    constexpr int NT = 64 * WAVES;
+    __shared__ float red[WAVES > 1 ? WAVES * BlockSum<WAVES>::NV : 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    BlockSum<WAVES> bs{ red };
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= p.n) return;
     const int i = p.index ? (int)p.index[b] : b;
@@ -251,7 +340,7 @@ __global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
     // entry gate, src/oc_icgn.cpp:184-191
     if (!ref_gate_ok(x, y, rx, ry, w, h) || fabsf(u_in) >= (float)w || fabsf(v_in) >= (float)h
         || z_in < 0.f || isnan(u_in) || isnan(v_in)) {
-        if (lane == 0) poi->result[2] = (z_in >= 0.f) ? -3.f : z_in;
+        if (tid == 0) poi->result[2] = (z_in >= 0.f) ? -3.f : z_in;
         return;
     }
     const PoiPrep* pr = p.prep + (p.prep_index ? p.prep_index[i] : (uint32_t)i);
@@ -264,26 +353,26 @@ __global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
     const int ulx = (int)(x - rx), uly = (int)(y - ry);
     const int px = (int)x, py = (int)y;
 
-    // per-sample reference data
-    float rz[CACHE ? SPL : 1], gxv[CACHE ? SPL : 1], gyv[CACHE ? SPL : 1];
-    float xlv[CACHE ? SPL : 1], ylv[CACHE ? SPL : 1];
-    if (CACHE) {
+    // Slots past the subset (s >= ns) sample the centre (xl = yl = 0) with zero
+    // reference data and are masked out of the sums: no per-slot branch.
+    float rz[SPL], gxv[SPL], gyv[SPL], xlv[SPL], ylv[SPL];
 #pragma unroll
-        for (int k = 0; k < SPL; k++) {
-            const int s = lane + 64 * k;
-            rz[k] = gxv[k] = gyv[k] = xlv[k] = ylv[k] = 0.f;
-            if (s < g.ns) {
-                int c, r;
-                sample_coords(g, s, c, r);
-                rz[k] = p.ref[(size_t)(uly + r) * w + (ulx + c)] - ref_mean;
-                const size_t gi = (size_t)(py + r - ry) * w + (px + c - rx);
-                gxv[k] = p.gx[gi];
-                gyv[k] = p.gy[gi];
-                xlv[k] = (float)(c - rx);
-                ylv[k] = (float)(r - ry);
-            }
+    for (int k = 0; k < SPL; k++) {
+        const int s = tid + NT * k;
+        rz[k] = gxv[k] = gyv[k] = xlv[k] = ylv[k] = 0.f;
+        if (s < g.ns) {
+            int c, r;
+            sample_coords(g, s, c, r);
+            rz[k] = p.ref[(size_t)(uly + r) * w + (ulx + c)] - ref_mean;
+            const size_t gi = (size_t)(py + r - ry) * w + (px + c - rx);
+            gxv[k] = p.gx[gi];
+            gyv[k] = p.gy[gi];
+            xlv[k] = (float)(c - rx);
+            ylv[k] = (float)(r - ry);
         }
     }
+    const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)((size_t)w * h * 64));
+    const uint32_t plane_bytes = (uint32_t)((size_t)w * h * 16);
 
     // warp matrix, Deformation2D1::setWarp (src/oc_deformation.cpp:117-128)
     float W00 = 1.f + ux_in, W01 = uy_in, W02 = u_in;
@@ -298,96 +387,95 @@ __global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
     float conv = 0.f, znssd = 0.f;
     do {
         it++;
-        bool oob = false, nanc = false;
-        CSum st;
+        // pass 1: warp, range test, B-spline sample, sum of t (two
+        // interleaved per-lane accumulators keep the FMA chains short)
+        bool outside = false;
+        float sa = 0.f, sb = 0.f;
 #pragma unroll
         for (int k = 0; k < SPL; k++) {
-            const int s = lane + 64 * k;
-            t[k] = 0.f;
-            if (s < g.ns) {
-                float xl, yl;
-                if (CACHE) {
-                    xl = xlv[k];
-                    yl = ylv[k];
-                } else {
-                    int c, r;
-                    sample_coords(g, s, c, r);
-                    xl = (float)(c - rx);
-                    yl = (float)(r - ry);
-                }
-                const float gxp = x + __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl, W02));
-                const float gyp = y + __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl, W12));
-                const bool o = (gxp < 1.f) || (gyp < 1.f) || (gxp >= wm2) || (gyp >= hm2);
-                const bool nn = isnan(gxp) || isnan(gyp);
-                oob |= o;
-                nanc |= nn;
-                if (!(o || nn)) t[k] = bspline_sample(p.coef, w, gxp, gyp);
-                st.add(t[k]);
-            }
+            const bool slot = tid + NT * k < g.ns;
+            const float gxp = x + __builtin_fmaf(W00, xlv[k], __builtin_fmaf(W01, ylv[k], W02));
+            const float gyp = y + __builtin_fmaf(W10, xlv[k], __builtin_fmaf(W11, ylv[k], W12));
+            // false for a NaN coordinate as well as for an out-of-range one
+            const bool in = (gxp >= 1.f) & (gyp >= 1.f) & (gxp < wm2) & (gyp < hm2);
+            outside |= !in;
+            const float v = bspline_sample(table, w, plane_bytes, gxp, gyp);
+            t[k] = (in & slot) ? v : 0.f;
+            if (k & 1) sb += t[k]; else sa += t[k];
         }
-        // ballots over the whole wave, taken before any lane-0 branch
-        const bool any_oob = wave_any(oob), any_nan = wave_any(nanc);
-        if (any_oob || any_nan) {
-            // out of the interpolatable range takes precedence (:290-309)
-            if (lane == 0) {
-                poi->result[2] = any_oob ? -3.f : -5.f;
+        // first reduction: the sum of t and the number of lanes holding an
+        // unusable sample (a count, so the compensated sum is exact)
+        float v2[2];
+        {
+            float a, e;
+            two_sum(sa, sb, a, e);
+            v2[0] = a + e;
+            v2[1] = outside ? 1.f : 0.f;
+        }
+        bs.reduce(v2, 2, wave, lane);
+        if (v2[1] > 0.f) {
+            // Some sample is unusable: out of the interpolatable range (-3,
+            // :39-43) takes precedence over a NaN coordinate (-5), :290-309.
+            bool oob = false;
+#pragma unroll
+            for (int k = 0; k < SPL; k++) {
+                const float gxp = x + __builtin_fmaf(W00, xlv[k], __builtin_fmaf(W01, ylv[k], W02));
+                const float gyp = y + __builtin_fmaf(W10, xlv[k], __builtin_fmaf(W11, ylv[k], W12));
+                oob |= (gxp < 1.f) || (gyp < 1.f) || (gxp >= wm2) || (gyp >= hm2);
+            }
+            float v1[1] = { oob ? 1.f : 0.f };
+            bs.reduce(v1, 1, wave, lane);
+            if (tid == 0) {
+                poi->result[2] = (v1[0] > 0.f) ? -3.f : -5.f;
                 if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
             }
             return;
         }
-        const float tar_mean = wave_sum_comp(st) / (float)g.ns;
-        CSum sq;
-#pragma unroll
-        for (int k = 0; k < SPL; k++)
-            if (lane + 64 * k < g.ns) {
-                t[k] -= tar_mean;
-                sq.add(t[k] * t[k]);
-            }
-        const float tar_norm = sqrtf(wave_sum_comp(sq));
-        const float scale = ref_norm / tar_norm;
-
-        CSum se, sb0, sb1, sb2, sb3, sb4, sb5;
+        float v1[1] = { v2[0] };
+        const float tar_mean = v1[0] / (float)g.ns;
+        // pass 2: centre, sum of squares (Subset2D::zeroMeanNorm, two-pass)
+        float qa = 0.f, qb = 0.f;
 #pragma unroll
         for (int k = 0; k < SPL; k++) {
-            const int s = lane + 64 * k;
-            if (s < g.ns) {
-                float r_k, gx_k, gy_k, xl, yl;
-                if (CACHE) {
-                    r_k = rz[k]; gx_k = gxv[k]; gy_k = gyv[k]; xl = xlv[k]; yl = ylv[k];
-                } else {
-                    int c, r;
-                    sample_coords(g, s, c, r);
-                    r_k = p.ref[(size_t)(uly + r) * w + (ulx + c)] - ref_mean;
-                    const size_t gi = (size_t)(py + r - ry) * w + (px + c - rx);
-                    gx_k = p.gx[gi];
-                    gy_k = p.gy[gi];
-                    xl = (float)(c - rx);
-                    yl = (float)(r - ry);
-                }
-                const float e = __builtin_fmaf(t[k], scale, -r_k);
-                se.add(e * e);
-                const float ge = gx_k * e, he = gy_k * e;
-                sb0.add(ge);
-                sb1.add(ge * xl);
-                sb2.add(ge * yl);
-                sb3.add(he);
-                sb4.add(he * xl);
-                sb5.add(he * yl);
-            }
+            t[k] = (tid + NT * k < g.ns) ? t[k] - tar_mean : 0.f;
+            if (k & 1) qb = __builtin_fmaf(t[k], t[k], qb); else qa = __builtin_fmaf(t[k], t[k], qa);
         }
-        const float esq = wave_sum_comp(se);
-        const float bnum[6] = { wave_sum_comp(sb0), wave_sum_comp(sb1), wave_sum_comp(sb2),
-                                wave_sum_comp(sb3), wave_sum_comp(sb4), wave_sum_comp(sb5) };
-        znssd = esq / (ref_norm * ref_norm);
+        {
+            float a, e;
+            two_sum(qa, qb, a, e);
+            v1[0] = a + e;
+        }
+        bs.reduce(v1, 1, wave, lane);
+        const float tar_norm = sqrtf(v1[0]);
+        const float scale = ref_norm / tar_norm;
+
+        // pass 3: error e = t * (nr / nt) - r, ZNSSD numerator and the six
+        // steepest-descent products (:311-322); padding slots give e = 0
+        float acc[7] = { 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+        for (int k = 0; k < SPL; k++) {
+            const float e = __builtin_fmaf(t[k], scale, -rz[k]);
+            acc[0] = __builtin_fmaf(e, e, acc[0]);
+            const float ge = gxv[k] * e, he = gyv[k] * e;
+            acc[1] += ge;
+            acc[2] = __builtin_fmaf(ge, xlv[k], acc[2]);
+            acc[3] = __builtin_fmaf(ge, ylv[k], acc[3]);
+            acc[4] += he;
+            acc[5] = __builtin_fmaf(he, xlv[k], acc[5]);
+            acc[6] = __builtin_fmaf(he, ylv[k], acc[6]);
+        }
+        bs.reduce(acc, 7, wave, lane);
+        znssd = acc[0] / (ref_norm * ref_norm);
+        const float* bnum = acc + 1;
 
         // dp = H^-1 b (src/oc_icgn.cpp:324-331)
         float dp[6];
 #pragma unroll
         for (int a = 0; a < 6; a++) {
-            float acc = 0.f;
+            float d = 0.f;
 #pragma unroll
-            for (int c = 0; c < 6; c++) acc = __builtin_fmaf(hinv[a * 6 + c], bnum[c], acc);
-            dp[a] = acc;
+            for (int c = 0; c < 6; c++) d = __builtin_fmaf(hinv[a * 6 + c], bnum[c], d);
+            dp[a] = d;
         }
         // W <- W * W(dp)^-1 (:333-340); the 3x3 inverse by cofactors down column 0
         const float D00 = 1.f + dp[1], D01 = dp[2], D02 = dp[0];
@@ -418,7 +506,7 @@ __global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
         conv = (float)sqrt(cn);
     } while ((float)it < stop && conv >= conv_crit);
 
-    if (lane == 0) {
+    if (tid == 0) {
         if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
         float u = W02, ux = W00 - 1.f, uy = W01, v = W12, vx = W10, vy = W11 - 1.f;
         float zncc = 0.5f * (2.f - znssd);
@@ -444,12 +532,30 @@ __global__ __launch_bounds__(64) void k_icgn(IcgnParams p)
     }
 }
 
-// samples per lane for a subset of ns samples
-static int spl_for(int ns) { return (ns + 63) / 64; }
+// Lane-to-POI shapes: (waves per POI, samples per lane).  The default picks
+// the fewest waves that keep SPL <= 16; an explicit shape (autotuner) must
+// cover the subset.
+#define FI_ICGN_SHAPES(X)                                                        \
+    X(1, 1) X(1, 2) X(1, 4) X(1, 7) X(1, 10) X(1, 16) X(2, 8) X(2, 14) X(2, 16)   \
+    X(4, 4) X(4, 7) X(4, 10) X(4, 16)
 
-#define FI_ICGN_CASES(X)                                                           \
-    X(1, true) X(2, true) X(4, true) X(7, true) X(10, true) X(16, true) X(20, true) \
-    X(27, true) X(36, false) X(48, false) X(63, false)
+bool icgn_shape_supported(int waves, int spl)
+{
+#define X(W, S) if (waves == W && spl == S) return true;
+    FI_ICGN_SHAPES(X)
+#undef X
+    return false;
+}
+
+void icgn_default_shape(int ns, int& waves, int& spl)
+{
+    const int cand[][2] = { { 1, 1 }, { 1, 2 }, { 1, 4 }, { 1, 7 }, { 1, 10 }, { 1, 16 },
+                            { 2, 14 }, { 2, 16 }, { 4, 10 }, { 4, 16 } };
+    for (auto& c : cand)
+        if (64 * c[0] * c[1] >= ns) { waves = c[0]; spl = c[1]; return; }
+    waves = 4;
+    spl = 16;
+}
 
 void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 {
@@ -457,18 +563,24 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
     hipLaunchKernelGGL(k_icgn_prep, dim3(p.n), dim3(64), 0, s, p);
 }
 
-void launch_icgn(const IcgnParams& p, hipStream_t s)
+void launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s)
 {
     if (p.n <= 0) return;
-    const int spl = spl_for((2 * p.rx + 1) * (2 * p.ry + 1));
-    const dim3 grid(p.n), block(64);
-#define X(S, C)                                                            \
-    if (spl <= S) {                                                        \
-        hipLaunchKernelGGL((k_icgn<S, C>), grid, block, 0, s, p);          \
-        return;                                                            \
+    const dim3 grid(p.n);
+#define X(W, S)                                                                 \
+    if (waves == W && spl == S) {                                               \
+        hipLaunchKernelGGL((k_icgn<W, S>), grid, dim3(64 * W), 0, s, p);        \
+        return;                                                                 \
     }
-    FI_ICGN_CASES(X)
+    FI_ICGN_SHAPES(X)
 #undef X
+}
+
+void launch_icgn(const IcgnParams& p, hipStream_t s)
+{
+    int waves, spl;
+    icgn_default_shape((2 * p.rx + 1) * (2 * p.ry + 1), waves, spl);
+    launch_icgn_shape(p, waves, spl, s);
 }
 
 }  // namespace fi

@@ -65,9 +65,11 @@ __device__ __forceinline__ float bc(int i, int j)
 }
 
 // BicubicBspline::prepare, src/oc_cubic_bspline.cpp:84-132.  One thread per
-// texel; texels outside 1 <= r < h-2, 1 <= c < w-2 are zero.  Output: 16
-// floats per texel, coef[k][l] = P[3-k][3-l],
-// P[k][l] = sum_m sum_n (BC[l][m] * BC[k][n]) * Q[n][m] in (m, n) order.
+// texel; texels outside 1 <= r < h-2, 1 <= c < w-2 are zero.  Per texel the 16
+// values coef[k][l] = P[3-k][3-l], P[k][l] = sum_m sum_n (BC[l][m] * BC[k][n]) *
+// Q[n][m] in (m, n) order.  Stored as four planes: plane k holds the float4
+// coef[k][0..3] of every texel, row-major, so a wavefront reading one plane for
+// consecutive texels reads one contiguous run.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bspline(const T* __restrict__ raw, size_t pitch, int w,
                                                  int h, float scale, float* __restrict__ coef)
@@ -75,10 +77,11 @@ __global__ __launch_bounds__(256) void k_bspline(const T* __restrict__ raw, size
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (c >= w || r >= h) return;
-    float4* dst = reinterpret_cast<float4*>(coef + 16 * ((size_t)r * w + c));
+    const size_t plane = (size_t)w * h;
+    float4* dst = reinterpret_cast<float4*>(coef) + ((size_t)r * w + c);
     if (r < 1 || r >= h - 2 || c < 1 || c >= w - 2) {
         const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-        dst[0] = z; dst[1] = z; dst[2] = z; dst[3] = z;
+        dst[0] = z; dst[plane] = z; dst[2 * plane] = z; dst[3 * plane] = z;
         return;
     }
     float q[4][4];
@@ -100,9 +103,9 @@ __global__ __launch_bounds__(256) void k_bspline(const T* __restrict__ raw, size
             p[k][l] = acc;
         }
     dst[0] = make_float4(p[3][3], p[3][2], p[3][1], p[3][0]);
-    dst[1] = make_float4(p[2][3], p[2][2], p[2][1], p[2][0]);
-    dst[2] = make_float4(p[1][3], p[1][2], p[1][1], p[1][0]);
-    dst[3] = make_float4(p[0][3], p[0][2], p[0][1], p[0][0]);
+    dst[plane] = make_float4(p[2][3], p[2][2], p[2][1], p[2][0]);
+    dst[2 * plane] = make_float4(p[1][3], p[1][2], p[1][1], p[1][0]);
+    dst[3 * plane] = make_float4(p[0][3], p[0][2], p[0][1], p[0][0]);
 }
 
 static dim3 tile_grid(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }

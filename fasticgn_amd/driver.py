@@ -53,6 +53,10 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))
         return out
 
+    def set_shape(self, waves: int, samples_per_lane: int) -> None:
+        """IC-GN lane-to-POI shape (0, 0 = default)."""
+        nat.call("fi_frame_set_shape", self._h, int(waves), int(samples_per_lane))
+
     def reset_sequence(self) -> None:
         nat.call("fi_frame_reset_sequence", self._h)
 

@@ -103,30 +103,52 @@ int or_image_to_float(const void* px, int pixel, int width, int height,
 }
 
 /* ---------------------------------------------------------------------------
- * FFTCC DFT specification.
+ * FFTCC transform specification.
  *
- * FFTW's r2c_2d(n0 = n, n1 = n) on the row-major window window[y*n + x] keeps
- * the half spectrum along x: X[ky][kx], ky < n, kx <= n/2.  The c2r inverse is
- * unnormalised.  The restatement computes both as separable DFTs with one
- * twiddle table and a fixed operation order, so that any implementation that
- * performs the same float operations in the same order reproduces every bit:
+ * FFTW's r2c_2d(n0 = n, n1 = n) on the row-major window w[y*n + x] keeps the
+ * half spectrum along x: X[ky][kx], ky < n, kx <= n/2; its c2r inverse is
+ * unnormalised and ignores the imaginary parts of the kx = 0 and kx = n/2
+ * bins.  FFTW's own rounding is not reproducible, so this restatement fixes one
+ * algorithm, written out to the individual float operation; any
+ * implementation performing the same operations in the same order (the gfx950
+ * kernel does) reproduces every bit.
  *
- *   forward, along x:  A[y][kx]  = sum_{x asc} w[y][x] * tw[(x*kx) % n]
- *                        re = fma(v, wr, re); im = fma(v, wi, im)
- *   forward, along y:  X[ky][kx] = sum_{y asc} A[y][kx] * tw[(y*ky) % n]
- *                        re = fma(ar, wr, re); re = fma(-ai, wi, re);
- *                        im = fma(ar, wi, im); im = fma(ai, wr, im)
- *   product:           Z = conj(R) * T = (Rr*Tr + Ri*Ti, Rr*Ti - Ri*Tr), unfused
- *   inverse, along y:  G[y][kx]  = sum_{ky asc} Z[ky][kx] * conj(tw[(y*ky) % n])
- *                        re = fma(zr, wr, re); re = fma(zi, wi, re);
- *                        im = fma(zi, wr, im); im = fma(-zr, wi, im)
- *   inverse, along x:  out[y][x] = G[y][0].re
- *                        + sum_{kx=1..n/2-1} [s = gi*wi; s = fma(gr, wr, s);
- *                                             acc = fma(2, s, acc)]
- *                        + (x odd ? -1 : 1) * G[y][n/2].re
- * Each sum starts from 0.0f and fma is the correctly rounded fused
- * multiply-add (C99 fmaf).  Imaginary parts of the kx = 0 and kx = n/2 bins
- * are ignored by the inverse, as FFTW's c2r does.
+ * fft(n): length-n complex DFT, X[k] = sum_j x[j] e^{-2 pi i jk/n}, by
+ * recursive decimation in time.  Base cases n = 1..5 are the butterflies
+ * below.  Otherwise p = 4 if 4 | n, else 2 if 2 | n, else 3 if 3 | n, else 5
+ * if 5 | n, else n (a prime > 5: direct sum).  With m = n/p:
+ *   Y_r = fft(m) of x[r], x[r+p], ...              (r = 0..p-1)
+ *   T_r = Y_r[k] if r*k == 0 else cmul(Y_r[k], tw[r*k*step])
+ *   X[k + m*t], t = 0..p-1 = butterfly_p(T_0..T_{p-1})   (k = 0..m-1)
+ * tw[j] = (cos 2pi j/N, -sin 2pi j/N) rounded from double, N the top-level
+ * size, step = N/n.  cmul(a, w) = (fma(ax, wx, -(ay*wy)), fma(ax, wy, ay*wx)).
+ * Butterflies (componentwise on (re, im)):
+ *   2: X0 = a + b; X1 = a - b
+ *   3: s1 = b + c; s2 = b - c; X0 = a + s1; m = a - 0.5*s1; q = K3*s2;
+ *      X1 = (m.x + q.y, m.y - q.x); X2 = (m.x - q.y, m.y + q.x)
+ *   4: s0 = a + c; s1 = a - c; s2 = b + d; s3 = b - d; X0 = s0 + s2;
+ *      X2 = s0 - s2; X1 = (s1.x + s3.y, s1.y - s3.x); X3 = (s1.x - s3.y, s1.y + s3.x)
+ *   5: t1 = x1 + x4; t2 = x2 + x3; t3 = x1 - x4; t4 = x2 - x3;
+ *      X0 = x0 + (t1 + t2); a1 = x0 + fma(C1, t1, C2*t2); a2 = x0 + fma(C2, t1, C1*t2);
+ *      b1 = fma(S1, t3, S2*t4); b2 = fma(S2, t3, -(S1*t4));
+ *      X1 = (a1.x + b1.y, a1.y - b1.x); X4 = (a1.x - b1.y, a1.y + b1.x);
+ *      X2 = (a2.x + b2.y, a2.y - b2.x); X3 = (a2.x - b2.y, a2.y + b2.x)
+ *   prime p > 5: X[t] = x[0] + sum_{r=1}^{p-1} (j = r*t mod p; j == 0 ? x[r] : cmul(x[r], tw[j*step]))
+ *   with K3 = sin(2pi/3), C1 = cos(2pi/5), C2 = cos(4pi/5), S1 = sin(2pi/5),
+ *   S2 = sin(4pi/5) as float literals.
+ * ifft(z) = conj(fft(conj(z))), unnormalised.
+ *
+ * 2-D forward (real window w, n even): rows in pairs (2j, 2j+1):
+ *   z[x] = (w[2j][x], w[2j+1][x]); Z = fft(z); for kx <= n/2, nk = (n - kx) % n:
+ *   A[kx] = (0.5*(Z[kx].x + Z[nk].x), 0.5*(Z[kx].y - Z[nk].y))  -> row 2j
+ *   B[kx] = (0.5*(Z[kx].y + Z[nk].y), 0.5*(Z[nk].x - Z[kx].x))  -> row 2j+1
+ * then fft along y of every column kx.
+ * Cross power: Z = conj(R) * T = (Rr*Tr + Ri*Ti, Rr*Ti - Ri*Tr), unfused.
+ * 2-D inverse: ifft along y of every column, then rows in pairs: the
+ * imaginary parts of kx = 0 and kx = n/2 of both rows are zeroed, then for
+ * k <= n/2: Zr[k] = (A.x - B.y, A.y + B.x); for k > n/2, j = n - k:
+ * Zr[k] = (A[j].x + B[j].y, B[j].x - A[j].y); z = ifft(Zr); row 2j = z.x,
+ * row 2j+1 = z.y.
  *
  * Window mean and squared norm use row partial sums: for each row the values
  * are summed left to right from 0.0f, and the row sums are added top to
@@ -142,70 +164,182 @@ void or_dft_twiddles(int n, float* tw)
     }
 }
 
-static void dft_r2c_2d(const float* x, int n, const float* tw, float* tmp, float* X)
+typedef struct cpx { float x, y; } cpx;
+
+static const float K3 = 0.866025403784438646763723170753f;
+static const float C51 = 0.309016994374947424102293417183f;
+static const float C52 = -0.809016994374947424102293417183f;
+static const float S51 = 0.951056516295153572116439333379f;
+static const float S52 = 0.587785252292473129168705954639f;
+
+static cpx cadd(cpx a, cpx b) { cpx r = { a.x + b.x, a.y + b.y }; return r; }
+static cpx csub(cpx a, cpx b) { cpx r = { a.x - b.x, a.y - b.y }; return r; }
+static cpx cmul(cpx a, cpx w)
 {
-    const int hc = n / 2 + 1;
-    for (int y = 0; y < n; y++)
-        for (int kx = 0; kx < hc; kx++) {
-            float re = 0.f, im = 0.f;
-            for (int xx = 0; xx < n; xx++) {
-                int t = (xx * kx) % n;
-                float v = x[y * n + xx];
-                re = fmaf(v, tw[2 * t], re);
-                im = fmaf(v, tw[2 * t + 1], im);
-            }
-            tmp[2 * (y * hc + kx)] = re;
-            tmp[2 * (y * hc + kx) + 1] = im;
-        }
-    for (int kx = 0; kx < hc; kx++)
-        for (int ky = 0; ky < n; ky++) {
-            float re = 0.f, im = 0.f;
-            for (int y = 0; y < n; y++) {
-                int t = (y * ky) % n;
-                float ar = tmp[2 * (y * hc + kx)], ai = tmp[2 * (y * hc + kx) + 1];
-                float wr = tw[2 * t], wi = tw[2 * t + 1];
-                re = fmaf(ar, wr, re);
-                re = fmaf(-ai, wi, re);
-                im = fmaf(ar, wi, im);
-                im = fmaf(ai, wr, im);
-            }
-            X[2 * (ky * hc + kx)] = re;
-            X[2 * (ky * hc + kx) + 1] = im;
-        }
+    cpx r;
+    r.x = fmaf(a.x, w.x, -(a.y * w.y));
+    r.y = fmaf(a.x, w.y, a.y * w.x);
+    return r;
+}
+static cpx cconj(cpx a) { cpx r = { a.x, -a.y }; return r; }
+
+static int fft_radix(int n)
+{
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    if (n % 3 == 0) return 3;
+    if (n % 5 == 0) return 5;
+    return n;
 }
 
-static void dft_c2r_2d(const float* Z, int n, const float* tw, float* tmp, float* out)
+static void butterfly(int p, const cpx* T, cpx* X, int m, const cpx* tw, int step)
+{
+    /* X[m*t] for t < p from T[0..p-1] */
+    if (p == 2) {
+        X[0] = cadd(T[0], T[1]);
+        X[m] = csub(T[0], T[1]);
+    } else if (p == 3) {
+        cpx s1 = cadd(T[1], T[2]), s2 = csub(T[1], T[2]);
+        X[0] = cadd(T[0], s1);
+        cpx mm = { T[0].x - 0.5f * s1.x, T[0].y - 0.5f * s1.y };
+        cpx q = { K3 * s2.x, K3 * s2.y };
+        cpx x1 = { mm.x + q.y, mm.y - q.x }, x2 = { mm.x - q.y, mm.y + q.x };
+        X[m] = x1;
+        X[2 * m] = x2;
+    } else if (p == 4) {
+        cpx s0 = cadd(T[0], T[2]), s1 = csub(T[0], T[2]), s2 = cadd(T[1], T[3]), s3 = csub(T[1], T[3]);
+        X[0] = cadd(s0, s2);
+        X[2 * m] = csub(s0, s2);
+        cpx x1 = { s1.x + s3.y, s1.y - s3.x }, x3 = { s1.x - s3.y, s1.y + s3.x };
+        X[m] = x1;
+        X[3 * m] = x3;
+    } else if (p == 5) {
+        cpx t1 = cadd(T[1], T[4]), t2 = cadd(T[2], T[3]), t3 = csub(T[1], T[4]), t4 = csub(T[2], T[3]);
+        X[0] = cadd(T[0], cadd(t1, t2));
+        cpx a1 = { T[0].x + fmaf(C51, t1.x, C52 * t2.x), T[0].y + fmaf(C51, t1.y, C52 * t2.y) };
+        cpx a2 = { T[0].x + fmaf(C52, t1.x, C51 * t2.x), T[0].y + fmaf(C52, t1.y, C51 * t2.y) };
+        cpx b1 = { fmaf(S51, t3.x, S52 * t4.x), fmaf(S51, t3.y, S52 * t4.y) };
+        cpx b2 = { fmaf(S52, t3.x, -(S51 * t4.x)), fmaf(S52, t3.y, -(S51 * t4.y)) };
+        cpx x1 = { a1.x + b1.y, a1.y - b1.x }, x4 = { a1.x - b1.y, a1.y + b1.x };
+        cpx x2 = { a2.x + b2.y, a2.y - b2.x }, x3 = { a2.x - b2.y, a2.y + b2.x };
+        X[m] = x1;
+        X[2 * m] = x2;
+        X[3 * m] = x3;
+        X[4 * m] = x4;
+    } else {
+        /* prime p > 5: direct sum; step here is N/p */
+        for (int t = 0; t < p; t++) {
+            cpx acc = T[0];
+            for (int r = 1; r < p; r++) {
+                int j = (r * t) % p;
+                acc = (j == 0) ? cadd(acc, T[r]) : cadd(acc, cmul(T[r], tw[j * step]));
+            }
+            X[t * m] = acc;
+        }
+    }
+}
+
+/* X[0..n-1] = fft of x[0], x[s], ..., x[(n-1)s]; step = N / n */
+static void fft_rec(const cpx* x, int s, cpx* X, int n, const cpx* tw, int step)
+{
+    if (n == 1) {
+        X[0] = x[0];
+        return;
+    }
+    int p = (n <= 5) ? n : fft_radix(n);
+    int m = n / p;
+    if (m == 1) {
+        cpx T[64];
+        for (int r = 0; r < p; r++) T[r] = x[r * s];
+        butterfly(p, T, X, 1, tw, step);
+        return;
+    }
+    cpx Y[64];
+    for (int r = 0; r < p; r++) fft_rec(x + r * s, s * p, Y + r * m, m, tw, step * p);
+    for (int k = 0; k < m; k++) {
+        cpx T[64];
+        for (int r = 0; r < p; r++)
+            T[r] = (r * k == 0) ? Y[r * m + k] : cmul(Y[r * m + k], tw[r * k * step]);
+        butterfly(p, T, X + k, m, tw, step * m);
+    }
+}
+
+static void fft_n(const cpx* x, cpx* X, int n, const cpx* tw) { fft_rec(x, 1, X, n, tw, 1); }
+
+static void ifft_n(const cpx* z, cpx* out, int n, const cpx* tw)
+{
+    cpx a[64] = { { 0.f, 0.f } }, b[64];
+    for (int i = 0; i < n; i++) a[i] = cconj(z[i]);
+    fft_rec(a, 1, b, n, tw, 1);
+    for (int i = 0; i < n; i++) out[i] = cconj(b[i]);
+}
+
+/* forward 2-D of the real window x (n x n) into X (n rows ky, hc columns kx) */
+static void dft_r2c_2d(const float* x, int n, const float* twf, float* tmp, float* Xf)
 {
     const int hc = n / 2 + 1;
-    for (int kx = 0; kx < hc; kx++)
-        for (int y = 0; y < n; y++) {
-            float re = 0.f, im = 0.f;
-            for (int ky = 0; ky < n; ky++) {
-                int t = (y * ky) % n;
-                float zr = Z[2 * (ky * hc + kx)], zi = Z[2 * (ky * hc + kx) + 1];
-                float wr = tw[2 * t], wi = tw[2 * t + 1];
-                re = fmaf(zr, wr, re);
-                re = fmaf(zi, wi, re);
-                im = fmaf(zi, wr, im);
-                im = fmaf(-zr, wi, im);
-            }
-            tmp[2 * (y * hc + kx)] = re;
-            tmp[2 * (y * hc + kx) + 1] = im;
-        }
-    for (int y = 0; y < n; y++)
+    const cpx* tw = (const cpx*)twf;
+    cpx* S = (cpx*)tmp;
+    cpx* X = (cpx*)Xf;
+    for (int j = 0; j < n / 2; j++) {
+        cpx z[64], Z[64];
         for (int xx = 0; xx < n; xx++) {
-            float acc = tmp[2 * (y * hc)];
-            for (int kx = 1; kx < hc - 1; kx++) {
-                int t = (xx * kx) % n;
-                float gr = tmp[2 * (y * hc + kx)], gi = tmp[2 * (y * hc + kx) + 1];
-                float s = gi * tw[2 * t + 1];
-                s = fmaf(gr, tw[2 * t], s);
-                acc = fmaf(2.f, s, acc);
-            }
-            float nyq = tmp[2 * (y * hc + hc - 1)];
-            acc += (xx & 1) ? -nyq : nyq;
-            out[y * n + xx] = acc;
+            z[xx].x = x[(2 * j) * n + xx];
+            z[xx].y = x[(2 * j + 1) * n + xx];
         }
+        fft_n(z, Z, n, tw);
+        for (int kx = 0; kx < hc; kx++) {
+            int nk = (n - kx) % n;
+            cpx A = { 0.5f * (Z[kx].x + Z[nk].x), 0.5f * (Z[kx].y - Z[nk].y) };
+            cpx B = { 0.5f * (Z[kx].y + Z[nk].y), 0.5f * (Z[nk].x - Z[kx].x) };
+            S[(2 * j) * hc + kx] = A;
+            S[(2 * j + 1) * hc + kx] = B;
+        }
+    }
+    for (int kx = 0; kx < hc; kx++) {
+        cpx col[64], C[64];
+        for (int y = 0; y < n; y++) col[y] = S[y * hc + kx];
+        fft_n(col, C, n, tw);
+        for (int ky = 0; ky < n; ky++) X[ky * hc + kx] = C[ky];
+    }
+}
+
+/* inverse 2-D of the half spectrum Z into the real n x n out */
+static void dft_c2r_2d(const float* Zf, int n, const float* twf, float* tmp, float* out)
+{
+    const int hc = n / 2 + 1;
+    const cpx* tw = (const cpx*)twf;
+    const cpx* Z = (const cpx*)Zf;
+    cpx* G = (cpx*)tmp;
+    for (int kx = 0; kx < hc; kx++) {
+        cpx col[64], g[64];
+        for (int ky = 0; ky < n; ky++) col[ky] = Z[ky * hc + kx];
+        ifft_n(col, g, n, tw);
+        for (int y = 0; y < n; y++) G[y * hc + kx] = g[y];
+    }
+    for (int j = 0; j < n / 2; j++) {
+        cpx A[33], B[33], Zr[64], z[64];
+        for (int k = 0; k < hc; k++) {
+            A[k] = G[(2 * j) * hc + k];
+            B[k] = G[(2 * j + 1) * hc + k];
+        }
+        A[0].y = 0.f; B[0].y = 0.f; A[hc - 1].y = 0.f; B[hc - 1].y = 0.f;
+        for (int k = 0; k < n; k++) {
+            if (k <= n / 2) {
+                Zr[k].x = A[k].x - B[k].y;
+                Zr[k].y = A[k].y + B[k].x;
+            } else {
+                int q = n - k;
+                Zr[k].x = A[q].x + B[q].y;
+                Zr[k].y = B[q].x - A[q].y;
+            }
+        }
+        ifft_n(Zr, z, n, tw);
+        for (int xx = 0; xx < n; xx++) {
+            out[(2 * j) * n + xx] = z[xx].x;
+            out[(2 * j + 1) * n + xx] = z[xx].y;
+        }
+    }
 }
 
 static void cross_power(const float* R, const float* T, int len, float* Z)

@@ -49,7 +49,7 @@ def test_lcg_known_answer_cached(oracle):
     assert np.all(q["deformation"][:, 0] == 2.0) and np.all(q["deformation"][:, 6] == -1.0)
 
 
-@pytest.mark.parametrize("n", [2, 4, 14, 20, 30, 40])
+@pytest.mark.parametrize("n", [2, 4, 6, 8, 10, 12, 14, 18, 20, 22, 26, 28, 30, 36, 40, 42, 50, 56, 60, 62, 64])
 def test_dft_correlation_matches_brute_force(oracle, n):
     rng = np.random.default_rng(n)
     a = rng.normal(size=(n, n)).astype(np.float32)
