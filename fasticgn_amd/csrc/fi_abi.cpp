@@ -271,6 +271,9 @@ int fi_frame_set_shape(fi_frame* h, int waves, int spl)
     return guarded([&] {
         need(h);
         check_shape(waves, spl);
+        if (waves && !icgn_shape_covers(waves, spl, h->r, h->r))
+            fail(FI_E_INVALID_ARGUMENT, "IC-GN shape " + std::to_string(waves) + "x" + std::to_string(spl) +
+                                            " does not fit the driver's subset radius " + std::to_string(h->r));
         h->shape_waves = waves;
         h->shape_spl = spl;
     });

@@ -24,15 +24,21 @@ std::string size_str(int w, int h) { return std::to_string(w) + "x" + std::to_st
 // IC-GN launch with an optional explicit lane-to-POI shape
 void icgn_launch(const IcgnParams& p, int waves, int spl, hipStream_t s)
 {
+    const std::string sub = std::to_string(2 * p.rx + 1) + "x" + std::to_string(2 * p.ry + 1);
     if (waves == 0) {
-        launch_icgn(p, s);
+        if (!launch_icgn(p, s)) fail(FI_E_UNSUPPORTED, "no IC-GN lane shape fits a " + sub + " subset");
         return;
     }
-    const int ns = (2 * p.rx + 1) * (2 * p.ry + 1);
-    if (64 * waves * spl < ns)
+    if (!launch_icgn_shape(p, waves, spl, s))
         fail(FI_E_INVALID_ARGUMENT, "IC-GN shape " + std::to_string(waves) + "x" + std::to_string(spl) +
-                                        " does not cover a subset of " + std::to_string(ns) + " samples");
-    launch_icgn_shape(p, waves, spl, s);
+                                        " does not fit a " + sub + " subset");
+}
+
+// The IC-GN kernels address the B-spline table with 32-bit byte offsets.
+void check_table_size(int w, int h)
+{
+    if ((double)bspline_table_floats(w, h) * 4.0 > 4294967295.0)
+        fail(FI_E_UNSUPPORTED, "image " + size_str(w, h) + " exceeds the 4 GiB B-spline table limit");
 }
 
 size_t max_lds_per_block(int device)
@@ -319,7 +325,8 @@ void fi_icgn::prepare_tar()
 {
     if (!tar_raw.set) fail(FI_E_LOGIC, "ICGN2D1::prepareTar requires a target image");
     DeviceGuard g(device);
-    coef.reserve((size_t)16 * tar_raw.view.width * tar_raw.view.height);
+    check_table_size(tar_raw.view.width, tar_raw.view.height);
+    coef.reserve(bspline_table_floats(tar_raw.view.width, tar_raw.view.height));
     launch_bspline(tar_raw.view, coef.p, stream);
     FI_HIP(hipGetLastError());
     FI_HIP(hipStreamSynchronize(stream));
@@ -406,12 +413,11 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
             p.index = d_index.p + spans[gi].first;
             p.n = (int)spans[gi].second;
             if (!prep_hit) launch_icgn_prep(p, stream);
-            // an explicit shape applies only where it covers the group's subset
-            const int ns = (2 * p.rx + 1) * (2 * p.ry + 1);
-            if (shape_waves && 64 * shape_waves * shape_spl >= ns)
-                launch_icgn_shape(p, shape_waves, shape_spl, stream);
+            // an explicit shape applies only where it fits the group's subset
+            if (shape_waves && icgn_shape_covers(shape_waves, shape_spl, p.rx, p.ry))
+                icgn_launch(p, shape_waves, shape_spl, stream);
             else
-                launch_icgn(p, stream);
+                icgn_launch(p, 0, 0, stream);
             FI_HIP(hipGetLastError());
             gi++;
         }
@@ -621,7 +627,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 
     FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
     FI_HIP(hipEventRecord(ev[0], stream));
-    coef.reserve((size_t)16 * w * h);
+    check_table_size(w, h);
+    coef.reserve(bspline_table_floats(w, h));
     tar_f.reserve((size_t)w * h);
     launch_bspline(tar.view, coef.p, stream);
     launch_to_float(tar.view, tar_f.p, stream);

@@ -15,14 +15,16 @@ void launch_bspline(const DevImage& img, float* coef, hipStream_t s);
 size_t fftcc_lds_bytes(int r, int mode);
 void launch_fftcc(const FftccParams& p, hipStream_t s);
 
-// Largest radius the wave-per-POI IC-GN kernels handle.
+// Largest radius the block-per-POI IC-GN kernels handle.
 constexpr int kIcgnMaxRadius = 31;
 void launch_icgn_prep(const IcgnParams& p, hipStream_t s);
-void launch_icgn(const IcgnParams& p, hipStream_t s);
-// Lane-to-POI shape: waves per POI and samples per lane (waves*64*spl >= ns).
+// Lane shape: waves per POI and rows per band (k_icgn_impl.h).  The launchers
+// return false when the shape does not fit the subset of p.rx, p.ry.
 bool icgn_shape_supported(int waves, int spl);
-void icgn_default_shape(int ns, int& waves, int& spl);
-void launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s);
+bool icgn_shape_covers(int waves, int spl, int rx, int ry);
+void icgn_default_shape(int rx, int ry, int& waves, int& spl);
+bool launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s);
+bool launch_icgn(const IcgnParams& p, hipStream_t s);
 
 // Frame-policy helpers (k_policy.hip).
 struct FrameCounters {

@@ -22,15 +22,24 @@ struct DevImage {
 
 // Per-POI reference-side data kept resident across frames (the reference
 // recomputes it per POI per call on the CPU, src/oc_icgn.cpp:198-234, and
-// caches it per layout on its GPU).  160 bytes, float4-aligned.
+// caches it per layout on its GPU).  176 bytes, float4-aligned.
 struct alignas(16) PoiPrep {
-    float mean;      // reference subset mean
-    float norm;      // sqrt(sum (r - mean)^2)
-    int32_t status;  // 1 prepared, 0 rejected by the entry gate
-    int32_t rx;      // radius the record was prepared for
-    float hinv[36];  // inverse Hessian, row-major (FP64 inversion, stored FP32)
+    float mean;       // reference subset mean
+    float norm;       // sqrt(sum r^2), r = ref - mean
+    int32_t status;   // 1 prepared, 0 rejected by the entry gate
+    int32_t rx;       // radius the record was prepared for
+    float hinv[36];   // inverse Hessian, row-major (FP64 inversion, stored FP32)
+    float ref_sum;    // sum r (zero up to rounding), for the ZNCC
+    int32_t ry;
+    float pad[2];
 };
-static_assert(sizeof(PoiPrep) == 160, "PoiPrep layout");
+static_assert(sizeof(PoiPrep) == 176, "PoiPrep layout");
+
+// Target B-spline table: for texel row y, four planes of float4, plane k at
+// float4 index (4 * y + k) * w + x holding coef[k][0..3] (the dy^k row), then
+// three planes of zero padding so a plane offset added to any in-table texel
+// offset stays inside the allocation.
+inline size_t bspline_table_floats(int w, int h) { return (size_t)16 * w * h + (size_t)12 * w; }
 
 // FFTCC reference-spectrum cache slot metadata.
 struct FftccSlot {

@@ -1,0 +1,486 @@
+// ICGN2D1 iteration kernel for gfx950 (template; instantiated per lane shape
+// in k_icgn*.hip).
+//
+// Semantics follow the CPU reference ICGN2D1::compute(POI2D*),
+// src/oc_icgn.cpp:168-387: the entry gate (-3), the geometric out-of-range
+// predicate of :39-43 (-3, deformation keeps the incoming guess), a NaN sample
+// coordinate (-5), not converged within the iteration cap (-4), NaN result
+// (-5 with u, v reset to the initial guess).  The reference GPU's extra gates
+// (margin-2 reference bound, zero-texture -3, |dp| > 100) are deliberately not
+// applied: the CPU path is the one this is judged against.
+//
+// An iteration reduces twice.  With t' = t - ref_mean for the target samples
+// (kept per lane) and r = ref - ref_mean, the first round sums t', t'^2 and
+// t' r, which give the target mean and norm and the ZNCC
+//     sum (t - tbar)^2 = sum t'^2 - (sum t')^2 / n,
+//     ZNCC = (sum t' r - tbar' * sum r) / (nr * nt);
+// the second sums SD_j e with the reference's per-sample error
+// e = (t - tbar) * (nr / nt) - r, so the numerator, small near convergence, is
+// never the difference of two large sums.  Shifting by ref_mean keeps the
+// one-pass variance free of cancellation.
+//
+// Lane layout ("bands"): the subset's sw columns map to lanes, G = NT / sw
+// bands of R = ceil(sh / G) rows stack vertically, the last band ending at the
+// bottom row (its first ovl rows repeat rows of the band above and are masked
+// out of the sums).  A lane's local x is fixed and its y steps by one per
+// slot, so the warped coordinate of slot k is base + k * W01 and the y-weighted
+// sums come from a running suffix sum.  Lanes beyond the bands mirror band 0
+// (their samples are real ones) and contribute nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fi_device.h"
+#include "fi_launch.h"
+
+namespace fi {
+
+// entry gate of src/oc_icgn.cpp:184-186 (reference side; the deformation and
+// zncc parts are checked by the caller) plus finite coordinates, which keep
+// (int) conversions in bounds
+__device__ __forceinline__ bool ref_gate_ok(float x, float y, int rx, int ry, int w, int h)
+{
+    if (!(isfinite(x) && isfinite(y))) return false;
+    return !(y - ry < 0 || x - rx < 0 || y + ry > h - 1 || x + rx > w - 1);
+}
+
+struct Bands {
+    int R;        // rows per band
+    int ovl;      // leading rows of the last band already covered above
+    int col;      // this lane's subset column
+    int row0;     // this lane's first subset row
+    bool active;  // the lane owns a column of a band
+    bool last;    // the lane is in the last band and that band overlaps
+};
+
+__device__ __forceinline__ Bands make_bands(int rx, int ry, int nt, int tid)
+{
+    Bands b;
+    const int sw = 2 * rx + 1, sh = 2 * ry + 1;
+    const int G = nt / sw;
+    b.R = (sh + G - 1) / G;
+    const int gu = (sh + b.R - 1) / b.R;
+    b.ovl = gu * b.R - sh;
+    int g = tid / sw;
+    b.col = tid - g * sw;
+    b.active = g < gu;
+    if (!b.active) g = 0;
+    b.row0 = (g == gu - 1) ? sh - b.R : g * b.R;
+    b.last = b.active && g == gu - 1 && b.ovl > 0;
+    return b;
+}
+
+// Bicubic B-spline sample from the 16-float texel (coef[i][j] multiplies
+// dy^i dx^j, src/oc_cubic_bspline.cpp:154-177), Horner in dx then dy.
+__device__ __forceinline__ float bspline_eval(const float4 c0, const float4 c1, const float4 c2,
+                                              const float4 c3, float dx, float dy)
+{
+    const float r0 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c0.w, dx, c0.z), dx, c0.y), dx, c0.x);
+    const float r1 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c1.w, dx, c1.z), dx, c1.y), dx, c1.x);
+    const float r2 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c2.w, dx, c2.z), dx, c2.y), dx, c2.x);
+    const float r3 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c3.w, dx, c3.z), dx, c3.y), dx, c3.x);
+    return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(r3, dy, r2), dy, r1), dy, r0);
+}
+
+// The coefficient table is read through a raw buffer resource whose range is
+// the texel rows (64 * w * h bytes).  A texel offset is range checked by the
+// hardware (zeros past the end, no access); the plane is added as the scalar
+// offset, at most 3 * 16 * w bytes, which the three zero planes after the last
+// row cover.  So a sample outside the image (whose value is discarded) needs
+// no clamp and can never read outside the allocation.
+constexpr uint32_t kBufferWord3 = 0x00020000;  // gfx9-family raw buffer, dword data
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const float* coef, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(coef), (short)0, (int)bytes, (int)kBufferWord3);
+}
+
+__device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0);
+    return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+}
+
+// Sample at (gx, gy).  For coordinates in 1 <= x < w-2, 1 <= y < h-2 the
+// truncating conversion is the floor and fract is exact.
+__device__ __forceinline__ float bspline_sample(__amdgpu_buffer_rsrc_t r, uint32_t row_bytes,
+                                                uint32_t plane_bytes, float gx, float gy)
+{
+    const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
+    const uint32_t off = __umul24((uint32_t)(int)gy, row_bytes) + ((uint32_t)(int)gx << 4);
+    return bspline_eval(ld4(r, off, 0), ld4(r, off, plane_bytes), ld4(r, off, 2 * plane_bytes),
+                        ld4(r, off, 3 * plane_bytes), dx, dy);
+}
+
+// Per-lane partial sums -> wave totals, identical on every lane.  Transposing
+// exchanges (permlane32_swap, permlane16_swap) fold pairs of values so each
+// register carries up to four values, one per 16-lane row; DPP then reduces
+// inside the rows and lane reads collect the row totals.
+__device__ __forceinline__ float2 swap32(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return make_float2(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
+__device__ __forceinline__ float2 swap16(float a, float b)
+{
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return make_float2(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
+__device__ __forceinline__ float row_sum16(float v)
+{
+    v += dpp_f<kDppQuadXor1>(v);
+    v += dpp_f<kDppQuadXor2>(v);
+    v += dpp_f<kDppRowRor4>(v);
+    v += dpp_f<kDppRowRor8>(v);
+    return v;
+}
+
+// fold(a, b): rows 0-1 carry a summed over the two lane halves, rows 2-3 b
+__device__ __forceinline__ float fold32(float a, float b)
+{
+    const float2 s = swap32(a, b);
+    return s.x + s.y;
+}
+
+// fold16(p, q) of two fold32 results: rows carry p.lo, q.lo, p.hi, q.hi, each
+// summed over four lanes 16 apart
+__device__ __forceinline__ float fold16(float p, float q)
+{
+    const float2 s = swap16(p, q);
+    return s.x + s.y;
+}
+
+__device__ __forceinline__ void wave_sum3(float (&v)[3])
+{
+    const float w0 = fold32(v[0], v[1]);
+    const float w1 = fold32(v[2], v[2]);
+    const float z = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 2
+    v[0] = lane_read(z, 0);
+    v[2] = lane_read(z, 16);
+    v[1] = lane_read(z, 32);
+}
+
+__device__ __forceinline__ void wave_sum6(float (&v)[6])
+{
+    const float w0 = fold32(v[0], v[1]);
+    const float w1 = fold32(v[2], v[3]);
+    const float w2 = fold32(v[4], v[5]);
+    const float z0 = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 3
+    const float z1 = row_sum16(fold16(w2, w2));  // rows: 4, 4, 5, 5
+    v[0] = lane_read(z0, 0);
+    v[2] = lane_read(z0, 16);
+    v[1] = lane_read(z0, 32);
+    v[3] = lane_read(z0, 48);
+    v[4] = lane_read(z1, 0);
+    v[5] = lane_read(z1, 32);
+}
+
+__device__ __forceinline__ void wave_sum(float (&v)[3]) { wave_sum3(v); }
+__device__ __forceinline__ void wave_sum(float (&v)[6]) { wave_sum6(v); }
+
+// Block totals: wave totals combined through LDS in wave order, so every wave
+// holds the same bits.
+template <int WAVES, int N>
+__device__ __forceinline__ void block_sum(float (&v)[N], float* lds, int wave, int lane)
+{
+    wave_sum(v);
+    if (WAVES == 1) return;
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) lds[wave * N + q] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        float s = lds[q];
+#pragma unroll
+        for (int u = 1; u < WAVES; u++) s += lds[u * N + q];
+        v[q] = s;
+    }
+    __syncthreads();
+}
+
+// Any lane of the block: bit 0 = a, bit 1 = b.
+template <int WAVES>
+__device__ __forceinline__ int block_any2(bool a, bool b, int* lds, int wave, int lane)
+{
+    const int m = (__builtin_amdgcn_ballot_w64(a) != 0 ? 1 : 0) | (__builtin_amdgcn_ballot_w64(b) != 0 ? 2 : 0);
+    if (WAVES == 1) return m;
+    if (lane == 0) lds[wave] = m;
+    __syncthreads();
+    int r = 0;
+#pragma unroll
+    for (int u = 0; u < WAVES; u++) r |= lds[u];
+    __syncthreads();
+    return r;
+}
+
+// A wave-uniform pointer into the constant address space that the optimiser
+// cannot prove loop-invariant: loads through it are scalar loads at their
+// point of use instead of values hoisted into registers held across the
+// sampling pass.  For data the kernel never writes.
+template <typename T>
+__device__ __forceinline__ const __attribute__((address_space(4))) T* reread(const T* ptr)
+{
+    uint64_t v = reinterpret_cast<uint64_t>(ptr);
+    asm volatile("" : "+v"(v));
+    // readfirstlane returns int: widen through uint32_t (no sign extension)
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    const uint64_t u = ((uint64_t)hi << 32) | (uint64_t)lo;
+    return (const __attribute__((address_space(4))) T*)u;
+}
+
+// Largest band overlap a launched shape may have (checked on the host).
+constexpr int kMaxOverlap = 4;
+
+// Minimum resident waves per SIMD the register allocation must allow (the
+// compiler otherwise trades occupancy for deeper load pipelining).
+#ifndef FI_ICGN_MIN_WAVES
+#define FI_ICGN_MIN_WAVES 3
+#endif
+
+// One block of WAVES wavefronts per POI; each lane owns SPL rows of one subset
+// column.  EXACT: the host guarantees R == SPL, so no slot is skipped.
+template <int WAVES, int SPL, bool EXACT>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
+void k_icgn(IcgnParams p)
+{
+    constexpr int NT = 64 * WAVES;
+    __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
+    __shared__ int flags[WAVES];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= p.n) return;
+    const int i = p.index ? (int)p.index[b] : b;
+    fi_poi* poi = p.pois + i;
+    const float x = poi->x, y = poi->y;
+    const float u_in = poi->deformation[0], ux_in = poi->deformation[1], uy_in = poi->deformation[2];
+    const float v_in = poi->deformation[6], vx_in = poi->deformation[7], vy_in = poi->deformation[8];
+    const float z_in = poi->result[2];
+    int rx = p.rx, ry = p.ry;
+    if (p.self_adaptive) {
+        rx = (int)poi->subset_radius[0];
+        ry = (int)poi->subset_radius[1];
+    }
+    const int w = p.w, h = p.h;
+    // entry gate, src/oc_icgn.cpp:184-191
+    if (!ref_gate_ok(x, y, rx, ry, w, h) || fabsf(u_in) >= (float)w || fabsf(v_in) >= (float)h
+        || z_in < 0.f || isnan(u_in) || isnan(v_in)) {
+        if (tid == 0) poi->result[2] = (z_in >= 0.f) ? -3.f : z_in;
+        return;
+    }
+    const PoiPrep* pr = p.prep + (p.prep_index ? p.prep_index[i] : (uint32_t)i);
+    const float ref_mean = pr->mean, ref_norm = pr->norm;
+    const float inv_ref_norm = 1.f / ref_norm;
+
+    const Bands bd = make_bands(rx, ry, NT, tid);
+    const int R = bd.R, ovl = bd.ovl;
+    // Per-slot reference data; zero where the slot repeats a row of the band
+    // above or the lane mirrors band 0, which removes those samples from every
+    // sum but the two of t' itself (masked with m_all / m_dup below).
+    const int ulx = (int)(x - rx), uly = (int)(y - ry);
+    const int px = (int)x, py = (int)y;
+    float rz[SPL], gxv[SPL], gyv[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+        rz[k] = gxv[k] = gyv[k] = 0.f;
+        const bool valid = bd.active && !(bd.last && k < ovl);
+        if ((EXACT || k < R) && valid) {
+            const int row = bd.row0 + k;
+            rz[k] = p.ref[(size_t)(uly + row) * w + (ulx + bd.col)] - ref_mean;
+            const size_t gi = (size_t)(py + row - ry) * w + (px + bd.col - rx);
+            gxv[k] = p.gx[gi];
+            gyv[k] = p.gy[gi];
+        }
+    }
+    const float m_all = bd.active ? 1.f : 0.f;
+    const float m_dup = bd.last ? 0.f : 1.f;
+    const float xl = (float)(bd.col - rx), yl0 = (float)(bd.row0 - ry);
+    const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)w * (uint32_t)h * 64u);
+    const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
+
+    // warp matrix, Deformation2D1::setWarp (src/oc_deformation.cpp:117-128)
+    float W00 = 1.f + ux_in, W01 = uy_in, W02 = u_in;
+    float W10 = vx_in, W11 = 1.f + vy_in, W12 = v_in;
+    float W20 = 0.f, W21 = 0.f, W22 = 1.f;
+    const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
+    const float rxf = (float)rx, ryf = (float)ry;
+    const float rx2 = (float)(rx * rx), ry2 = (float)(ry * ry);
+    const float inv_ns = 1.f / (float)((2 * rx + 1) * (2 * ry + 1));
+    const float conv_crit = p.conv, stop = p.stop;
+    // corners closer than this to the interpolatable range's edge send the
+    // iteration through the exact per-sample test
+    constexpr float kEdgeMargin = 1.f / 64.f;
+
+    int it = 0;
+    float conv = 0.f, zncc = 0.f;
+    do {
+        it++;
+        const float bx = __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
+        const float by = __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
+        // The samples are an affine image of the grid: they lie within the
+        // warped bounding box x + W02 +- (|W00| rx + |W01| ry), so all are in
+        // the interpolatable range (:39-43) when the box is, by a margin.  A NaN
+        // anywhere in the warp fails the test.
+        const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
+        const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
+        const float cx = x + W02, cy = y + W12;
+        const bool box_in = (cx - ex >= 1.f + kEdgeMargin) & (cx + ex < wm2 - kEdgeMargin) &
+                            (cy - ey >= 1.f + kEdgeMargin) & (cy + ey < hm2 - kEdgeMargin);
+        if (!box_in) {
+            // Per-sample test: out of range (-3) takes precedence over a NaN
+            // coordinate (-5), :290-309; neither means the box was only near
+            // the edge.
+            bool oob = false, nan = false;
+#pragma unroll
+            for (int k = 0; k < SPL; k++) {
+                if (EXACT || k < R) {
+                    const float gxp = x + __builtin_fmaf((float)k, W01, bx);
+                    const float gyp = y + __builtin_fmaf((float)k, W11, by);
+                    oob |= (gxp < 1.f) | (gyp < 1.f) | (gxp >= wm2) | (gyp >= hm2);
+                    nan |= (gxp != gxp) | (gyp != gyp);
+                }
+            }
+            const int any = block_any2<WAVES>(oob, nan, flags, wave, lane);
+            if (any) {
+                if (tid == 0) {
+                    poi->result[2] = (any & 1) ? -3.f : -5.f;
+                    if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
+                }
+                return;
+            }
+        }
+
+        // pass 1: sample; t' per slot, and the sums of t', t'^2, t' r (a
+        // repeated row counts once: its t' is masked, its r is zero)
+        float tpv[SPL];
+        float st = 0.f, stt = 0.f, str = 0.f;
+#pragma unroll
+        for (int k = 0; k < SPL; k++) {
+            tpv[k] = 0.f;
+            if (EXACT || k < R) {
+                const float gxp = x + __builtin_fmaf((float)k, W01, bx);
+                const float gyp = y + __builtin_fmaf((float)k, W11, by);
+                const float tp = bspline_sample(table, row_bytes, plane_bytes, gxp, gyp) - ref_mean;
+                const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
+                tpv[k] = tp;
+                st += tpm;
+                stt = __builtin_fmaf(tpm, tp, stt);
+                str = __builtin_fmaf(tp, rz[k], str);
+            }
+        }
+        // lanes mirroring band 0 drop their t' sums (their r is zero)
+        float s1[3] = { st * m_all, stt * m_all, str };
+        block_sum<WAVES>(s1, red, wave, lane);
+
+        // ZNSSD terms (Subset2D::zeroMeanNorm, :311-322); the per-POI
+        // constant is re-read (scalar load) rather than held in a register
+        const auto pc = reread(pr);
+        const float tm = s1[0] * inv_ns;                      // target mean - reference mean
+        const float szz = __builtin_fmaf(-s1[0], tm, s1[1]);  // sum of squared centred target
+        const float inv_tar_norm = __builtin_amdgcn_rsqf(szz);
+        // e = 0 * inf when the target subset is flat, as in the reference
+        const float scale = (szz > 0.f) ? ref_norm * inv_tar_norm : __builtin_nanf("");
+        zncc = __builtin_fmaf(-tm, pc->ref_sum, s1[2]) * inv_tar_norm * inv_ref_norm;
+
+        // pass 2: e = (t - tbar) * (nr / nt) - r and the numerator sum SD_j e
+        // (:311-322); slots in descending order so AK = sum_k k * a_k
+        // accumulates as a running suffix sum (repeated rows have SD = 0)
+        float A = 0.f, AK = 0.f, B = 0.f, BK = 0.f;
+#pragma unroll
+        for (int k = SPL - 1; k >= 0; k--) {
+            if (EXACT || k < R) {
+                const float e = __builtin_fmaf(scale, tpv[k] - tm, -rz[k]);
+                AK += A;
+                BK += B;
+                A = __builtin_fmaf(gxv[k], e, A);
+                B = __builtin_fmaf(gyv[k], e, B);
+            }
+        }
+        float bnum[6] = { A, xl * A, __builtin_fmaf(yl0, A, AK), B, xl * B, __builtin_fmaf(yl0, B, BK) };
+        block_sum<WAVES>(bnum, red, wave, lane);
+
+        // dp = H^-1 b (src/oc_icgn.cpp:324-331)
+        float dp[6];
+#pragma unroll
+        for (int a = 0; a < 6; a++) {
+            float d = 0.f;
+#pragma unroll
+            for (int c = 0; c < 6; c++) d = __builtin_fmaf(pc->hinv[a * 6 + c], bnum[c], d);
+            dp[a] = d;
+        }
+        // W <- W * W(dp)^-1 (:333-340); the 3x3 inverse by cofactors down column 0
+        const float D00 = 1.f + dp[1], D01 = dp[2], D02 = dp[0];
+        const float D10 = dp[4], D11 = 1.f + dp[5], D12 = dp[3];
+        const float D20 = 0.f, D21 = 0.f, D22 = 1.f;
+        const float c00 = D11 * D22 - D12 * D21, c01 = D12 * D20 - D10 * D22, c02 = D10 * D21 - D11 * D20;
+        const float c10 = D21 * D02 - D22 * D01, c11 = D22 * D00 - D20 * D02, c12 = D20 * D01 - D21 * D00;
+        const float c20 = D01 * D12 - D02 * D11, c21 = D02 * D10 - D00 * D12, c22 = D00 * D11 - D01 * D10;
+        const float det = c00 * D00 + (c10 * D10 + c20 * D20);
+        const float invdet = __builtin_amdgcn_rcpf(det);
+        const float I00 = c00 * invdet, I01 = c10 * invdet, I02 = c20 * invdet;
+        const float I10 = c01 * invdet, I11 = c11 * invdet, I12 = c21 * invdet;
+        const float I20 = c02 * invdet, I21 = c12 * invdet, I22 = c22 * invdet;
+        const float N00 = W00 * I00 + (W01 * I10 + W02 * I20), N01 = W00 * I01 + (W01 * I11 + W02 * I21);
+        const float N02 = W00 * I02 + (W01 * I12 + W02 * I22), N10 = W10 * I00 + (W11 * I10 + W12 * I20);
+        const float N11 = W10 * I01 + (W11 * I11 + W12 * I21), N12 = W10 * I02 + (W11 * I12 + W12 * I22);
+        const float N20 = W20 * I00 + (W21 * I10 + W22 * I20), N21 = W20 * I01 + (W21 * I11 + W22 * I21);
+        const float N22 = W20 * I02 + (W21 * I12 + W22 * I22);
+        W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12; W20 = N20; W21 = N21; W22 = N22;
+
+        // convergence norm (:343-352): FP32 sum in the reference's order, the
+        // square root taken in FP64 and rounded
+        float cn = dp[0] * dp[0];
+        cn = cn + dp[1] * dp[1] * rx2;
+        cn = cn + dp[2] * dp[2] * ry2;
+        cn = cn + dp[3] * dp[3];
+        cn = cn + dp[4] * dp[4] * rx2;
+        cn = cn + dp[5] * dp[5] * ry2;
+        conv = (float)sqrt((double)cn);
+    } while ((float)it < stop && conv >= conv_crit);
+
+    if (tid == 0) {
+        if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
+        float u = W02, ux = W00 - 1.f, uy = W01, v = W12, vx = W10, vy = W11 - 1.f;
+        if (conv >= conv_crit && (float)it >= stop) zncc = -4.f;
+        if (isnan(zncc) || isnan(u) || isnan(v)) {
+            u = u_in;
+            v = v_in;
+            zncc = -5.f;
+        }
+        poi->deformation[0] = u;
+        poi->deformation[1] = ux;
+        poi->deformation[2] = uy;
+        poi->deformation[6] = v;
+        poi->deformation[7] = vx;
+        poi->deformation[8] = vy;
+        poi->result[0] = u_in;
+        poi->result[1] = v_in;
+        poi->result[2] = zncc;
+        poi->result[3] = (float)it;
+        poi->result[4] = conv;
+        poi->subset_radius[0] = (float)rx;
+        poi->subset_radius[1] = (float)ry;
+    }
+}
+
+// Instantiated lane shapes (waves, rows per band).  EXACT_A/B are compiled in
+// k_icgn_xa.hip / k_icgn_xb.hip, GENERIC in k_icgn.hip.
+#define FI_ICGN_EXACT_A(X) \
+    X(1, 1) X(1, 2) X(1, 3) X(1, 4) X(1, 6) X(1, 7) X(1, 12) X(1, 13) X(1, 14) X(1, 15) X(1, 16)
+#define FI_ICGN_EXACT_B(X)                                                      \
+    X(2, 4) X(2, 8) X(2, 11) X(2, 12) X(2, 13) X(2, 14)                          \
+    X(4, 2) X(4, 4) X(4, 7) X(4, 9) X(4, 10) X(4, 11) X(4, 14) X(4, 15) X(4, 16)
+#define FI_ICGN_GENERIC(X) X(1, 16) X(2, 16) X(4, 16)
+
+template <int WAVES, int SPL, bool EXACT>
+void launch_icgn_t(const IcgnParams& p, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), 0, s, p);
+}
+
+}  // namespace fi

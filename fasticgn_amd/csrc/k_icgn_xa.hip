@@ -1,0 +1,11 @@
+// Exact-shape IC-GN kernels (FI_ICGN_EXACT_A); split from k_icgn.hip so the
+// instantiations compile in parallel.
+#include "k_icgn_impl.h"
+
+namespace fi {
+
+#define X(W, S) template void launch_icgn_t<W, S, true>(const IcgnParams&, hipStream_t);
+FI_ICGN_EXACT_A(X)
+#undef X
+
+}  // namespace fi

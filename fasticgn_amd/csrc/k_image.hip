@@ -67,9 +67,11 @@ __device__ __forceinline__ float bc(int i, int j)
 // BicubicBspline::prepare, src/oc_cubic_bspline.cpp:84-132.  One thread per
 // texel; texels outside 1 <= r < h-2, 1 <= c < w-2 are zero.  Per texel the 16
 // values coef[k][l] = P[3-k][3-l], P[k][l] = sum_m sum_n (BC[l][m] * BC[k][n]) *
-// Q[n][m] in (m, n) order.  Stored as four planes: plane k holds the float4
-// coef[k][0..3] of every texel, row-major, so a wavefront reading one plane for
-// consecutive texels reads one contiguous run.
+// Q[n][m] in (m, n) order.  Row r of the table is four planes of w float4s,
+// plane k holding coef[k][0..3] (bspline_table_floats), so one sample reads
+// four float4s w*16 bytes apart and a wavefront reading one plane for
+// consecutive texels reads one contiguous run.  Threads of the first row also
+// clear the three padding planes after the last row.
 template <typename T>
 __global__ __launch_bounds__(256) void k_bspline(const T* __restrict__ raw, size_t pitch, int w,
                                                  int h, float scale, float* __restrict__ coef)
@@ -77,10 +79,15 @@ __global__ __launch_bounds__(256) void k_bspline(const T* __restrict__ raw, size
     const int c = blockIdx.x * 64 + (threadIdx.x & 63);
     const int r = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (c >= w || r >= h) return;
-    const size_t plane = (size_t)w * h;
-    float4* dst = reinterpret_cast<float4*>(coef) + ((size_t)r * w + c);
+    float4* tab = reinterpret_cast<float4*>(coef);
+    const size_t plane = (size_t)w;
+    float4* dst = tab + (size_t)4 * r * w + c;
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (r == 0) {
+        float4* pad = tab + (size_t)4 * h * w + c;
+        pad[0] = z; pad[plane] = z; pad[2 * plane] = z;
+    }
     if (r < 1 || r >= h - 2 || c < 1 || c >= w - 2) {
-        const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
         dst[0] = z; dst[plane] = z; dst[2 * plane] = z; dst[3 * plane] = z;
         return;
     }

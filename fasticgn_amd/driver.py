@@ -53,9 +53,10 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))
         return out
 
-    def set_shape(self, waves: int, samples_per_lane: int) -> None:
-        """IC-GN lane-to-POI shape (0, 0 = default)."""
-        nat.call("fi_frame_set_shape", self._h, int(waves), int(samples_per_lane))
+    def set_shape(self, waves: int, rows_per_band: int) -> None:
+        """IC-GN lane-to-POI shape: waves per POI and rows per band (0, 0 =
+        default); it must fit the driver's subset."""
+        nat.call("fi_frame_set_shape", self._h, int(waves), int(rows_per_band))
 
     def reset_sequence(self) -> None:
         nat.call("fi_frame_reset_sequence", self._h)

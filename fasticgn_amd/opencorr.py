@@ -192,9 +192,10 @@ class ICGN2D1(_Handle):
         nat.call("fi_icgn_set_self_adaptive", self._h, int(bool(on)))
         self.self_adaptive = bool(on)
 
-    def setShape(self, waves: int, samples_per_lane: int) -> None:
-        """Lane-to-POI shape of the IC-GN kernel (0, 0 = default)."""
-        nat.call("fi_icgn_set_shape", self._h, int(waves), int(samples_per_lane))
+    def setShape(self, waves: int, rows_per_band: int) -> None:
+        """Lane-to-POI shape of the IC-GN kernel: waves per POI and rows per
+        band (0, 0 = default)."""
+        nat.call("fi_icgn_set_shape", self._h, int(waves), int(rows_per_band))
 
     def setImages(self, ref: Image2D, tar: Image2D) -> None:
         self.setRefImage(ref)

@@ -135,11 +135,13 @@ int fi_icgn_create(int rx, int ry, float conv, float stop, int device, fi_icgn**
 int fi_icgn_destroy(fi_icgn* h);
 int fi_icgn_set_iteration(fi_icgn* h, float conv, float stop);       /* :120-124 */
 int fi_icgn_set_self_adaptive(fi_icgn* h, int on);                   /* src/oc_dic.h:62 */
-/* Lane-to-POI shape of the IC-GN kernel: waves per POI (1, 2, 4) and samples
- * per lane; 0, 0 restores the default.  No reference counterpart (the CUDA
- * engine picks its block size by occupancy, cu:331-359).  Results depend on the
- * shape only through FP32 summation order. */
-int fi_icgn_set_shape(fi_icgn* h, int waves, int samples_per_lane);
+/* Lane-to-POI shape of the IC-GN kernel: waves per POI (1, 2, 4) and rows per
+ * band (the subset's columns map to lanes; 64 * waves / (2rx+1) bands of rows
+ * stack vertically and must cover the 2ry+1 rows); 0, 0 restores the default.
+ * A shape that does not fit the subset fails at compute (FI_E_INVALID_ARGUMENT).
+ * No reference counterpart (the CUDA engine picks its block size by occupancy,
+ * cu:331-359).  Results depend on the shape only through FP32 summation order. */
+int fi_icgn_set_shape(fi_icgn* h, int waves, int rows_per_band);
 int fi_icgn_set_ref(fi_icgn* h, const void* px, int width, int height,
                     size_t stride_bytes, int pixel, int depth12);
 int fi_icgn_set_tar(fi_icgn* h, const void* px, int width, int height,
@@ -174,7 +176,7 @@ int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
 /* Forget the previous frames (next run seeds by FFTCC again). */
 int fi_frame_reset_sequence(fi_frame* h);
 /* IC-GN lane-to-POI shape for this driver (see fi_icgn_set_shape). */
-int fi_frame_set_shape(fi_frame* h, int waves, int samples_per_lane);
+int fi_frame_set_shape(fi_frame* h, int waves, int rows_per_band);
 
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */
