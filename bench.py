@@ -113,7 +113,9 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-sample-every", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--shape", default="", help="IC-GN lane-to-POI shape WAVES,SPL")
+    ap.add_argument("--shape", default="", help="IC-GN lane-to-POI shape WAVES,ROWS")
+    ap.add_argument("--sampling", default="auto", choices=["auto", "table"],
+                    help="IC-GN target sampling (LDS tile with table fallback, or table only)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -138,6 +140,7 @@ def main() -> int:
     drv.set_pois(sc.pois)
     if args.shape:
         drv.set_shape(*[int(v) for v in args.shape.split(",")])
+    drv.set_sampling(args.sampling)
     t0 = time.perf_counter()
     drv.upload_target(fa.Image2D(sc.tar))
     upload_s = time.perf_counter() - t0

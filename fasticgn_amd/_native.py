@@ -27,6 +27,7 @@ FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT, FI_DEPTH12_LEFT = 0, 1, 2
 FI_POLICY_SKIP_FFTCC = 1
 FI_POLICY_POI_FALLBACK = 2
 FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
+FI_SAMPLING_AUTO, FI_SAMPLING_TABLE = 0, 1
 
 # opencorr::POI2D, src/oc_poi.h:102-108: 25 float32 = 100 bytes
 POI_DTYPE = np.dtype([
@@ -111,6 +112,7 @@ SIGNATURES = {
     "fi_icgn_set_iteration": [_P, _F, _F],
     "fi_icgn_set_self_adaptive": [_P, _I],
     "fi_icgn_set_shape": [_P, _I, _I],
+    "fi_icgn_set_sampling": [_P, _I],
     "fi_icgn_set_ref": [_P, _P, _I, _I, _S, _I, _I],
     "fi_icgn_set_tar": [_P, _P, _I, _I, _S, _I, _I],
     "fi_icgn_prepare_ref": [_P],
@@ -126,6 +128,7 @@ SIGNATURES = {
     "fi_frame_download": [_P, _P, _S],
     "fi_frame_reset_sequence": [_P],
     "fi_frame_set_shape": [_P, _I, _I],
+    "fi_frame_set_sampling": [_P, _I],
     "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
 }
 

@@ -187,6 +187,21 @@ int fi_icgn_set_shape(fi_icgn* h, int waves, int spl)
     });
 }
 
+static void check_sampling(int mode)
+{
+    if (mode != FI_SAMPLING_AUTO && mode != FI_SAMPLING_TABLE)
+        fail(FI_E_INVALID_ARGUMENT, "unknown IC-GN sampling mode " + std::to_string(mode));
+}
+
+int fi_icgn_set_sampling(fi_icgn* h, int mode)
+{
+    return guarded([&] {
+        need(h);
+        check_sampling(mode);
+        h->sampling = mode;
+    });
+}
+
 int fi_icgn_set_ref(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
 {
     return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
@@ -276,6 +291,15 @@ int fi_frame_set_shape(fi_frame* h, int waves, int spl)
                                             " does not fit the driver's subset radius " + std::to_string(h->r));
         h->shape_waves = waves;
         h->shape_spl = spl;
+    });
+}
+
+int fi_frame_set_sampling(fi_frame* h, int mode)
+{
+    return guarded([&] {
+        need(h);
+        check_sampling(mode);
+        h->sampling = mode;
     });
 }
 

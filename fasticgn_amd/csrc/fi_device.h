@@ -144,6 +144,20 @@ __device__ __forceinline__ int xcd_remap(int b, int nblocks)
     return xcd * q + min(xcd, rem) + slot;
 }
 
+// BC = B * C of the bicubic B-spline, src/oc_cubic_bspline.h:52-58, as the
+// float quotients the reference's initialisers produce.  Row i holds the
+// coefficients of d^(3-i) of the four neighbour weights.
+__device__ __forceinline__ float bc(int i, int j)
+{
+    constexpr float m[4][4] = {
+        { -144.0f / 336.0f, 384.0f / 336.0f, -384.0f / 336.0f, 144.0f / 336.0f },
+        { 342.0f / 336.0f, -702.0f / 336.0f, 450.0f / 336.0f, -90.0f / 336.0f },
+        { -198.0f / 336.0f, -18.0f / 336.0f, 270.0f / 336.0f, -54.0f / 336.0f },
+        { 0.0f / 336.0f, 336.0f / 336.0f, 0.0f / 336.0f, 0.0f / 336.0f },
+    };
+    return m[i][j];
+}
+
 template <typename T>
 __device__ __forceinline__ float load_px(const T* img, size_t idx, float scale);
 

@@ -328,6 +328,8 @@ void fi_icgn::prepare_tar()
     check_table_size(tar_raw.view.width, tar_raw.view.height);
     coef.reserve(bspline_table_floats(tar_raw.view.width, tar_raw.view.height));
     launch_bspline(tar_raw.view, coef.p, stream);
+    tar_f.reserve((size_t)tar_raw.view.width * tar_raw.view.height);
+    launch_to_float(tar_raw.view, tar_f.p, stream);
     FI_HIP(hipGetLastError());
     FI_HIP(hipStreamSynchronize(stream));
     coef_ready = true;
@@ -378,6 +380,7 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
     p.gx = gx.p;
     p.gy = gy.p;
     p.coef = coef.p;
+    p.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
     p.w = ref_raw.view.width;
     p.h = ref_raw.view.height;
     p.pois = d_pois.p;
@@ -520,6 +523,7 @@ void fi_frame::prepare()
     launch_gradient(ref_f.p, w, h, gx.p, gy.p, stream);
     const size_t cap = std::max<size_t>(n, 1);
     tmpl.reserve(cap);
+    iters.reserve(cap);
     cur.reserve(cap);
     prev.reserve(cap);
     save.reserve(cap);
@@ -586,6 +590,7 @@ void fi_frame::icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index)
     ip.gx = gx.p;
     ip.gy = gy.p;
     ip.coef = coef.p;
+    ip.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
     ip.pois = queue;
@@ -596,8 +601,9 @@ void fi_frame::icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index)
     ip.ry = r;
     ip.conv = conv;
     ip.stop = stop;
-    ip.iter_total = iter_total.p;
+    ip.iters = iters.p;
     icgn_launch(ip, shape_waves, shape_spl, stream);
+    launch_sum_iters(iters.p, count, iter_total.p, stream);
 }
 
 void fi_frame::zero_counters()

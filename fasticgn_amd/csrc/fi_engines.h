@@ -55,9 +55,10 @@ struct fi_icgn {
     float conv = 0.001f, stop = 10.f;
     bool self_adaptive = false;
     int shape_waves = 0, shape_spl = 0;
+    int sampling = FI_SAMPLING_AUTO;
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw, tar_raw;
-    fi::DevArray<float> ref_f, gx, gy, coef;
+    fi::DevArray<float> ref_f, gx, gy, coef, tar_f;
     bool grads_ready = false, coef_ready = false;
     uint64_t ref_version = 0;
     // per-POI reference records, keyed by (x, y, rx, ry) per POI
@@ -82,6 +83,7 @@ struct fi_frame {
     int r = 0;
     float conv = 0.001f, stop = 10.f;
     int shape_waves = 0, shape_spl = 0;
+    int sampling = FI_SAMPLING_AUTO;
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw, tar;
     fi::DevArray<float> ref_f, gx, gy, coef, tw, spectra, tar_f;
@@ -92,6 +94,7 @@ struct fi_frame {
     fi::DevArray<uint32_t> cand;
     fi::DevArray<fi::FrameCounters> counters;
     fi::PinnedArray<fi::FrameCounters> h_counters;
+    fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry
     fi::DevArray<unsigned long long> iter_total;
     fi::PinnedArray<unsigned long long> h_iter_total;
     size_t n = 0;

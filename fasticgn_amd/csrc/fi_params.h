@@ -68,7 +68,8 @@ struct IcgnParams {
     const float* ref;         // reference, f32 row-major
     const float* gx;
     const float* gy;
-    const float* coef;        // target B-spline table, 16 floats per texel
+    const float* coef;        // target B-spline table (bspline_table_floats)
+    const float* tar;         // target, f32 row-major (nullptr: table sampling only)
     int w, h;
     fi_poi* pois;
     const uint32_t* index;    // POIs this launch processes (nullptr = all)
@@ -78,7 +79,9 @@ struct IcgnParams {
     int rx, ry;               // uniform radius (self_adaptive = 0)
     int self_adaptive;
     float conv, stop;
-    unsigned long long* iter_total;  // optional: executed iterations, summed
+    uint32_t* iters;          // optional: executed iterations of launch entry b
+    // per-POI LDS tile of the target (set by the launcher; tile_w = 0: none)
+    int tile_w, tile_h, tile_stride;
 };
 
 }  // namespace fi

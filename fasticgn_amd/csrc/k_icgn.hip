@@ -262,8 +262,25 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
     hipLaunchKernelGGL(k_icgn_prep, dim3(p.n), dim3(64), 0, s, p);
 }
 
-bool launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s)
+// The per-POI LDS tile: the subset's neighbourhood (2r+1 + 3 pixels) plus
+// kTileMargin on each side, when the image is that large and the f32 target
+// is given; an odd row stride spreads the bands over the LDS banks.
+static IcgnParams with_tile(const IcgnParams& p)
 {
+    IcgnParams q = p;
+    q.tile_w = q.tile_h = q.tile_stride = 0;
+    if (p.tar == nullptr) return q;
+    const int tw = 2 * p.rx + 4 + 2 * kTileMargin, th = 2 * p.ry + 4 + 2 * kTileMargin;
+    if (tw > p.w || th > p.h) return q;
+    q.tile_w = tw;
+    q.tile_h = th;
+    q.tile_stride = tw | 1;
+    return q;
+}
+
+bool launch_icgn_shape(const IcgnParams& p0, int waves, int spl, hipStream_t s)
+{
+    const IcgnParams p = with_tile(p0);
     const int R = band_rows(waves, p.rx, p.ry);
     if (R == 0 || R > spl) return false;
     if (p.n <= 0) return true;

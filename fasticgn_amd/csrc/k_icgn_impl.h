@@ -17,7 +17,9 @@
 // the second sums SD_j e with the reference's per-sample error
 // e = (t - tbar) * (nr / nt) - r, so the numerator, small near convergence, is
 // never the difference of two large sums.  Shifting by ref_mean keeps the
-// one-pass variance free of cancellation.
+// one-pass variance free of cancellation.  Per-lane sums are FMA chains of at
+// most SPL terms; the cross-lane and cross-wave trees are compensated
+// (TwoSum); the convergence norm is accumulated in FP64.
 //
 // Lane layout ("bands"): the subset's sw columns map to lanes, G = NT / sw
 // bands of R = ceil(sh / G) rows stack vertically, the last band ending at the
@@ -113,6 +115,72 @@ __device__ __forceinline__ float bspline_sample(__amdgpu_buffer_rsrc_t r, uint32
                         ld4(r, off, 3 * plane_bytes), dx, dy);
 }
 
+// ---------------------------------------------------------------------------
+// Sampling from the raw target staged in LDS.  The B-spline value
+// sum_{k,l} coef[k][l] dy^k dx^l with coef = BC Q BC^T (flipped) is, term by
+// term, sum_n wy_n sum_m wx_m Q[n][m] over the 4x4 neighbourhood Q (rows
+// iy-1..iy+2, columns ix-1..ix+2) with w_m(d) = BC[0][m] d^3 + BC[1][m] d^2 +
+// BC[2][m] d + BC[3][m]: the same polynomial, evaluated from 16 raw pixels
+// instead of 16 stored coefficients.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void spline_weights(float d, float& w0, float& w1, float& w2, float& w3)
+{
+    w0 = __builtin_fmaf(__builtin_fmaf(bc(0, 0), d, bc(1, 0)), d, bc(2, 0)) * d;
+    w1 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(bc(0, 1), d, bc(1, 1)), d, bc(2, 1)), d, bc(3, 1));
+    w2 = __builtin_fmaf(__builtin_fmaf(bc(0, 2), d, bc(1, 2)), d, bc(2, 2)) * d;
+    w3 = __builtin_fmaf(__builtin_fmaf(bc(0, 3), d, bc(1, 3)), d, bc(2, 3)) * d;
+}
+
+__device__ __forceinline__ float row4(const float* q, float w0, float w1, float w2, float w3)
+{
+    return __builtin_fmaf(w3, q[3], __builtin_fmaf(w2, q[2], __builtin_fmaf(w1, q[1], w0 * q[0])));
+}
+
+// Sample at (gx, gy) from a tile whose element (0, 0) is image pixel
+// (tx0, ty0); cbase = (ty0 + 1) * stride + tx0 + 1 places the neighbourhood's
+// first row and column.  The caller guarantees the neighbourhood lies in the
+// tile.
+__device__ __forceinline__ float tile_sample(const float* tile, int stride, int cbase, float gx, float gy)
+{
+    const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
+    const int base = (int)__umul24((uint32_t)(int)gy, (uint32_t)stride) + (int)gx - cbase;
+    const float* q0 = tile + base;
+    const float* q1 = q0 + stride;
+    const float* q2 = q1 + stride;
+    const float* q3 = q2 + stride;
+    float x0, x1, x2, x3, y0, y1, y2, y3;
+    spline_weights(dx, x0, x1, x2, x3);
+    spline_weights(dy, y0, y1, y2, y3);
+    const float r0 = row4(q0, x0, x1, x2, x3), r1 = row4(q1, x0, x1, x2, x3);
+    const float r2 = row4(q2, x0, x1, x2, x3), r3 = row4(q3, x0, x1, x2, x3);
+    return __builtin_fmaf(y3, r3, __builtin_fmaf(y2, r2, __builtin_fmaf(y1, r1, y0 * r0)));
+}
+
+// Copy image rows ty0.. and columns tx0.. (tw x th pixels) into the tile,
+// twelve loads per lane in flight before their stores.
+__device__ __forceinline__ void stage_tile(float* tile, const float* tar, int w, int tx0, int ty0,
+                                           int tw, int th, int stride, int tid, int nt)
+{
+    constexpr int kBatch = 12;
+    const uint32_t magic = (uint32_t)((0x100000000ull + (unsigned long long)tw - 1) / (unsigned long long)tw);
+    const int n = tw * th;
+    for (int base = tid; base < n; base += kBatch * nt) {
+        float v[kBatch];
+        int dst[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; j++) {
+            const int idx = min(base + j * nt, n - 1);  // a clamped read is not stored
+            const int r = (int)__umulhi((uint32_t)idx, magic);
+            const int c = idx - r * tw;
+            v[j] = tar[(size_t)(ty0 + r) * w + (tx0 + c)];
+            dst[j] = r * stride + c;
+        }
+#pragma unroll
+        for (int j = 0; j < kBatch; j++)
+            if (base + j * nt < n) tile[dst[j]] = v[j];
+    }
+}
+
 // Per-lane partial sums -> wave totals, identical on every lane.  Transposing
 // exchanges (permlane32_swap, permlane16_swap) fold pairs of values so each
 // register carries up to four values, one per 16-lane row; DPP then reduces
@@ -129,60 +197,87 @@ __device__ __forceinline__ float2 swap16(float a, float b)
     return make_float2(__uint_as_float(r[0]), __uint_as_float(r[1]));
 }
 
-__device__ __forceinline__ float row_sum16(float v)
+// Compensated pair: value s plus the accumulated rounding errors c.
+struct CPair {
+    float s, c;
+};
+
+// s + t exactly as a rounded sum and its error (Knuth TwoSum), the errors of
+// both operands carried along
+__device__ __forceinline__ CPair cadd(CPair a, CPair b)
 {
-    v += dpp_f<kDppQuadXor1>(v);
-    v += dpp_f<kDppQuadXor2>(v);
-    v += dpp_f<kDppRowRor4>(v);
-    v += dpp_f<kDppRowRor8>(v);
-    return v;
+    float s, e;
+    two_sum(a.s, b.s, s, e);
+    return { s, (a.c + b.c) + e };
 }
 
-// fold(a, b): rows 0-1 carry a summed over the two lane halves, rows 2-3 b
-__device__ __forceinline__ float fold32(float a, float b)
+// fold32(a, b): rows 0-1 carry a summed over the two lane halves, rows 2-3 b
+__device__ __forceinline__ CPair fold32(float a, float b)
 {
-    const float2 s = swap32(a, b);
-    return s.x + s.y;
+    const float2 v = swap32(a, b);
+    CPair r;
+    two_sum(v.x, v.y, r.s, r.c);
+    return r;
 }
 
 // fold16(p, q) of two fold32 results: rows carry p.lo, q.lo, p.hi, q.hi, each
 // summed over four lanes 16 apart
-__device__ __forceinline__ float fold16(float p, float q)
+__device__ __forceinline__ CPair fold16(CPair p, CPair q)
 {
-    const float2 s = swap16(p, q);
-    return s.x + s.y;
+    const float2 s = swap16(p.s, q.s);
+    const float2 c = swap16(p.c, q.c);
+    return cadd({ s.x, c.x }, { s.y, c.y });
 }
 
+template <int CTRL>
+__device__ __forceinline__ CPair cdpp(CPair v)
+{
+    return cadd(v, { dpp_f<CTRL>(v.s), dpp_f<CTRL>(v.c) });
+}
+
+// sum of the 16 lanes of each row (every lane of the row holds it)
+__device__ __forceinline__ CPair row_sum16(CPair v)
+{
+    v = cdpp<kDppQuadXor1>(v);
+    v = cdpp<kDppQuadXor2>(v);
+    v = cdpp<kDppRowRor4>(v);
+    v = cdpp<kDppRowRor8>(v);
+    return v;
+}
+
+__device__ __forceinline__ float row_total(CPair z, int lane) { return lane_read(z.s, lane) + lane_read(z.c, lane); }
+
+// Per-lane partial sums -> wave totals with a compensated (TwoSum) tree.
 __device__ __forceinline__ void wave_sum3(float (&v)[3])
 {
-    const float w0 = fold32(v[0], v[1]);
-    const float w1 = fold32(v[2], v[2]);
-    const float z = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 2
-    v[0] = lane_read(z, 0);
-    v[2] = lane_read(z, 16);
-    v[1] = lane_read(z, 32);
+    const CPair w0 = fold32(v[0], v[1]);
+    const CPair w1 = fold32(v[2], v[2]);
+    const CPair z = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 2
+    v[0] = row_total(z, 0);
+    v[2] = row_total(z, 16);
+    v[1] = row_total(z, 32);
 }
 
 __device__ __forceinline__ void wave_sum6(float (&v)[6])
 {
-    const float w0 = fold32(v[0], v[1]);
-    const float w1 = fold32(v[2], v[3]);
-    const float w2 = fold32(v[4], v[5]);
-    const float z0 = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 3
-    const float z1 = row_sum16(fold16(w2, w2));  // rows: 4, 4, 5, 5
-    v[0] = lane_read(z0, 0);
-    v[2] = lane_read(z0, 16);
-    v[1] = lane_read(z0, 32);
-    v[3] = lane_read(z0, 48);
-    v[4] = lane_read(z1, 0);
-    v[5] = lane_read(z1, 32);
+    const CPair w0 = fold32(v[0], v[1]);
+    const CPair w1 = fold32(v[2], v[3]);
+    const CPair w2 = fold32(v[4], v[5]);
+    const CPair z0 = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 3
+    const CPair z1 = row_sum16(fold16(w2, w2));  // rows: 4, 4, 5, 5
+    v[0] = row_total(z0, 0);
+    v[2] = row_total(z0, 16);
+    v[1] = row_total(z0, 32);
+    v[3] = row_total(z0, 48);
+    v[4] = row_total(z1, 0);
+    v[5] = row_total(z1, 32);
 }
 
 __device__ __forceinline__ void wave_sum(float (&v)[3]) { wave_sum3(v); }
 __device__ __forceinline__ void wave_sum(float (&v)[6]) { wave_sum6(v); }
 
-// Block totals: wave totals combined through LDS in wave order, so every wave
-// holds the same bits.
+// Block totals: wave totals combined through LDS in wave order with TwoSum,
+// so every wave holds the same bits.
 template <int WAVES, int N>
 __device__ __forceinline__ void block_sum(float (&v)[N], float* lds, int wave, int lane)
 {
@@ -195,10 +290,10 @@ __device__ __forceinline__ void block_sum(float (&v)[N], float* lds, int wave, i
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < N; q++) {
-        float s = lds[q];
+        CPair acc = { lds[q], 0.f };
 #pragma unroll
-        for (int u = 1; u < WAVES; u++) s += lds[u * N + q];
-        v[q] = s;
+        for (int u = 1; u < WAVES; u++) acc = cadd(acc, { lds[u * N + q], 0.f });
+        v[q] = acc.s + acc.c;
     }
     __syncthreads();
 }
@@ -234,6 +329,10 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* reread(con
     return (const __attribute__((address_space(4))) T*)u;
 }
 
+// Pixels of slack around a subset's neighbourhood in the LDS tile, so the
+// tile survives the first iterations' motion without restaging.
+constexpr int kTileMargin = 2;
+
 // Largest band overlap a launched shape may have (checked on the host).
 constexpr int kMaxOverlap = 4;
 
@@ -252,6 +351,7 @@ void k_icgn(IcgnParams p)
     constexpr int NT = 64 * WAVES;
     __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
     __shared__ int flags[WAVES];
+    extern __shared__ float tile[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int b = xcd_remap(blockIdx.x, gridDim.x);
     if (b >= p.n) return;
@@ -270,7 +370,10 @@ void k_icgn(IcgnParams p)
     // entry gate, src/oc_icgn.cpp:184-191
     if (!ref_gate_ok(x, y, rx, ry, w, h) || fabsf(u_in) >= (float)w || fabsf(v_in) >= (float)h
         || z_in < 0.f || isnan(u_in) || isnan(v_in)) {
-        if (tid == 0) poi->result[2] = (z_in >= 0.f) ? -3.f : z_in;
+        if (tid == 0) {
+            poi->result[2] = (z_in >= 0.f) ? -3.f : z_in;
+            if (p.iters) p.iters[b] = 0;
+        }
         return;
     }
     const PoiPrep* pr = p.prep + (p.prep_index ? p.prep_index[i] : (uint32_t)i);
@@ -284,25 +387,6 @@ void k_icgn(IcgnParams p)
     // sum but the two of t' itself (masked with m_all / m_dup below).
     const int ulx = (int)(x - rx), uly = (int)(y - ry);
     const int px = (int)x, py = (int)y;
-    float rz[SPL], gxv[SPL], gyv[SPL];
-#pragma unroll
-    for (int k = 0; k < SPL; k++) {
-        rz[k] = gxv[k] = gyv[k] = 0.f;
-        const bool valid = bd.active && !(bd.last && k < ovl);
-        if ((EXACT || k < R) && valid) {
-            const int row = bd.row0 + k;
-            rz[k] = p.ref[(size_t)(uly + row) * w + (ulx + bd.col)] - ref_mean;
-            const size_t gi = (size_t)(py + row - ry) * w + (px + bd.col - rx);
-            gxv[k] = p.gx[gi];
-            gyv[k] = p.gy[gi];
-        }
-    }
-    const float m_all = bd.active ? 1.f : 0.f;
-    const float m_dup = bd.last ? 0.f : 1.f;
-    const float xl = (float)(bd.col - rx), yl0 = (float)(bd.row0 - ry);
-    const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)w * (uint32_t)h * 64u);
-    const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
-
     // warp matrix, Deformation2D1::setWarp (src/oc_deformation.cpp:117-128)
     float W00 = 1.f + ux_in, W01 = uy_in, W02 = u_in;
     float W10 = vx_in, W11 = 1.f + vy_in, W12 = v_in;
@@ -315,6 +399,54 @@ void k_icgn(IcgnParams p)
     // corners closer than this to the interpolatable range's edge send the
     // iteration through the exact per-sample test
     constexpr float kEdgeMargin = 1.f / 64.f;
+
+    // The LDS tile holds the raw target around the warped subset; it is
+    // (re)staged when an iteration's neighbourhoods leave it, and an iteration
+    // whose neighbourhoods cannot fit in it samples the table instead.
+    const int tile_w = p.tile_w, tile_h = p.tile_h, tile_stride = p.tile_stride;
+    int tx0 = -(1 << 30), ty0 = -(1 << 30);
+    constexpr float kTileSlack = 0.125f;  // bound on sample vs box rounding
+
+    // Every lane loads (idle lanes and repeated rows address real subset
+    // samples; a generic shape's slots past R repeat row R-1), then masks: no
+    // divergent branch, so all loads are in flight together.
+    float rz[SPL], gxv[SPL], gyv[SPL];
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+        const int row = bd.row0 + (EXACT ? k : min(k, R - 1));
+        const float rv = p.ref[(size_t)(uly + row) * w + (ulx + bd.col)];
+        const size_t gi = (size_t)(py + row - ry) * w + (px + bd.col - rx);
+        const float gxl = p.gx[gi], gyl = p.gy[gi];
+        const bool valid = bd.active && !(bd.last && k < ovl) && (EXACT || k < R);
+        rz[k] = valid ? rv - ref_mean : 0.f;
+        gxv[k] = valid ? gxl : 0.f;
+        gyv[k] = valid ? gyl : 0.f;
+    }
+    // Stage the tile for the incoming warp now, so its loads are in flight
+    // together with the reference loads above (one round trip, not two).
+    {
+        const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
+        const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
+        const float cx = x + W02, cy = y + W12;
+        if (tile_w > 0 && cx - ex >= 1.f + kEdgeMargin && cx + ex < wm2 - kEdgeMargin &&
+            cy - ey >= 1.f + kEdgeMargin && cy + ey < hm2 - kEdgeMargin) {
+            const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
+            const int nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
+            const int ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
+            const int ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
+            if (nx1 - nx0 < tile_w && ny1 - ny0 < tile_h) {
+                tx0 = min(max(nx0 - (tile_w - (nx1 - nx0 + 1)) / 2, 0), w - tile_w);
+                ty0 = min(max(ny0 - (tile_h - (ny1 - ny0 + 1)) / 2, 0), h - tile_h);
+                stage_tile(tile, p.tar, w, tx0, ty0, tile_w, tile_h, tile_stride, tid, NT);
+                __syncthreads();
+            }
+        }
+    }
+    const float m_all = bd.active ? 1.f : 0.f;
+    const float m_dup = bd.last ? 0.f : 1.f;
+    const float xl = (float)(bd.col - rx), yl0 = (float)(bd.row0 - ry);
+    const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)w * (uint32_t)h * 64u);
+    const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
 
     int it = 0;
     float conv = 0.f, zncc = 0.f;
@@ -349,30 +481,58 @@ void k_icgn(IcgnParams p)
             if (any) {
                 if (tid == 0) {
                     poi->result[2] = (any & 1) ? -3.f : -5.f;
-                    if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
+                    if (p.iters) p.iters[b] = (uint32_t)it;
                 }
                 return;
             }
         }
 
+        // neighbourhood columns/rows this iteration needs (within the image:
+        // every sample passed the range test)
+        const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
+        const int nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
+        const int ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
+        const int ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
+        const bool use_tile = tile_w > 0 && nx1 - nx0 < tile_w && ny1 - ny0 < tile_h;
+        if (use_tile && !(nx0 >= tx0 && nx1 < tx0 + tile_w && ny0 >= ty0 && ny1 < ty0 + tile_h)) {
+            // centre the tile on the neighbourhood, inside the image
+            tx0 = min(max(nx0 - (tile_w - (nx1 - nx0 + 1)) / 2, 0), w - tile_w);
+            ty0 = min(max(ny0 - (tile_h - (ny1 - ny0 + 1)) / 2, 0), h - tile_h);
+            if (WAVES > 1) __syncthreads();  // every wave is done with the old tile
+            stage_tile(tile, p.tar, w, tx0, ty0, tile_w, tile_h, tile_stride, tid, NT);
+            __syncthreads();
+        }
+        const int cbase = (ty0 + 1) * tile_stride + tx0 + 1;
+
         // pass 1: sample; t' per slot, and the sums of t', t'^2, t' r (a
         // repeated row counts once: its t' is masked, its r is zero)
         float tpv[SPL];
         float st = 0.f, stt = 0.f, str = 0.f;
+        auto pass1 = [&](auto sample) {
 #pragma unroll
-        for (int k = 0; k < SPL; k++) {
-            tpv[k] = 0.f;
-            if (EXACT || k < R) {
-                const float gxp = x + __builtin_fmaf((float)k, W01, bx);
-                const float gyp = y + __builtin_fmaf((float)k, W11, by);
-                const float tp = bspline_sample(table, row_bytes, plane_bytes, gxp, gyp) - ref_mean;
-                const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
-                tpv[k] = tp;
-                st += tpm;
-                stt = __builtin_fmaf(tpm, tp, stt);
-                str = __builtin_fmaf(tp, rz[k], str);
+            for (int k = 0; k < SPL; k++) {
+                tpv[k] = 0.f;
+                if (EXACT || k < R) {
+                    const float gxp = x + __builtin_fmaf((float)k, W01, bx);
+                    const float gyp = y + __builtin_fmaf((float)k, W11, by);
+                    const float tp = sample(gxp, gyp, k) - ref_mean;
+                    const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
+                    tpv[k] = tp;
+                    st += tpm;
+                    stt = __builtin_fmaf(tpm, tp, stt);
+                    str = __builtin_fmaf(tp, rz[k], str);
+                }
             }
-        }
+        };
+        if (use_tile)
+            pass1([&](float gx, float gy, int k) {
+#ifdef FI_ICGN_HYBRID
+                if (k % FI_ICGN_HYBRID == 0) return bspline_sample(table, row_bytes, plane_bytes, gx, gy);
+#endif
+                return tile_sample(tile, tile_stride, cbase, gx, gy);
+            });
+        else
+            pass1([&](float gx, float gy, int) { return bspline_sample(table, row_bytes, plane_bytes, gx, gy); });
         // lanes mirroring band 0 drop their t' sums (their r is zero)
         float s1[3] = { st * m_all, stt * m_all, str };
         block_sum<WAVES>(s1, red, wave, lane);
@@ -432,19 +592,18 @@ void k_icgn(IcgnParams p)
         const float N22 = W20 * I02 + (W21 * I12 + W22 * I22);
         W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12; W20 = N20; W21 = N21; W22 = N22;
 
-        // convergence norm (:343-352): FP32 sum in the reference's order, the
-        // square root taken in FP64 and rounded
-        float cn = dp[0] * dp[0];
-        cn = cn + dp[1] * dp[1] * rx2;
-        cn = cn + dp[2] * dp[2] * ry2;
-        cn = cn + dp[3] * dp[3];
-        cn = cn + dp[4] * dp[4] * rx2;
-        cn = cn + dp[5] * dp[5] * ry2;
-        conv = (float)sqrt((double)cn);
+        // convergence norm (:343-352), accumulated in FP64
+        double cn = (double)dp[0] * dp[0];
+        cn += (double)dp[1] * dp[1] * rx2;
+        cn += (double)dp[2] * dp[2] * ry2;
+        cn += (double)dp[3] * dp[3];
+        cn += (double)dp[4] * dp[4] * rx2;
+        cn += (double)dp[5] * dp[5] * ry2;
+        conv = (float)sqrt(cn);
     } while ((float)it < stop && conv >= conv_crit);
 
     if (tid == 0) {
-        if (p.iter_total) atomicAdd(p.iter_total, (unsigned long long)it);
+        if (p.iters) p.iters[b] = (uint32_t)it;
         float u = W02, ux = W00 - 1.f, uy = W01, v = W12, vx = W10, vy = W11 - 1.f;
         if (conv >= conv_crit && (float)it >= stop) zncc = -4.f;
         if (isnan(zncc) || isnan(u) || isnan(v)) {
@@ -480,7 +639,8 @@ void k_icgn(IcgnParams p)
 template <int WAVES, int SPL, bool EXACT>
 void launch_icgn_t(const IcgnParams& p, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), 0, s, p);
+    const size_t lds = p.tile_w > 0 ? (size_t)p.tile_stride * p.tile_h * sizeof(float) : 0;
+    hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), lds, s, p);
 }
 
 }  // namespace fi

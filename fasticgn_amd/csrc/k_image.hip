@@ -51,19 +51,6 @@ __global__ __launch_bounds__(256) void k_gradient(const float* __restrict__ img,
     gy[i] = ay;
 }
 
-// BC = B * C, src/oc_cubic_bspline.h:52-58, as the float quotients the
-// reference's initialisers produce.
-__device__ __forceinline__ float bc(int i, int j)
-{
-    constexpr float m[4][4] = {
-        { -144.0f / 336.0f, 384.0f / 336.0f, -384.0f / 336.0f, 144.0f / 336.0f },
-        { 342.0f / 336.0f, -702.0f / 336.0f, 450.0f / 336.0f, -90.0f / 336.0f },
-        { -198.0f / 336.0f, -18.0f / 336.0f, 270.0f / 336.0f, -54.0f / 336.0f },
-        { 0.0f / 336.0f, 336.0f / 336.0f, 0.0f / 336.0f, 0.0f / 336.0f },
-    };
-    return m[i][j];
-}
-
 // BicubicBspline::prepare, src/oc_cubic_bspline.cpp:84-132.  One thread per
 // texel; texels outside 1 <= r < h-2, 1 <= c < w-2 are zero.  Per texel the 16
 // values coef[k][l] = P[3-k][3-l], P[k][l] = sum_m sum_n (BC[l][m] * BC[k][n]) *

@@ -67,17 +67,68 @@ __global__ void k_seed(const fi_poi* __restrict__ tmpl, const fi_poi* __restrict
     out[i] = q;
 }
 
-__global__ void k_scan(const fi_poi* __restrict__ cur, const fi_poi* __restrict__ prev, int n,
-                       int collect, FrameCounters* counters, uint32_t* candidates)
+// Block sums of per-thread counts: one atomic per block and counter instead
+// of one per POI (same-address atomics serialise in L2).
+constexpr int kPolicyThreads = 256;
+
+template <int N>
+__device__ __forceinline__ void block_reduce(unsigned (&v)[N], unsigned (&mx)[1])
+{
+    __shared__ unsigned red[N + 1][kPolicyThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < N; q++) {
+        unsigned x = v[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+        v[q] = x;
+    }
+    unsigned m = mx[0];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = max(m, (unsigned)__shfl_xor(m, o));
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) red[q][wave] = v[q];
+        red[N][wave] = m;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; q++) {
+            unsigned t = 0;
+            for (int u = 0; u < kPolicyThreads / 64; u++) t += red[q][u];
+            v[q] = t;
+        }
+        unsigned t = 0;
+        for (int u = 0; u < kPolicyThreads / 64; u++) t = max(t, red[N][u]);
+        mx[0] = t;
+    }
+}
+
+__global__ __launch_bounds__(kPolicyThreads) void k_scan(const fi_poi* __restrict__ cur,
+                                                          const fi_poi* __restrict__ prev, int n,
+                                                          int collect, FrameCounters* counters,
+                                                          uint32_t* candidates)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const bool bad = is_bad(cur[i]);
-    if (bad) atomicAdd(&counters->bad, 1ull);
-    if (collect && is_candidate(cur[i], prev ? prev + i : nullptr, bad)) {
-        const unsigned slot = atomicAdd(&counters->candidates, 1u);
-        candidates[slot] = (uint32_t)i;
+    const bool in = i < n;
+    const bool bad = in && is_bad(cur[i]);
+    const bool cand = in && collect && is_candidate(cur[i], prev ? prev + i : nullptr, bad);
+    // candidates: one slot range per wave, in lane order
+    const unsigned long long mask = __builtin_amdgcn_ballot_w64(cand);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        unsigned base = 0;
+        if (lane == __builtin_ctzll(mask)) base = atomicAdd(&counters->candidates, (unsigned)__popcll(mask));
+        base = __shfl(base, __builtin_ctzll(mask));
+        if (cand) {
+            const unsigned below = (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+            candidates[base + below] = (uint32_t)i;
+        }
     }
+    unsigned v[1] = { bad ? 1u : 0u }, mx[1] = { 0u };
+    block_reduce<1>(v, mx);
+    if (threadIdx.x == 0 && v[0]) atomicAdd(&counters->bad, (unsigned long long)v[0]);
 }
 
 __global__ void k_gather(const fi_poi* __restrict__ src, const uint32_t* __restrict__ idx, int n,
@@ -109,23 +160,45 @@ __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restric
 
 // computeFrameQa (:629-700): valid = finite zncc >= 0; iteration statistics
 // over finite iteration counts; bad per isBadPoiForFallback.
-__global__ void k_qa(const fi_poi* __restrict__ cur, int n, FrameCounters* counters)
+__global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict__ cur, int n,
+                                                        FrameCounters* counters)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const fi_poi& p = cur[i];
-    const float z = p.result[2];
-    if (finite_f(z) && z >= 0.f) atomicAdd(&counters->valid, 1ull);
-    if (is_bad(p)) atomicAdd(&counters->bad, 1ull);
-    const float it = p.result[3];
-    if (finite_f(it)) {
-        atomicAdd(&counters->iter_finite, 1ull);
-        atomicAdd(&counters->iter_sum, (unsigned long long)(long long)it);
-        atomicMax(&counters->iter_max, (unsigned)(int)(it + 0.5f));
+    unsigned v[4] = { 0u, 0u, 0u, 0u }, mx[1] = { 0u };
+    if (i < n) {
+        const fi_poi& p = cur[i];
+        const float z = p.result[2];
+        v[0] = (finite_f(z) && z >= 0.f) ? 1u : 0u;
+        v[1] = is_bad(p) ? 1u : 0u;
+        const float it = p.result[3];
+        if (finite_f(it)) {
+            v[2] = 1u;
+            v[3] = (unsigned)(long long)it;
+            mx[0] = (unsigned)(int)(it + 0.5f);
+        }
+    }
+    block_reduce<4>(v, mx);
+    if (threadIdx.x == 0) {
+        if (v[0]) atomicAdd(&counters->valid, (unsigned long long)v[0]);
+        if (v[1]) atomicAdd(&counters->bad, (unsigned long long)v[1]);
+        if (v[2]) atomicAdd(&counters->iter_finite, (unsigned long long)v[2]);
+        if (v[3]) atomicAdd(&counters->iter_sum, (unsigned long long)v[3]);
+        if (mx[0]) atomicMax(&counters->iter_max, mx[0]);
     }
 }
 
-static dim3 grid_for(int n) { return dim3((n + 255) / 256); }
+// Sum of per-entry executed iteration counts into *total.
+__global__ __launch_bounds__(kPolicyThreads) void k_sum_iters(const uint32_t* __restrict__ iters, int n,
+                                                               unsigned long long* total)
+{
+    unsigned long long acc = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += iters[i];
+    unsigned v[1] = { (unsigned)acc }, mx[1] = { 0u };
+    block_reduce<1>(v, mx);
+    if (threadIdx.x == 0 && v[0]) atomicAdd(total, (unsigned long long)v[0]);
+}
+
+static dim3 grid_for(int n) { return dim3((n + kPolicyThreads - 1) / kPolicyThreads); }
 
 void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, fi_poi* out, int n,
                                hipStream_t s)
@@ -137,7 +210,7 @@ void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
                  FrameCounters* counters, uint32_t* candidates, hipStream_t s)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_scan, grid_for(n), dim3(256), 0, s, cur, prev, n, collect, counters,
+        hipLaunchKernelGGL(k_scan, grid_for(n), dim3(kPolicyThreads), 0, s, cur, prev, n, collect, counters,
                            candidates);
 }
 
@@ -161,7 +234,14 @@ void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* id
 
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_qa, grid_for(n), dim3(256), 0, s, cur, n, counters);
+    if (n > 0) hipLaunchKernelGGL(k_qa, grid_for(n), dim3(kPolicyThreads), 0, s, cur, n, counters);
+}
+
+void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_sum_iters, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, 1024)),
+                           dim3(kPolicyThreads), 0, s, iters, n, total);
 }
 
 }  // namespace fi

@@ -197,6 +197,14 @@ class ICGN2D1(_Handle):
         band (0, 0 = default)."""
         nat.call("fi_icgn_set_shape", self._h, int(waves), int(rows_per_band))
 
+    def setSampling(self, mode: str) -> None:
+        """'auto' (raw neighbourhood staged in LDS, table fallback) or 'table'
+        (every sample from the B-spline table)."""
+        modes = {"auto": nat.FI_SAMPLING_AUTO, "table": nat.FI_SAMPLING_TABLE}
+        if mode not in modes:
+            raise ValueError(f"sampling must be one of {sorted(modes)}")
+        nat.call("fi_icgn_set_sampling", self._h, modes[mode])
+
     def setImages(self, ref: Image2D, tar: Image2D) -> None:
         self.setRefImage(ref)
         self.setTarImage(tar)

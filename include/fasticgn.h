@@ -142,6 +142,16 @@ int fi_icgn_set_self_adaptive(fi_icgn* h, int on);                   /* src/oc_d
  * No reference counterpart (the CUDA engine picks its block size by occupancy,
  * cu:331-359).  Results depend on the shape only through FP32 summation order. */
 int fi_icgn_set_shape(fi_icgn* h, int waves, int rows_per_band);
+/* Where the IC-GN kernel reads the target: FI_SAMPLING_AUTO stages each
+ * subset's raw neighbourhood in LDS and evaluates the B-spline polynomial from
+ * the 16 pixels around a sample (the B-spline table serves iterations whose
+ * warp outgrows the staged tile); FI_SAMPLING_TABLE reads every sample's 16
+ * coefficients from the table (src/oc_cubic_bspline.cpp:134-181).  The two
+ * evaluate the same polynomial and differ only in FP32 rounding.  No reference
+ * counterpart. */
+#define FI_SAMPLING_AUTO 0
+#define FI_SAMPLING_TABLE 1
+int fi_icgn_set_sampling(fi_icgn* h, int mode);
 int fi_icgn_set_ref(fi_icgn* h, const void* px, int width, int height,
                     size_t stride_bytes, int pixel, int depth12);
 int fi_icgn_set_tar(fi_icgn* h, const void* px, int width, int height,
@@ -177,6 +187,8 @@ int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
 int fi_frame_reset_sequence(fi_frame* h);
 /* IC-GN lane-to-POI shape for this driver (see fi_icgn_set_shape). */
 int fi_frame_set_shape(fi_frame* h, int waves, int rows_per_band);
+/* IC-GN target sampling for this driver (see fi_icgn_set_sampling). */
+int fi_frame_set_sampling(fi_frame* h, int mode);
 
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */

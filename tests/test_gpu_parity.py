@@ -114,9 +114,10 @@ def test_fftcc_empty_queue(c0):
 
 # ------------------------------------------------------------------- ICGN --
 
-def run_icgn_gpu(ref, tar, radius, q, conv=0.001, stop=10.0, self_adaptive=False):
+def run_icgn_gpu(ref, tar, radius, q, conv=0.001, stop=10.0, self_adaptive=False, sampling="auto"):
     g = q.copy()
     with fa.ICGN2D1(radius, radius, conv, stop) as ic:
+        ic.setSampling(sampling)
         if self_adaptive:
             ic.setSelfAdaptive(True)
         ic.setImages(fa.Image2D(ref), fa.Image2D(tar))
@@ -227,6 +228,63 @@ def test_icgn_iteration_cap_and_conv(oracle, c0):
     report("icgn cap", cmp)
     assert (o["result"][:, 2] == -4).sum() > 100
     assert cmp["status_mismatch"] == 0
+
+
+def test_icgn_table_sampling_parity(oracle, c0):
+    """Every sample from the B-spline table (the fallback path) against the
+    oracle, and against the default LDS-tile path, which evaluates the same
+    polynomial from raw pixels."""
+    ref, tar = c0.ref, c0.tar
+    seeds = c0.pois.copy()
+    oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), 15, seeds)
+    g_tab = run_icgn_gpu(ref, tar, 15, seeds, sampling="table")
+    g_lds = run_icgn_gpu(ref, tar, 15, seeds)
+    o = seeds.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o)
+    cmp = compare(o, g_tab)
+    report("icgn table sampling vs oracle", cmp)
+    assert cmp["pass"], cmp
+    both = compare(g_tab, g_lds)
+    report("icgn table vs LDS tile", both)
+    assert both["status_mismatch"] == 0 and max(both["max_du"], both["max_dv"]) < 1e-3
+
+
+def test_icgn_large_strain_uses_table_fallback(oracle):
+    """20 % strain: the warped subset outgrows the LDS tile, so iterations fall
+    back to the table; parity holds."""
+    sc = synth.affine_scene("strain", 384, 384, 3400, 7, t=(1.2, -0.7), grad=(0.2, 0.0, 0.0, 0.2),
+                            radius=15, grid=16)
+    q = sc.pois.copy()
+    u, v, ux, uy, vx, vy = sc.truth(q["x"], q["y"])
+    q["deformation"][:, 0] = np.round(u)
+    q["deformation"][:, 6] = np.round(v)
+    q["deformation"][:, 1] = 0.2
+    q["deformation"][:, 8] = 0.2
+    g = run_icgn_gpu(sc.ref, sc.tar, 15, q)
+    o = q.copy()
+    oracle.icgn(sc.ref.astype(np.float32), sc.tar.astype(np.float32), 15, o)
+    cmp = compare(o, g)
+    report("icgn 20% strain", cmp)
+    assert cmp["pass"], cmp
+    assert finite_valid(g).mean() > 0.5
+
+
+def test_icgn_far_seed_restages_tile(oracle, c0):
+    """Seeds 2.4 px from the solution move the subset past the tile's margin
+    during the iterations, so the tile is staged again; parity holds."""
+    ref, tar = c0.ref, c0.tar
+    q = c0.pois.copy()
+    u, v, *_ = c0.truth(q["x"], q["y"])
+    q["deformation"][:, 0] = u + 2.4
+    q["deformation"][:, 6] = v - 2.4
+    g = run_icgn_gpu(ref, tar, 15, q, stop=20)
+    o = q.copy()
+    oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o, stop=20)
+    cmp = compare(o, g)
+    report("icgn far seeds", cmp)
+    assert cmp["pass"], cmp
+    moved = finite_valid(g) & (np.abs(g["deformation"][:, 0] - q["deformation"][:, 0]) > 2.0)
+    assert moved.sum() > 100
 
 
 def test_icgn_deterministic(c0):
