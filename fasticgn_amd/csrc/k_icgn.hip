@@ -264,7 +264,8 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 
 // The per-POI LDS tile: the subset's neighbourhood (2r+1 + 3 pixels) plus
 // kTileMargin on each side, when the image is that large and the f32 target
-// is given; an odd row stride spreads the bands over the LDS banks.
+// is given.  (A wave64 two-dword LDS read occupies the 32 banks for four
+// cycles whatever the row stride; padding the stride buys nothing.)
 static IcgnParams with_tile(const IcgnParams& p)
 {
     IcgnParams q = p;
@@ -274,7 +275,7 @@ static IcgnParams with_tile(const IcgnParams& p)
     if (tw > p.w || th > p.h) return q;
     q.tile_w = tw;
     q.tile_h = th;
-    q.tile_stride = tw | 1;
+    q.tile_stride = tw;
     return q;
 }
 

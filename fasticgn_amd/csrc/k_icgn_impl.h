@@ -525,12 +525,7 @@ void k_icgn(IcgnParams p)
             }
         };
         if (use_tile)
-            pass1([&](float gx, float gy, int k) {
-#ifdef FI_ICGN_HYBRID
-                if (k % FI_ICGN_HYBRID == 0) return bspline_sample(table, row_bytes, plane_bytes, gx, gy);
-#endif
-                return tile_sample(tile, tile_stride, cbase, gx, gy);
-            });
+            pass1([&](float gx, float gy, int) { return tile_sample(tile, tile_stride, cbase, gx, gy); });
         else
             pass1([&](float gx, float gy, int) { return bspline_sample(table, row_bytes, plane_bytes, gx, gy); });
         // lanes mirroring band 0 drop their t' sums (their r is zero)
