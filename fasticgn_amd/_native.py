@@ -28,6 +28,7 @@ FI_POLICY_SKIP_FFTCC = 1
 FI_POLICY_POI_FALLBACK = 2
 FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
 FI_SAMPLING_AUTO, FI_SAMPLING_TABLE = 0, 1
+FI_POLICY_LEARNED_SEED = 4
 
 # opencorr::POI2D, src/oc_poi.h:102-108: 25 float32 = 100 bytes
 POI_DTYPE = np.dtype([
@@ -62,6 +63,9 @@ class FrameStats(ctypes.Structure):
         ("bspline_ms", ctypes.c_double),
         ("fftcc_ms", ctypes.c_double),
         ("icgn_ms", ctypes.c_double),
+        ("seed_pois", ctypes.c_int64),
+        ("seed_fallback_count", ctypes.c_int64),
+        ("seed_ms", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -129,6 +133,16 @@ SIGNATURES = {
     "fi_frame_reset_sequence": [_P],
     "fi_frame_set_shape": [_P, _I, _I],
     "fi_frame_set_sampling": [_P, _I],
+    "fi_frame_set_seeder": [_P, _P, _F],
+    "fi_seeder_create": [_I, ctypes.POINTER(_P)],
+    "fi_seeder_destroy": [_P],
+    "fi_seeder_param_count": [ctypes.POINTER(_S)],
+    "fi_seeder_set_params": [_P, _P, _S],
+    "fi_seeder_get_params": [_P, _P, _S],
+    "fi_seeder_set_training_data": [_P, _P, _P, _S],
+    "fi_seeder_train_epoch": [_P, _I, _F, ctypes.c_uint64, ctypes.POINTER(_F)],
+    "fi_seeder_forward": [_P, _P, _S, _P],
+    "fi_seeder_seed": [_P, _P, _P, _I, _I, _P, _S, _P],
     "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
 }
 

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "fi_engines.h"
+#include "fi_seeder_engine.h"
 
 namespace fi {
 const std::string& last_error();
@@ -301,6 +302,80 @@ int fi_frame_set_sampling(fi_frame* h, int mode)
         check_sampling(mode);
         h->sampling = mode;
     });
+}
+
+int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold)
+{
+    return guarded([&] {
+        need(h);
+        if (!(zncc_threshold <= 1.f)) fail(FI_E_INVALID_ARGUMENT, "seed ZNCC threshold must be <= 1");
+        if (seeder && seeder->device != h->device)
+            fail(FI_E_INVALID_ARGUMENT, "seeder and frame driver are on different devices");
+        h->seeder = seeder;
+        h->seed_threshold = zncc_threshold;
+    });
+}
+
+// ---- learned seeder ----
+int fi_seeder_create(int device, fi_seeder** out)
+{
+    return guarded([&] {
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *out = nullptr;
+        *out = new fi_seeder(device);
+    });
+}
+
+int fi_seeder_destroy(fi_seeder* h)
+{
+    return guarded([&] { delete h; });
+}
+
+int fi_seeder_param_count(size_t* n)
+{
+    return guarded([&] {
+        if (!n) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *n = seed_param_count();
+    });
+}
+
+int fi_seeder_set_params(fi_seeder* h, const float* params, size_t n)
+{
+    return guarded([&] {
+        need(h);
+        if (!params) fail(FI_E_INVALID_ARGUMENT, "null parameters");
+        h->set_params(params, n);
+    });
+}
+
+int fi_seeder_get_params(fi_seeder* h, float* params, size_t n)
+{
+    return guarded([&] {
+        need(h);
+        if (!params) fail(FI_E_INVALID_ARGUMENT, "null output");
+        h->get_params(params, n);
+    });
+}
+
+int fi_seeder_set_training_data(fi_seeder* h, const uint16_t* x1, const float* targets, size_t n)
+{
+    return guarded([&] { need(h); h->set_training_data(x1, targets, n); });
+}
+
+int fi_seeder_train_epoch(fi_seeder* h, int batch, float lr, uint64_t seed, float* mean_loss)
+{
+    return guarded([&] { need(h); h->train_epoch(batch, lr, seed, mean_loss); });
+}
+
+int fi_seeder_forward(fi_seeder* h, const uint16_t* x1, size_t n, float* out)
+{
+    return guarded([&] { need(h); h->forward(x1, n, out); });
+}
+
+int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, int height, fi_poi* pois, size_t n,
+                   uint32_t* ok)
+{
+    return guarded([&] { need(h); h->seed(ref, tar, width, height, pois, n, ok); });
 }
 
 // ---- raw record, opencorr_cli_cuda_mvp.cpp:2498-2536 ----

@@ -1,5 +1,6 @@
 // Engine implementations: host orchestration of the gfx950 kernels.
 #include "fi_engines.h"
+#include "fi_seeder_engine.h"
 
 #include <climits>
 #include <cstring>
@@ -524,6 +525,7 @@ void fi_frame::prepare()
     const size_t cap = std::max<size_t>(n, 1);
     tmpl.reserve(cap);
     iters.reserve(cap);
+    seed_ok.reserve(cap);
     cur.reserve(cap);
     prev.reserve(cap);
     save.reserve(cap);
@@ -643,7 +645,32 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     const bool skip = (policy & FI_POLICY_SKIP_FFTCC) && baseline_ready && have_prev;
     float bad_ratio = 0.f;
     double fftcc_ms = 0.0, icgn_ms = 0.0;
-    if (!skip) {
+    const bool learned = !skip && (policy & FI_POLICY_LEARNED_SEED) && seeder != nullptr;
+    if (learned) {
+        // learned seeds (fused inference), IC-GN, FFTCC fallback for the POIs
+        // whose result has a low ZNCC or failed, per-POI merge
+        FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+        seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, seed_ok.p, stream);
+        FI_HIP(hipEventRecord(ev[2], stream));
+        icgn_pass(cur.p, cnt, nullptr);
+        FI_HIP(hipEventRecord(ev[3], stream));
+        zero_counters();
+        launch_seed_scan(cur.p, seed_ok.p, cnt, seed_threshold, counters.p, cand.p, stream);
+        FI_HIP(hipGetLastError());
+        const FrameCounters c = read_counters();
+        st.seed_pois = cnt;
+        st.seed_fallback_count = c.candidates;
+        if (c.candidates > 0) {
+            const int m = (int)c.candidates;
+            launch_gather(tmpl.p, cand.p, m, repair.p, stream);
+            fftcc_pass(repair.p, m, cand.p);
+            icgn_pass(repair.p, m, cand.p);
+            zero_counters();
+            launch_merge_repair(cur.p, repair.p, cand.p, m, counters.p, stream);
+            FI_HIP(hipGetLastError());
+        }
+        st.fftcc_pois = c.candidates;
+    } else if (!skip) {
         FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
         fftcc_pass(cur.p, cnt, nullptr);
         FI_HIP(hipEventRecord(ev[2], stream));
@@ -718,7 +745,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     st.bspline_ms = ms;
     FI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
-    fftcc_ms = ms;
+    if (learned) st.seed_ms = ms;
+    else fftcc_ms = ms;
     FI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
     icgn_ms = ms;
     st.fftcc_ms = fftcc_ms;

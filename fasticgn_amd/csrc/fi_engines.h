@@ -84,6 +84,8 @@ struct fi_frame {
     float conv = 0.001f, stop = 10.f;
     int shape_waves = 0, shape_spl = 0;
     int sampling = FI_SAMPLING_AUTO;
+    fi_seeder* seeder = nullptr;   // learned seeder (FI_POLICY_LEARNED_SEED)
+    float seed_threshold = 0.8f;   // ZNCC below which a learned seed falls back to FFTCC
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw, tar;
     fi::DevArray<float> ref_f, gx, gy, coef, tw, spectra, tar_f;
@@ -91,7 +93,7 @@ struct fi_frame {
     fi::DevArray<fi::PoiPrep> prep;
     std::vector<fi_poi> tmpl_host;
     fi::DevArray<fi_poi> tmpl, cur, prev, save, repair;
-    fi::DevArray<uint32_t> cand;
+    fi::DevArray<uint32_t> cand, seed_ok;
     fi::DevArray<fi::FrameCounters> counters;
     fi::PinnedArray<fi::FrameCounters> h_counters;
     fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry

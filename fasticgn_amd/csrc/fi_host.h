@@ -119,6 +119,9 @@ void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int 
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
                          FrameCounters* counters, hipStream_t s);
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s);
+// learned-seed fallback candidates (see k_policy.hip)
+void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float threshold, FrameCounters* counters,
+                      uint32_t* candidates, hipStream_t s);
 // *total += sum of iters[0..n)
 void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s);
 

@@ -131,6 +131,29 @@ __global__ __launch_bounds__(kPolicyThreads) void k_scan(const fi_poi* __restric
     if (threadIdx.x == 0 && v[0]) atomicAdd(&counters->bad, (unsigned long long)v[0]);
 }
 
+// Learned-seed fallback candidates: not seeded, or an IC-GN result that is not
+// a finite ZNCC at or above the threshold (failures -3/-4/-5 included).
+__global__ __launch_bounds__(kPolicyThreads) void k_seed_scan(const fi_poi* __restrict__ cur,
+                                                               const uint32_t* __restrict__ ok, int n,
+                                                               float threshold, FrameCounters* counters,
+                                                               uint32_t* candidates)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool cand = false;
+    if (i < n) {
+        const float z = cur[i].result[2];
+        cand = ok[i] == 0u || !(finite_f(z) && z >= threshold);
+    }
+    const unsigned long long mask = __builtin_amdgcn_ballot_w64(cand);
+    if (mask) {
+        const int lane = threadIdx.x & 63;
+        unsigned base = 0;
+        if (lane == __builtin_ctzll(mask)) base = atomicAdd(&counters->candidates, (unsigned)__popcll(mask));
+        base = __shfl(base, __builtin_ctzll(mask));
+        if (cand) candidates[base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull))] = (uint32_t)i;
+    }
+}
+
 __global__ void k_gather(const fi_poi* __restrict__ src, const uint32_t* __restrict__ idx, int n,
                          fi_poi* __restrict__ dst)
 {
@@ -211,6 +234,14 @@ void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
 {
     if (n > 0)
         hipLaunchKernelGGL(k_scan, grid_for(n), dim3(kPolicyThreads), 0, s, cur, prev, n, collect, counters,
+                           candidates);
+}
+
+void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float threshold, FrameCounters* counters,
+                      uint32_t* candidates, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_seed_scan, grid_for(n), dim3(kPolicyThreads), 0, s, cur, ok, n, threshold, counters,
                            candidates);
 }
 

@@ -84,11 +84,15 @@ typedef struct fi_frame_stats {
     double bspline_ms;            /* target B-spline table */
     double fftcc_ms;              /* FFTCC seeding (or temporal seed copy) */
     double icgn_ms;               /* first IC-GN pass over the frame */
+    int64_t seed_pois;            /* POIs seeded by the learned seeder */
+    int64_t seed_fallback_count;  /* of those, re-run from FFTCC (low ZNCC or failure) */
+    double seed_ms;               /* learned-seeder inference */
 } fi_frame_stats;
 
 typedef struct fi_fftcc fi_fftcc;
 typedef struct fi_icgn fi_icgn;
 typedef struct fi_frame fi_frame;
+typedef struct fi_seeder fi_seeder;
 
 /* ---- library ------------------------------------------------------------ */
 int fi_version(int* major, int* minor, int* patch);
@@ -189,6 +193,42 @@ int fi_frame_reset_sequence(fi_frame* h);
 int fi_frame_set_shape(fi_frame* h, int waves, int rows_per_band);
 /* IC-GN target sampling for this driver (see fi_icgn_set_sampling). */
 int fi_frame_set_sampling(fi_frame* h, int mode);
+
+/* ---- learned seeder --------------------------------------------------------
+ * No reference counterpart: the seeder that replaces FFTCC2D as the IC-GN
+ * initial guess.  A four-layer patchify CNN maps a reference/target crop pair
+ * (32x32 each around the POI, the target crop shifted by the POI's rounded
+ * u, v) to (u, v, ux, uy, vx, vy); BF16 MFMA with FP32 accumulation, trained on
+ * the device by fi_seeder_train_epoch.  The parameter vector (FP32) is
+ * W1 (32x16), b1, W2 (64x32), b2, W3 (128x64), b3, W4 (256x128), b4,
+ * W5 (128x6), b5; outputs are (u, v) in pixels and gradients x 100.
+ * Layer-1 input rows (BF16 bit patterns, 64 rows of 32 per sample) are
+ * patch (py, px) of the normalised crops, column c*16 + dy*4 + dx. */
+int fi_seeder_create(int device, fi_seeder** out);
+int fi_seeder_destroy(fi_seeder* h);
+int fi_seeder_param_count(size_t* n);
+int fi_seeder_set_params(fi_seeder* h, const float* params, size_t n);
+int fi_seeder_get_params(fi_seeder* h, float* params, size_t n);
+/* Training set: n samples of layer-1 rows (n*64*32 BF16) and scaled targets
+ * (n*6 FP32), kept on the device. */
+int fi_seeder_set_training_data(fi_seeder* h, const uint16_t* x1, const float* targets, size_t n);
+/* One epoch of Adam over a seeded shuffle in batches (multiple of 16; the
+ * remainder of the shuffle is dropped).  mean_loss may be NULL. */
+int fi_seeder_train_epoch(fi_seeder* h, int batch, float lr, uint64_t seed, float* mean_loss);
+/* Network outputs (n*6, scaled) for layer-1 rows, through the training
+ * kernels. */
+int fi_seeder_forward(fi_seeder* h, const uint16_t* x1, size_t n, float* out);
+/* Seed POIs in place from f32 images (the fused inference kernel); ok[i] = 1
+ * where both crops lie inside the images (others untouched).  ok may be NULL. */
+int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, int height, fi_poi* pois,
+                   size_t n, uint32_t* ok);
+
+/* Frame driver with the learned seeder (policy FI_POLICY_LEARNED_SEED): the
+ * FFTCC seeding of a frame is replaced by the seeder; POIs whose IC-GN result
+ * has ZNCC below zncc_threshold (or failed, or could not be seeded) are re-run
+ * from FFTCC and merged (better ZNCC wins).  The seeder must outlive its use. */
+#define FI_POLICY_LEARNED_SEED 4u
+int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold);
 
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */

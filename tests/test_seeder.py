@@ -1,0 +1,172 @@
+"""Learned seeder: the CPU forward restatement pinned by construction, the
+crop layout, and the device kernels (training forward, fused inference,
+training) against the restatement."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import fasticgn_amd as fa
+from fasticgn_amd import _native as nat
+from fasticgn_amd import seeder as sd
+from fasticgn_amd import synth
+from oracle import seeder_ref
+
+
+def random_params(seed=3, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(sd.param_count(), dtype=np.float32)
+    o = 0
+    for k, n in sd.LAYERS + [(128, 6)]:
+        p[o:o + k * n] = rng.normal(0, scale * np.sqrt(2.0 / k), k * n)
+        o += k * n
+        p[o:o + n] = rng.normal(0, 0.1, n)
+        o += n
+    return p
+
+
+# ------------------------------------------------------------------ CPU --
+
+def test_param_count_matches_library():
+    n = ctypes.c_size_t(0)
+    nat.call("fi_seeder_param_count", ctypes.byref(n))
+    assert n.value == sd.param_count() == 44534
+
+
+def test_bf16_rounding_nearest_even():
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 3.14159265, 0.0], dtype=np.float32)
+    r = seeder_ref.bf16_round(x)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: to even (1.0); 1 + 3*2^-9 rounds up
+    assert list(r[:4]) == [1.0, 1.0, 1.0 + 2 ** -7, -2.5]
+    assert abs(r[4] - 3.140625) < 1e-7 and r[5] == 0.0
+    np.testing.assert_array_equal(sd.bf16_value(sd.bf16_bits(x)), r)
+
+
+def test_merge_patches_by_hand():
+    n, g, c = 2, 4, 3
+    y = np.arange(n * g * g * c, dtype=np.float32).reshape(n * g * g, c)
+    m = seeder_ref.merge_patches(y, n, g)
+    for s in range(n):
+        for py in range(g):
+            for px in range(g):
+                row = s * (g // 2) ** 2 + (py // 2) * (g // 2) + px // 2
+                q = (py % 2) * 2 + px % 2
+                np.testing.assert_array_equal(m[row, q * c:(q + 1) * c], y[s * g * g + py * g + px])
+
+
+def test_forward_identity_layers_and_linear_head():
+    """With identity-like layers the network reduces to a closed form: layer 1
+    copies input column j into channel j (j < 16), later layers copy channels,
+    the head sums what reaches it."""
+    p = np.zeros(sd.param_count(), dtype=np.float32)
+    layers = seeder_ref.unpack(p)
+    w1 = layers[0][0]
+    for j in range(16):
+        w1[j, j] = 1.0
+    for l in (1, 2, 3):
+        w = layers[l][0]
+        for j in range(w.shape[1]):
+            w[j, j] = 1.0
+    w5, b5 = layers[4]
+    w5[:, 0] = 1.0
+    b5[1] = 0.5
+    rng = np.random.default_rng(0)
+    x = rng.normal(0, 1, (3, 64, 32)).astype(np.float32)
+    bits = sd.bf16_bits(x)
+    out = seeder_ref.forward(p, bits)
+    xv = sd.bf16_value(bits).reshape(3, 64, 32)
+    # layer 1 keeps max(x[:, :, j], 0) for j < 16; layer 2 keeps channels 0..31 of
+    # the merged rows = channels 0..15 of patches (0,0) and (0,1); layer 3 keeps
+    # 0..63 = patch blocks (0,0),(0,1),(1,0),(1,1) x 16 channels... the chain
+    # keeps exactly the channels j < 16 of layer-1 patch (0, 0) of each 2x2 group
+    # that survives: check the head sum against a direct evaluation.
+    y1 = np.maximum(xv[:, :, :16], 0).reshape(3 * 64, 16)
+    x2 = seeder_ref.merge_patches(y1, 3, 8)[:, :32]
+    x3 = seeder_ref.merge_patches(np.maximum(x2, 0), 3, 4)[:, :64]
+    x4 = seeder_ref.merge_patches(np.maximum(x3, 0), 3, 2)[:, :128]
+    expect0 = np.maximum(x4, 0).sum(axis=1)
+    np.testing.assert_allclose(out[:, 0], expect0, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(out[:, 1], 0.5)
+
+
+def test_crop_rows_layout():
+    rng = np.random.default_rng(1)
+    ref = rng.uniform(0, 255, (80, 96)).astype(np.float32)
+    tar = rng.uniform(0, 255, (80, 96)).astype(np.float32)
+    q = fa.opencorr.poi_queue([(40.0, 36.0), (20.0, 20.0), (3.0, 40.0)])
+    q["deformation"][:, 0] = [2.4, -1.6, 0.0]
+    q["deformation"][:, 6] = [-3.5, 0.4, 0.0]
+    x1, ok, guess = sd.crop_rows(ref, tar, q)
+    assert list(ok) == [True, True, False]
+    np.testing.assert_array_equal(guess[0], [2.0, -4.0])  # rint, half to even
+    for s in (0, 1):
+        x0, y0 = int(q["x"][s]) - 15, int(q["y"][s]) - 15
+        crops = []
+        for c, (img, dx0, dy0) in enumerate(((ref, 0, 0), (tar, int(guess[s, 0]), int(guess[s, 1])))):
+            cr = img[y0 + dy0:y0 + dy0 + 32, x0 + dx0:x0 + dx0 + 32].astype(np.float64)
+            crops.append((cr - cr.mean()) / np.sqrt(cr.var() + 1e-6))
+        v = sd.bf16_value(x1[s])
+        for py, px, c, dy, dx in [(0, 0, 0, 0, 0), (3, 5, 1, 2, 3), (7, 7, 1, 3, 3), (2, 6, 0, 1, 2)]:
+            want = crops[c][4 * py + dy, 4 * px + dx]
+            assert abs(v[py * 8 + px, c * 16 + dy * 4 + dx] - want) <= 1e-2 * max(1.0, abs(want))
+    assert not x1[2].any()
+
+
+# ------------------------------------------------------------------ GPU --
+
+@pytest.mark.gpu
+def test_training_forward_matches_oracle():
+    p = random_params()
+    rng = np.random.default_rng(7)
+    x1 = sd.bf16_bits(rng.normal(0, 1, (300, 64, 32)).astype(np.float32))
+    with sd.LearnedSeeder(p) as s:
+        got = s.forward(x1)
+        np.testing.assert_array_equal(s.get_params(), p)
+    want = seeder_ref.forward(p, x1)
+    err = np.abs(got - want) / (np.abs(want) + 1.0)
+    print("forward max rel err", float(err.max()), flush=True)
+    assert err.max() < 2e-2 and np.median(err) < 2e-3
+
+
+@pytest.mark.gpu
+def test_fused_inference_matches_oracle():
+    sc = synth.config0()
+    ref, tar = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
+    q = sc.pois.copy()
+    q["deformation"][:, 0] = 2.0
+    q["deformation"][:, 6] = -2.0
+    q[::17]["x"] = 3.0  # crop outside the image: left untouched
+    p = random_params(5, 0.5)
+    g = q.copy()
+    with sd.LearnedSeeder(p) as s:
+        ok = s.seed(ref, tar, g)
+    x1, ok_ref, guess = sd.crop_rows(ref, tar, q)
+    np.testing.assert_array_equal(ok, ok_ref)
+    out = seeder_ref.forward(p, x1[ok_ref])
+    d = g["deformation"][ok]
+    got = np.stack([d[:, 0] - guess[ok, 0], d[:, 6] - guess[ok, 1], 100 * d[:, 1], 100 * d[:, 2],
+                    100 * d[:, 7], 100 * d[:, 8]], axis=1)
+    err = np.abs(got - out) / (np.abs(out) + 1.0)
+    print("fused max rel err", float(err.max()), "median", float(np.median(err)), flush=True)
+    assert np.median(err) < 5e-3 and np.percentile(err, 99) < 5e-2
+    np.testing.assert_array_equal(g[~ok], q[~ok])
+
+
+@pytest.mark.gpu
+def test_training_reduces_loss():
+    """A linear-ish target (u = mean of input channel 0's first rows) is
+    learnable: ten epochs on 4096 samples cut the loss."""
+    rng = np.random.default_rng(11)
+    x = rng.normal(0, 1, (4096, 64, 32)).astype(np.float32)
+    bits = sd.bf16_bits(x)
+    xv = sd.bf16_value(bits)
+    t = np.zeros((4096, 6), dtype=np.float32)
+    t[:, 0] = xv[:, :8, 0].mean(axis=1) * 3
+    t[:, 1] = xv[:, 8:16, 16].mean(axis=1) * 3
+    with sd.LearnedSeeder(random_params(2)) as s:
+        s.set_training_data(bits, t)
+        first = s.train_epoch(256, 2e-3, 1)
+        for e in range(9):
+            last = s.train_epoch(256, 2e-3, 2 + e)
+    print("loss", first, "->", last, flush=True)
+    assert last < 0.5 * first
