@@ -57,6 +57,9 @@ fi_seeder::fi_seeder(int dev) : device(dev)
     adam_m.reserve(np);
     adam_v.reserve(np);
     for (int l = 0; l < kSeedLayers; l++) wt[l].reserve((size_t)kSeedK[l] * kSeedN[l]);
+    frag.reserve(seed_frag_elems());
+    w5f.reserve(seed_w5f_elems());
+    FI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
     // He-style initialisation, seeded (a fresh seeder is usable but untrained)
     std::vector<float> host(np, 0.f);
     std::mt19937 rng(20260);
@@ -76,8 +79,12 @@ fi_seeder::~fi_seeder()
 
 void fi_seeder::refresh_bf16()
 {
-    for (int l = 0; l < kSeedLayers; l++)
+    const float* w[kSeedLayers];
+    for (int l = 0; l < kSeedLayers; l++) {
         launch_transpose_bf16(params.p + offset_w(l), kSeedK[l], kSeedN[l], wt[l].p, stream);
+        w[l] = params.p + offset_w(l);
+    }
+    launch_pack_infer(w, params.p + offset_w5(), frag.p, w5f.p, stream);
     FI_HIP(hipGetLastError());
 }
 
@@ -88,6 +95,8 @@ SeedWeights fi_seeder::weights() const
         w.wt[l] = wt[l].p;
         w.b[l] = params.p + offset_b(l);
     }
+    w.frag = frag.p;
+    w.w5f = w5f.p;
     w.w5 = params.p + offset_w5();
     w.b5 = params.p + offset_b5();
     return w;
@@ -299,6 +308,7 @@ void fi_seeder::run_seed(const float* ref, const float* tar, int w, int h, fi_po
     sp.n = n;
     sp.wts = weights();
     sp.ok = ok;
+    sp.cus = cus;
     launch_seed_infer(sp, s);
     FI_HIP(hipGetLastError());
 }

@@ -11,9 +11,11 @@
 //   4      1x1   1            256  128
 //   head                      128  6     (FP32, no activation)
 //
-// GEMMs run on BF16 MFMA with FP32 accumulation (v_mfma_f32_16x16x16_bf16);
-// activations and weights are rounded to BF16 (round to nearest even) before
-// each GEMM.  Outputs are scaled: (u, v) in pixels, gradients x kGradScale.
+// GEMMs run on BF16 MFMA with FP32 accumulation (training:
+// v_mfma_f32_16x16x16_bf16; fused inference: v_mfma_f32_16x16x32_bf16, head
+// on v_mfma_f32_16x16x4_f32); activations and weights are rounded to BF16
+// (round to nearest even) before each GEMM.  Outputs are scaled: (u, v) in
+// pixels, gradients x kGradScale.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,14 +44,21 @@ constexpr size_t seed_param_count()
 }
 
 // Device weights for the kernels: BF16 copies of W1..W4 transposed to
-// N x K (so an MFMA B fragment is four consecutive K values), FP32 biases,
-// FP32 head.
+// N x K (training GEMMs), the packed MFMA fragments of the fused inference
+// kernel (k_pack_infer), FP32 biases, FP32 head.
 struct SeedWeights {
     const uint16_t* wt[kSeedLayers];
+    const uint16_t* frag;  // seed_frag_elems() BF16
+    const float* w5f;      // seed_w5f_elems() FP32
     const float* b[kSeedLayers];
     const float* w5;  // 128 x 6
     const float* b5;  // 6
 };
+
+size_t seed_frag_elems();
+size_t seed_w5f_elems();
+void launch_pack_infer(const float* const w[kSeedLayers], const float* w5, uint16_t* frag, float* w5f,
+                       hipStream_t s);
 
 struct SeedInferParams {
     const float* ref;  // f32 row-major w x h
@@ -60,12 +69,9 @@ struct SeedInferParams {
     int n;
     SeedWeights wts;
     uint32_t* ok;      // per POI 1 = seeded, 0 = crop outside the image (untouched)
+    int cus;           // compute units of the device (persistent grid size)
 };
 
 void launch_seed_infer(const SeedInferParams& p, hipStream_t s);
-
-// Forward pass on pre-made layer-1 inputs (rows = samples x 64, BF16), for
-// tests: out[n x 6] (scaled outputs).
-void launch_seed_forward_rows(const uint16_t* x1, int n, const SeedWeights& w, float* out, hipStream_t s);
 
 }  // namespace fi

@@ -10,7 +10,9 @@ struct fi_seeder {
     hipStream_t stream = nullptr;
     fi::DevArray<float> params, grads, adam_m, adam_v;
     int adam_t = 0;
-    fi::DevArray<uint16_t> wt[fi::kSeedLayers];
+    fi::DevArray<uint16_t> wt[fi::kSeedLayers], frag;
+    fi::DevArray<float> w5f;
+    int cus = 0;
     // training set
     fi::DevArray<uint16_t> data_x;
     fi::DevArray<float> data_t;

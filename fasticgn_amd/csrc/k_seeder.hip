@@ -15,6 +15,8 @@
 //   D (16 x 16):                  lane holds D[4 (l / 16) + j][l % 16]
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fi_device.h"
 #include "fi_seeder.h"
 
@@ -61,194 +63,258 @@ __device__ __forceinline__ void patchify(int s, int p, int g, int n, int c, int&
 // ---------------------------------------------------------------------------
 // Fused inference
 // ---------------------------------------------------------------------------
-constexpr int kInferPois = 16;
-constexpr int kInferThreads = 256;
+// One wave seeds 16 POIs per pass with every GEMM transposed: D (channels x
+// POIs) = W^T (channels x depth) X^T (depth x POIs), POI s in MFMA column s
+// (lane l works on POI l % 16).  The crop quadtree is walked depth first
+// (layer-3 position q3, layer-2 position q2, layer-1 patch q1), so a layer's
+// D fragments, after bias, ReLU and BF16 rounding, are the next layer's B
+// fragment as they stand: in v_mfma_f32_16x16x32_bf16 lane l holds D rows
+// 4g..4g+3 (g = l / 16) of a 16-row tile and B depth positions 8g..8g+7 of a
+// 32-deep step, so two D tiles make one B fragment.  The packed weights
+// (k_pack_infer) carry the matching depth order; nothing moves between lanes
+// or through LDS from one layer to the next.  W1, W2 sit in VGPRs, W3 and W4
+// (80 KiB) in LDS, staged once per persistent block.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+// four consecutive pixels at any 4-byte boundary (global_load_dwordx4 takes
+// dword-aligned addresses)
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 
-__global__ __launch_bounds__(kInferThreads) void k_seed_infer(SeedInferParams p)
+constexpr int kInferWaves = 4;
+constexpr int kInferThreads = 64 * kInferWaves;
+constexpr int kInferGroup = 16;  // POIs per wave pass
+constexpr int kInferBlocksPerCu = 2;
+// packed fragments: 64 lanes x 8 BF16 (1 KiB) each
+constexpr int kFragOff2 = 1, kFragOff3 = kFragOff2 + 4, kFragOff4 = kFragOff3 + 16, kFragTotal = kFragOff4 + 64;
+constexpr int kLdsFrags = kFragTotal - kFragOff3;  // W3 (16) + W4 (64)
+constexpr int kW5Frags = 32;                       // head: 8 channel tiles x 4 rows, 64 FP32 each
+
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
 {
-    __shared__ __align__(16) uint16_t x2[kInferPois * 16 * 64];  // layer 2 input
-    __shared__ __align__(16) uint16_t x3[kInferPois * 4 * 128];  // layer 3 input
-    __shared__ __align__(16) uint16_t x4[kInferPois * 256];      // layer 4 input
-    __shared__ float h4[kInferPois * 128];                       // layer 4 output
-    __shared__ float outv[kInferPois][kSeedOut];
-    __shared__ int crop[kInferPois][4];             // x0, y0 (ref), x0, y0 (target)
-    __shared__ float guess[kInferPois][2];
-    __shared__ int valid[kInferPois];
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int base = blockIdx.x * kInferPois;
-    const int w = p.w, h = p.h;
+__device__ __forceinline__ __bf16 relu_bf16(float z) { return (__bf16)fmaxf(z, 0.f); }
 
-    if (tid < kInferPois) {
-        const int b = base + tid;
-        int ok = 0;
+__global__ __launch_bounds__(kInferThreads, 2) void k_seed_infer(SeedInferParams p)
+{
+    __shared__ bf16x8 wl[kLdsFrags * 64];  // W3 fragments, then W4
+    const bf16x8* gf = reinterpret_cast<const bf16x8*>(p.wts.frag);
+    for (int i = threadIdx.x; i < kLdsFrags * 64; i += kInferThreads) wl[i] = gf[kFragOff3 * 64 + i];
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = lane >> 4, s = lane & 15;
+    const int crop_c = g >> 1;       // this lane's pixels: 0 reference, 1 target
+    const int rsub = 2 * (g & 1);    // and patch rows rsub, rsub + 1
+    const int w = p.w;
+    const bf16x8 w1 = gf[lane];
+    bf16x8 w2[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) w2[i] = gf[(kFragOff2 + i) * 64 + lane];
+    float b1[4], b2[2][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        b1[i] = p.wts.b[0][4 * g + i];
+        b2[0][i] = p.wts.b[1][4 * g + i];
+        b2[1][i] = p.wts.b[1][16 + 4 * g + i];
+    }
+    const int groups = (p.n + kInferGroup - 1) / kInferGroup;
+    if (p.w < kSeedCrop || p.h < kSeedCrop) {
+        // no crop fits: nothing is seeded
+        for (int b = (blockIdx.x * kInferWaves + wave) * 64 + lane; p.ok && b < p.n; b += gridDim.x * kInferThreads)
+            p.ok[b] = 0;
+        return;
+    }
+
+    for (int grp = blockIdx.x * kInferWaves + wave; grp < groups; grp += gridDim.x * kInferWaves) {
+        const int b = grp * kInferGroup + s;
+        const int qi = b < p.n ? (p.index ? (int)p.index[b] : b) : 0;
+        int ok = 0, x0 = 0, y0 = 0;
+        float gu = 0.f, gv = 0.f;
         if (b < p.n) {
-            const fi_poi& q = p.pois[p.index ? p.index[b] : b];
-            const float gu = rintf(q.deformation[0]), gv = rintf(q.deformation[6]);
+            const fi_poi& q = p.pois[qi];
+            gu = rintf(q.deformation[0]);
+            gv = rintf(q.deformation[6]);
             const float cx = floorf(q.x), cy = floorf(q.y);
             const float rx0 = cx - kSeedCropOrigin, ry0 = cy - kSeedCropOrigin;
             const float tx0 = rx0 + gu, ty0 = ry0 + gv;
-            const float hi_x = (float)(w - kSeedCrop), hi_y = (float)(h - kSeedCrop);
+            const float hx = (float)(p.w - kSeedCrop), hy = (float)(p.h - kSeedCrop);
             // finite and inside both images (NaN compares false)
-            ok = (rx0 >= 0.f && rx0 <= hi_x && ry0 >= 0.f && ry0 <= hi_y && tx0 >= 0.f && tx0 <= hi_x &&
-                  ty0 >= 0.f && ty0 <= hi_y) ? 1 : 0;
-            crop[tid][0] = ok ? (int)rx0 : 0;
-            crop[tid][1] = ok ? (int)ry0 : 0;
-            crop[tid][2] = ok ? (int)tx0 : 0;
-            crop[tid][3] = ok ? (int)ty0 : 0;
-            guess[tid][0] = gu;
-            guess[tid][1] = gv;
-            if (p.ok) p.ok[b] = (uint32_t)ok;
+            ok = (rx0 >= 0.f && rx0 <= hx && ry0 >= 0.f && ry0 <= hy && tx0 >= 0.f && tx0 <= hx && ty0 >= 0.f &&
+                  ty0 <= hy) ? 1 : 0;
+            if (ok) {
+                x0 = (int)(crop_c ? tx0 : rx0);
+                y0 = (int)(crop_c ? ty0 : ry0);
+            }
         }
-        valid[tid] = ok;
-    }
-    __syncthreads();
+        // crop origin; (0, 0) for a lane without a valid crop, so that every
+        // load below stays inside the image (width, height >= 32)
+        const float* base = (crop_c ? p.tar : p.ref) + (size_t)y0 * w + x0;
 
-    // ---- layer 1: wave w owns POIs 4w .. 4w+3 -------------------------------
-    for (int sp = 0; sp < 4; sp++) {
-        const int s = wave * 4 + sp;
-        float mean[2] = { 0.f, 0.f }, inv[2] = { 0.f, 0.f };
-        if (valid[s]) {
+        // crop statistics: this lane sums rows r with (r >> 1) & 1 == g & 1,
+        // the partner lane l ^ 16 the others
+        float sum = 0.f, sq = 0.f;
+        for (int rr = 0; rr < 16; rr++) {
+            const float* row = base + (size_t)(4 * (rr >> 1) + rsub + (rr & 1)) * w;
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
-                const float* img = c ? p.tar : p.ref;
-                const int x0 = crop[s][2 * c], y0 = crop[s][2 * c + 1];
-                // 1024 pixels: lane reads rows (lane / 8) + 8 t, columns 4 (lane % 8) .. +3
-                float sum = 0.f, sq = 0.f;
+            for (int x = 0; x < kSeedCrop; x++) {
+                const float v = row[x];
+                sum += v;
+                sq = __builtin_fmaf(v, v, sq);
+            }
+        }
+        sum += __shfl_xor(sum, 16);
+        sq += __shfl_xor(sq, 16);
+        const float mean = sum * (1.f / 1024.f);
+        const float var = fmaxf(sq * (1.f / 1024.f) - mean * mean, 0.f);
+        // a lane without a valid crop feeds zeros (its results are not stored)
+        const float inv = ok ? 1.f / sqrtf(var + 1e-6f) : 0.f;
+
+        f32x4 acc4[8];
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int r = (lane >> 3) + 8 * t;
-                    const float* q = img + (size_t)(y0 + r) * w + x0 + 4 * (lane & 7);
-                    const float v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
-                    sum += (v0 + v1) + (v2 + v3);
-                    sq += (v0 * v0 + v1 * v1) + (v2 * v2 + v3 * v3);
+        for (int i = 0; i < 8; i++) acc4[i] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+        for (int q3 = 0; q3 < 4; q3++) {
+            f32x4 acc3[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) acc3[i] = f32x4{ 0.f, 0.f, 0.f, 0.f };
+            // not unrolled: unrolling lets the scheduler hoist the pixel loads of
+            // all 16 patches of q3 and spill
+#pragma unroll 1
+            for (int q2 = 0; q2 < 4; q2++) {
+                f32x4 acc2[2] = { { 0.f, 0.f, 0.f, 0.f }, { 0.f, 0.f, 0.f, 0.f } };
+#pragma unroll
+                for (int t2 = 0; t2 < 2; t2++) {
+                    bf16x8 x2;  // layer-2 B fragment: patches q1 = 2 t2, 2 t2 + 1
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const int q1 = 2 * t2 + h;
+                        const int py = 4 * (q3 >> 1) + 2 * (q2 >> 1) + (q1 >> 1);
+                        const int px = 4 * (q3 & 1) + 2 * (q2 & 1) + (q1 & 1);
+                        const float* pp = base + (size_t)(4 * py + rsub) * w + 4 * px;
+                        // two rows of four pixels, in bounds for every lane
+                        const f32x4u r0 = *reinterpret_cast<const f32x4u*>(pp);
+                        const f32x4u r1 = *reinterpret_cast<const f32x4u*>(pp + w);
+                        bf16x8 x1;
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            x1[j] = (__bf16)((r0[j] - mean) * inv);
+                            x1[4 + j] = (__bf16)((r1[j] - mean) * inv);
+                        }
+                        const f32x4 d = mfma32(w1, x1, f32x4{ 0.f, 0.f, 0.f, 0.f });
+#pragma unroll
+                        for (int i = 0; i < 4; i++) x2[4 * h + i] = relu_bf16(d[i] + b1[i]);
+                    }
+                    acc2[0] = mfma32(w2[t2], x2, acc2[0]);
+                    acc2[1] = mfma32(w2[2 + t2], x2, acc2[1]);
                 }
-                sum = wave_sum(sum);
-                sq = wave_sum(sq);
-                const float m = sum * (1.f / 1024.f);
-                const float var = fmaxf(sq * (1.f / 1024.f) - m * m, 0.f);
-                mean[c] = m;
-                inv[c] = 1.f / sqrtf(var + 1e-6f);
-            }
-        }
-        // 4 row tiles of 16 patches, 2 depth steps (ref, target)
+                bf16x8 x3;  // layer-3 B fragment of position q2: channels 0..15 | 16..31
 #pragma unroll
-        for (int rt = 0; rt < 4; rt++) {
-            f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-            for (int ks = 0; ks < 2; ks++) {
-                const int patch = rt * 16 + (lane & 15);
-                const int py = patch >> 3, px = patch & 7, dy = lane >> 4;
-                bf16x4 a = { 0, 0, 0, 0 };
-                if (valid[s]) {
-                    const float* img = ks ? p.tar : p.ref;
-                    const int x0 = crop[s][2 * ks], y0 = crop[s][2 * ks + 1];
-                    const float* q = img + (size_t)(y0 + 4 * py + dy) * w + x0 + 4 * px;
-#pragma unroll
-                    for (int t = 0; t < 4; t++) a[t] = (short)to_bf16((q[t] - mean[ks]) * inv[ks]);
+                for (int i = 0; i < 4; i++) {
+                    x3[i] = relu_bf16(acc2[0][i] + b2[0][i]);
+                    x3[4 + i] = relu_bf16(acc2[1][i] + b2[1][i]);
                 }
-                const bf16x4 bw = load4(p.wts.wt[0] + (lane & 15) * 32 + ks * 16 + 4 * (lane >> 4));
-                acc = mfma16(a, bw, acc);
+#pragma unroll
+                for (int mt = 0; mt < 4; mt++) acc3[mt] = mfma32(wl[(mt * 4 + q2) * 64 + lane], x3, acc3[mt]);
             }
-            const int n = lane & 15;
-            const float bias = p.wts.b[0][n];
+            bf16x8 x4a, x4b;  // layer-4 B fragments of position q3: channels 0..31, 32..63
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const int patch = rt * 16 + 4 * (lane >> 4) + j;
-                int row, col;
-                patchify(s, patch, 8, 16, n, row, col);
-                x2[row * 64 + col] = to_bf16(fmaxf(acc[j] + bias, 0.f));
+            for (int i = 0; i < 4; i++) {
+                x4a[i] = relu_bf16(acc3[0][i] + p.wts.b[2][4 * g + i]);
+                x4a[4 + i] = relu_bf16(acc3[1][i] + p.wts.b[2][16 + 4 * g + i]);
+                x4b[i] = relu_bf16(acc3[2][i] + p.wts.b[2][32 + 4 * g + i]);
+                x4b[4 + i] = relu_bf16(acc3[3][i] + p.wts.b[2][48 + 4 * g + i]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 8; mt++) {
+                const int f = 16 + mt * 8 + 2 * q3;
+                acc4[mt] = mfma32(wl[f * 64 + lane], x4a, acc4[mt]);
+                acc4[mt] = mfma32(wl[(f + 1) * 64 + lane], x4b, acc4[mt]);
+            }
+        }
+
+        // head, FP32 MFMA 16x16x4: out[o][s] = sum_c W5[c][o] relu(acc4 + b4)[c][s];
+        // lane (g, s) supplies depth g of each step = channel 16 mt + 4 g + i
+        f32x4 o4 = { 0.f, 0.f, 0.f, 0.f };
+#pragma unroll
+        for (int mt = 0; mt < 8; mt++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float hv = fmaxf(acc4[mt][i] + p.wts.b[3][16 * mt + 4 * g + i], 0.f);
+                o4 = __builtin_amdgcn_mfma_f32_16x16x4f32(p.wts.w5f[(mt * 4 + i) * 64 + lane], hv, o4, 0, 0, 0);
+            }
+        }
+        // lane (0, s) holds outputs 0..3 of POI s, lane (1, s) outputs 4..7
+        if (b < p.n) {
+            if (g == 0 && p.ok) p.ok[b] = (uint32_t)ok;
+            if (ok && g < 2) {
+                fi_poi& q = p.pois[qi];
+                const float* b5 = p.wts.b5;
+                if (g == 0) {
+                    q.deformation[0] = gu + (o4[0] + b5[0]);
+                    q.deformation[6] = gv + (o4[1] + b5[1]);
+                    q.deformation[1] = (o4[2] + b5[2]) * (1.f / kGradScale);
+                    q.deformation[2] = (o4[3] + b5[3]) * (1.f / kGradScale);
+                } else {
+                    q.deformation[7] = (o4[0] + b5[4]) * (1.f / kGradScale);
+                    q.deformation[8] = (o4[1] + b5[5]) * (1.f / kGradScale);
+                }
             }
         }
     }
-    __syncthreads();
+}
 
-    // ---- layer 2: M = 256 (16 rows per POI), K = 64, N = 32 ------------------
-    for (int t = 0; t < 8; t++) {
-        const int rt = wave * 4 + (t >> 1), ct = t & 1;  // row tile = POI
-        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for (int ks = 0; ks < 4; ks++) {
-            const bf16x4 a = load4(x2 + (rt * 16 + (lane & 15)) * 64 + ks * 16 + 4 * (lane >> 4));
-            const bf16x4 bw = load4(p.wts.wt[1] + (ct * 16 + (lane & 15)) * 64 + ks * 16 + 4 * (lane >> 4));
-            acc = mfma16(a, bw, acc);
-        }
-        const int n = ct * 16 + (lane & 15);
-        const float bias = p.wts.b[1][n];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int patch = 4 * (lane >> 4) + j;
-            int row, col;
-            patchify(rt, patch, 4, 32, n, row, col);
-            x3[row * 128 + col] = to_bf16(fmaxf(acc[j] + bias, 0.f));
-        }
-    }
-    __syncthreads();
+// Packed inference weights from the FP32 parameters.  Fragment f of layer L
+// (m-tile mt, depth step t, f = mt * steps + t), lane l, element j holds
+// BF16(W_L[k][16 mt + l % 16]) with g = l / 16 and
+//   layer 1: k = 8 g + j                                (pixel c, dy, dx)
+//   layers 2-4: k = 32 t + 16 (j / 4) + 4 g + j % 4     (previous layer's D tiles)
+// Head fragment mt * 4 + i, lane l: W5[16 mt + 4 g + i][l % 16] for l % 16 < 6,
+// else 0.
+struct PackSrc {
+    const float* w[kSeedLayers];
+    const float* w5;
+};
 
-    // ---- layer 3: M = 64 (4 rows per POI), K = 128, N = 64 -------------------
-    for (int ct = 0; ct < 4; ct++) {
-        const int rt = wave;  // rows 16 w .. 16 w + 15 = POIs 4w .. 4w+3
-        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for (int ks = 0; ks < 8; ks++) {
-            const bf16x4 a = load4(x3 + (rt * 16 + (lane & 15)) * 128 + ks * 16 + 4 * (lane >> 4));
-            const bf16x4 bw = load4(p.wts.wt[2] + (ct * 16 + (lane & 15)) * 128 + ks * 16 + 4 * (lane >> 4));
-            acc = mfma16(a, bw, acc);
-        }
-        const int n = ct * 16 + (lane & 15);
-        const float bias = p.wts.b[2][n];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int r = rt * 16 + 4 * (lane >> 4) + j;  // POI r / 4, patch r % 4
-            int row, col;
-            patchify(r >> 2, r & 3, 2, 64, n, row, col);
-            x4[row * 256 + col] = to_bf16(fmaxf(acc[j] + bias, 0.f));
-        }
+__global__ void k_pack_infer(PackSrc src, uint16_t* __restrict__ frag, float* __restrict__ w5f)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < kFragTotal * 512) {
+        const int f = e >> 9, r = e & 511, l = r >> 3, j = r & 7, g = l >> 4, col = l & 15;
+        int layer, fl;
+        if (f < kFragOff2) { layer = 0; fl = 0; }
+        else if (f < kFragOff3) { layer = 1; fl = f - kFragOff2; }
+        else if (f < kFragOff4) { layer = 2; fl = f - kFragOff3; }
+        else { layer = 3; fl = f - kFragOff4; }
+        const int steps = kSeedK[layer] / 32, mt = fl / steps, t = fl - mt * steps;
+        const int k = layer == 0 ? 8 * g + j : 32 * t + 16 * (j >> 2) + 4 * g + (j & 3);
+        frag[e] = to_bf16(src.w[layer][(size_t)k * kSeedN[layer] + 16 * mt + col]);
+    } else if (e < kFragTotal * 512 + kW5Frags * 64) {
+        const int e2 = e - kFragTotal * 512, mi = e2 >> 6, l = e2 & 63, mt = mi >> 2, i = mi & 3;
+        const int o = l & 15, ch = 16 * mt + 4 * (l >> 4) + i;
+        w5f[e2] = o < kSeedOut ? src.w5[ch * kSeedOut + o] : 0.f;
     }
-    __syncthreads();
+}
 
-    // ---- layer 4: M = 16 POIs, K = 256, N = 128 ------------------------------
-    for (int t = 0; t < 2; t++) {
-        const int ct = wave * 2 + t;
-        f32x4 acc = { 0.f, 0.f, 0.f, 0.f };
-#pragma unroll
-        for (int ks = 0; ks < 16; ks++) {
-            const bf16x4 a = load4(x4 + (lane & 15) * 256 + ks * 16 + 4 * (lane >> 4));
-            const bf16x4 bw = load4(p.wts.wt[3] + (ct * 16 + (lane & 15)) * 256 + ks * 16 + 4 * (lane >> 4));
-            acc = mfma16(a, bw, acc);
-        }
-        const int n = ct * 16 + (lane & 15);
-        const float bias = p.wts.b[3][n];
-#pragma unroll
-        for (int j = 0; j < 4; j++) h4[(4 * (lane >> 4) + j) * 128 + n] = fmaxf(acc[j] + bias, 0.f);
-    }
-    __syncthreads();
+size_t seed_frag_elems() { return (size_t)kFragTotal * 512; }
+size_t seed_w5f_elems() { return (size_t)kW5Frags * 64; }
 
-    // ---- head (FP32) and seed ------------------------------------------------
-    if (tid < kInferPois * kSeedOut) {
-        const int s = tid / kSeedOut, o = tid - s * kSeedOut;
-        float acc = p.wts.b5[o];
-        for (int c = 0; c < 128; c++) acc = __builtin_fmaf(h4[s * 128 + c], p.wts.w5[c * kSeedOut + o], acc);
-        outv[s][o] = acc;
-    }
-    __syncthreads();
-    if (tid < kInferPois && valid[tid]) {
-        const int b = base + tid;
-        fi_poi& q = p.pois[p.index ? p.index[b] : b];
-        const float* o = outv[tid];
-        q.deformation[0] = guess[tid][0] + o[0];
-        q.deformation[6] = guess[tid][1] + o[1];
-        q.deformation[1] = o[2] * (1.f / kGradScale);
-        q.deformation[2] = o[3] * (1.f / kGradScale);
-        q.deformation[7] = o[4] * (1.f / kGradScale);
-        q.deformation[8] = o[5] * (1.f / kGradScale);
-    }
+void launch_pack_infer(const float* const w[kSeedLayers], const float* w5, uint16_t* frag, float* w5f,
+                       hipStream_t s)
+{
+    PackSrc src;
+    for (int l = 0; l < kSeedLayers; l++) src.w[l] = w[l];
+    src.w5 = w5;
+    const int total = kFragTotal * 512 + kW5Frags * 64;
+    hipLaunchKernelGGL(k_pack_infer, dim3((total + 255) / 256), dim3(256), 0, s, src, frag, w5f);
 }
 
 void launch_seed_infer(const SeedInferParams& p, hipStream_t s)
 {
     if (p.n <= 0) return;
-    hipLaunchKernelGGL(k_seed_infer, dim3((p.n + kInferPois - 1) / kInferPois), dim3(kInferThreads), 0, s, p);
+    const int cus = std::max(1, p.cus);
+    const int groups = (p.n + kInferGroup - 1) / kInferGroup;
+    const int blocks = std::max(1, std::min((groups + kInferWaves - 1) / kInferWaves, kInferBlocksPerCu * cus));
+    hipLaunchKernelGGL(k_seed_infer, dim3(blocks), dim3(kInferThreads), 0, s, p);
 }
 
 // ---------------------------------------------------------------------------

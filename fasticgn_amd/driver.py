@@ -18,6 +18,7 @@ from .opencorr import Image2D, _check_queue, _Handle
 
 SKIP_FFTCC = nat.FI_POLICY_SKIP_FFTCC
 POI_FALLBACK = nat.FI_POLICY_POI_FALLBACK
+LEARNED_SEED = nat.FI_POLICY_LEARNED_SEED
 DEFAULT_POLICY = nat.FI_POLICY_DEFAULT
 
 
@@ -64,6 +65,15 @@ class FrameDriver(_Handle):
         if mode not in modes:
             raise ValueError(f"sampling must be one of {sorted(modes)}")
         nat.call("fi_frame_set_sampling", self._h, modes[mode])
+
+    def set_seeder(self, seeder, zncc_threshold: float = 0.8) -> None:
+        """Learned seeder for policy LEARNED_SEED (a seeder.LearnedSeeder, or
+        None to detach).  POIs whose IC-GN result from the learned seed fails
+        or has ZNCC below zncc_threshold are redone from FFTCC.  The seeder
+        must stay open while it is attached."""
+        self._seeder = seeder
+        nat.call("fi_frame_set_seeder", self._h, seeder.handle if seeder is not None else None,
+                 float(zncc_threshold))
 
     def reset_sequence(self) -> None:
         nat.call("fi_frame_reset_sequence", self._h)

@@ -5,8 +5,10 @@
 
 Data: block-mosaic speckle pairs (one affine map per 64x64 block, u, v ~
 U[-8, 8] px, gradients ~ U[-0.05, 0.05]), generated by a pool of CPU workers
-and labelled by ICGN2D1 on the device (seeder.make_training_set); 5 % of the
-samples are held out and the prediction error on them is reported.  Training
+and labelled by ICGN2D1 on the device (as seeder.make_training_set); the
+target crop sits at guess 0 for half of the samples and at the rounded truth
+moved by U[-1, 1] px for the other half.  5 % of the samples are held out and
+the prediction error on them is reported for both halves.  Training
 runs entirely in the BF16-MFMA kernels (fi_seeder_train_epoch).
 """
 from __future__ import annotations
@@ -28,13 +30,14 @@ def _mosaic(args):
     return block_mosaic(blocks, np.random.default_rng(seed))
 
 
-def generate(n_samples: int, seed: int, workers: int, blocks: int = 32, log=print):
+def generate(n_samples: int, seed: int, workers: int, blocks: int = 32, near_fraction: float = 0.5,
+             log=print):
     """Like seeder.make_training_set, with the mosaics rendered in parallel."""
     from fasticgn_amd.opencorr import ICGN2D1, Image2D
     from fasticgn_amd.seeder import crop_rows, targets_from
     per = blocks * blocks
     n_mosaics = int(np.ceil(n_samples / (0.9 * per))) + 2
-    xs, ts, kept, total = [], [], 0, 0
+    xs, ts, nears, kept, total = [], [], [], 0, 0
     t0 = time.perf_counter()
     with ProcessPoolExecutor(max_workers=workers) as pool, ICGN2D1(15, 15, 1e-4, 20) as ic:
         for ref, tar, pois, truth in pool.map(_mosaic, [(seed * 100003 + m, blocks) for m in range(n_mosaics)]):
@@ -48,16 +51,26 @@ def generate(n_samples: int, seed: int, workers: int, blocks: int = 32, log=prin
             good = np.isfinite(z) & (z >= 0.9)
             good &= np.abs(q["deformation"][:, 0] - truth[:, 0]) < 0.25
             good &= np.abs(q["deformation"][:, 6] - truth[:, 1]) < 0.25
-            x1, ok, guess = crop_rows(ref.astype(np.float32), tar.astype(np.float32), pois.copy())
+            # crop guesses: half at 0 (the full +-8 px range), half at the
+            # rounded truth moved by U[-1, 1] px (the residual range of a
+            # second pass or of a good initial guess)
+            c = pois.copy()
+            rng = np.random.default_rng(seed * 7919 + total)
+            near = rng.random(len(c)) < near_fraction
+            c["deformation"][:, 0] = np.where(near, np.rint(truth[:, 0] + rng.uniform(-1, 1, len(c))), 0)
+            c["deformation"][:, 6] = np.where(near, np.rint(truth[:, 1] + rng.uniform(-1, 1, len(c))), 0)
+            x1, ok, guess = crop_rows(ref.astype(np.float32), tar.astype(np.float32), c)
             good &= ok
             xs.append(x1[good])
             ts.append(targets_from(q[good], guess[good]))
+            nears.append(near[good])
             kept += int(good.sum())
             total += len(q)
             log(f"[seeder] data {kept}/{n_samples} kept of {total} ({time.perf_counter() - t0:.1f} s)")
             if kept >= n_samples:
                 break
-    return np.concatenate(xs)[:n_samples], np.concatenate(ts)[:n_samples], total
+    return (np.concatenate(xs)[:n_samples], np.concatenate(ts)[:n_samples],
+            np.concatenate(nears)[:n_samples], total)
 
 
 def main() -> int:
@@ -70,13 +83,14 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=min(12, os.cpu_count() or 1))
     ap.add_argument("--out", default=str(Path(__file__).resolve().parent / "seeder_weights.npy"))
     ap.add_argument("--report", default="")
+    ap.add_argument("--near", type=float, default=0.5, help="fraction of crops near the truth")
     args = ap.parse_args()
 
     def log(msg):
         print(msg, flush=True)
 
     from fasticgn_amd import seeder as sd
-    x1, t, generated = generate(args.samples, args.seed, args.workers, log=log)
+    x1, t, near, generated = generate(args.samples, args.seed, args.workers, near_fraction=args.near, log=log)
     n_hold = max(1, len(x1) // 20)
     rng = np.random.default_rng(args.seed)
     perm = rng.permutation(len(x1))
@@ -90,12 +104,23 @@ def main() -> int:
         params = s.get_params()
     err = pred - t[hold]
     rmse = np.sqrt(np.mean(err ** 2, axis=0))
+    hn = near[hold]
+
+    def split(m):
+        e = err[m]
+        if not len(e):
+            return None
+        r = np.sqrt(np.mean(e ** 2, axis=0))
+        return {"n": int(len(e)), "rmse_u_v_px": [float(r[0]), float(r[1])],
+                "rmse_grad": [float(v / sd.GRAD_SCALE) for v in r[2:]],
+                "p99_abs_u_px": float(np.percentile(np.abs(e[:, 0]), 99))}
     report = {"samples": int(len(x1)), "generated": int(generated), "held_out": int(n_hold),
               "epochs": args.epochs, "batch": args.batch, "lr": args.lr, "train_seconds": round(train_s, 2),
               "final_loss": losses[-1], "losses": [round(v, 6) for v in losses],
               "holdout_rmse_u_v_px": [float(rmse[0]), float(rmse[1])],
               "holdout_rmse_grad": [float(v / sd.GRAD_SCALE) for v in rmse[2:]],
-              "holdout_p99_abs_u_px": float(np.percentile(np.abs(err[:, 0]), 99))}
+              "holdout_p99_abs_u_px": float(np.percentile(np.abs(err[:, 0]), 99)),
+              "holdout_near": split(hn), "holdout_far": split(~hn)}
     log("[seeder] " + json.dumps(report))
     np.save(args.out, params)
     if args.report:

@@ -374,3 +374,65 @@ def test_config1_full_size_properties(oracle):
     cmp = compare(o, res[pick])
     report("config1 sample", cmp)
     assert cmp["pass"], cmp
+
+
+# ------------------------------------------------------------ learned seed --
+
+def _learned_frame(sc, params, threshold=0.8):
+    from fasticgn_amd import seeder as sd
+    with sd.LearnedSeeder(params) as s, fa.FrameDriver(sc.radius) as d:
+        d.set_reference(fa.Image2D(sc.ref))
+        d.set_pois(sc.pois)
+        d.set_seeder(s, threshold)
+        d.upload_target(fa.Image2D(sc.tar))
+        st = d.run(fa.driver.LEARNED_SEED)
+        return d.download(), st
+
+
+def test_learned_seed_frame_parity(oracle):
+    """The shipped seeder on the 2048^2 frame (POIs start from the global
+    integer guess): same parity gate against the CPU oracle as the FFTCC-seeded
+    path on a POI sample, validity flags included, and accuracy against the
+    known field on every POI."""
+    from fasticgn_amd import seeder as sd
+    assert sd.WEIGHTS_PATH.exists()
+    sc = synth.config1()
+    res, st = _learned_frame(sc, None)
+    say("learned config1 stats", {k: st[k] for k in ("seed_pois", "seed_fallback_count", "valid_count",
+                                                     "iteration_mean", "seed_ms", "icgn_ms")})
+    assert st["seed_pois"] == len(sc.pois)
+    assert st["seed_fallback_count"] < 0.05 * len(sc.pois)
+    ok = finite_valid(res)
+    assert ok.mean() > 0.99
+    u, v, *_ = sc.truth(res["x"], res["y"])
+    err = np.maximum(np.abs(res["deformation"][ok, 0] - u[ok]), np.abs(res["deformation"][ok, 6] - v[ok]))
+    assert np.percentile(err, 99) < 0.1
+    rng = np.random.default_rng(1)
+    pick = np.sort(rng.choice(len(sc.pois), 3000, replace=False))
+    o = sc.pois[pick].copy()
+    ref32, tar32 = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
+    oracle.fftcc(ref32, tar32, 15, o)
+    oracle.icgn(ref32, tar32, 15, o)
+    cmp = compare(o, res[pick])
+    report("learned config1 sample", cmp)
+    assert cmp["pass"], cmp
+
+
+def test_learned_seed_bad_seeds_fall_back_to_fftcc(oracle, c0):
+    """A seeder whose u output is off by 40 px: every IC-GN from its seeds
+    fails or lands on a low ZNCC, FFTCC redoes those POIs, and the frame equals
+    the plain FFTCC-seeded frame wherever the fallback result won."""
+    from fasticgn_amd import seeder as sd
+    p = np.load(sd.WEIGHTS_PATH).copy()
+    p[-6] += 40.0  # b5[0]: u seed + 40 px
+    res, st = _learned_frame(c0, p)
+    with fa.FrameDriver(15) as d:
+        d.set_reference(fa.Image2D(c0.ref))
+        d.set_pois(c0.pois)
+        plain, _ = d.process(fa.Image2D(c0.tar), policy=0)
+    say("bad seeder stats", {k: st[k] for k in ("seed_pois", "seed_fallback_count", "valid_count")})
+    assert st["seed_fallback_count"] >= 0.95 * len(c0.pois)
+    same = np.all(res["deformation"] == plain["deformation"], axis=1) & np.all(res["result"] == plain["result"], axis=1)
+    assert same.mean() >= 0.95
+    cmp = compare(plain, res)
+    assert cmp["pass"], cmp
