@@ -72,6 +72,22 @@ class FrameStats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class TuneEntry(ctypes.Structure):
+    """fi_tune_entry (include/fasticgn.h)."""
+    _fields_ = [
+        ("radius", ctypes.c_int32),
+        ("waves", ctypes.c_int32),
+        ("rows_per_band", ctypes.c_int32),
+        ("sampling", ctypes.c_int32),
+        ("persistent", ctypes.c_int32),
+        ("chosen", ctypes.c_int32),
+        ("ms", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class LogicError(RuntimeError):
     """std::logic_error of the reference."""
 
@@ -134,6 +150,10 @@ SIGNATURES = {
     "fi_frame_set_shape": [_P, _I, _I],
     "fi_frame_set_sampling": [_P, _I],
     "fi_frame_set_seeder": [_P, _P, _F],
+    "fi_frame_set_self_adaptive": [_P, _I],
+    "fi_frame_set_graph": [_P, _I],
+    "fi_frame_set_persistent": [_P, _I],
+    "fi_frame_autotune": [_P, _P, _S, ctypes.POINTER(_S)],
     "fi_seeder_create": [_I, ctypes.POINTER(_P)],
     "fi_seeder_destroy": [_P],
     "fi_seeder_param_count": [ctypes.POINTER(_S)],

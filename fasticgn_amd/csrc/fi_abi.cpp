@@ -292,6 +292,11 @@ int fi_frame_set_shape(fi_frame* h, int waves, int spl)
                                             " does not fit the driver's subset radius " + std::to_string(h->r));
         h->shape_waves = waves;
         h->shape_spl = spl;
+        for (auto& c : h->classes) {  // an explicit shape replaces tuned ones
+            c->waves = c->spl = 0;
+            c->queue_blocks = -1;
+        }
+        h->invalidate_graphs();
     });
 }
 
@@ -301,6 +306,47 @@ int fi_frame_set_sampling(fi_frame* h, int mode)
         need(h);
         check_sampling(mode);
         h->sampling = mode;
+        for (auto& c : h->classes) c->sampling = -1;
+        h->invalidate_graphs();
+    });
+}
+
+int fi_frame_set_self_adaptive(fi_frame* h, int on)
+{
+    return guarded([&] {
+        need(h);
+        h->set_self_adaptive(on != 0);
+    });
+}
+
+int fi_frame_set_persistent(fi_frame* h, int blocks)
+{
+    return guarded([&] {
+        need(h);
+        if (blocks < 0) fail(FI_E_INVALID_ARGUMENT, "persistent block count must be >= 0");
+        h->persistent_blocks = blocks;
+        for (auto& c : h->classes) c->queue_blocks = -1;
+        h->invalidate_graphs();
+    });
+}
+
+int fi_frame_set_graph(fi_frame* h, int on)
+{
+    return guarded([&] {
+        need(h);
+        h->use_graph = on != 0;
+        h->invalidate_graphs();
+    });
+}
+
+int fi_frame_autotune(fi_frame* h, fi_tune_entry* report, size_t report_len, size_t* report_n)
+{
+    return guarded([&] {
+        need(h);
+        if (report == nullptr && report_len > 0) fail(FI_E_INVALID_ARGUMENT, "report pointer is null");
+        size_t m = 0;
+        h->autotune(report, report_len, &m);
+        if (report_n) *report_n = m;
     });
 }
 
@@ -313,6 +359,7 @@ int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold)
             fail(FI_E_INVALID_ARGUMENT, "seeder and frame driver are on different devices");
         h->seeder = seeder;
         h->seed_threshold = zncc_threshold;
+        h->invalidate_graphs();
     });
 }
 

@@ -2,9 +2,11 @@
 #include "fi_engines.h"
 #include "fi_seeder_engine.h"
 
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include <map>
+#include <numeric>
 #include <string>
 #include <utility>
 
@@ -454,21 +456,22 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
         fail(FI_E_UNSUPPORTED, "FFTCC window exceeds the LDS per block");
     stream = make_stream();
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
-    std::vector<float> t(4 * r);
-    dft_twiddles(2 * r, t.data());
-    tw.reserve(t.size());
-    FI_HIP(hipMemcpy(tw.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
-    counters.reserve(1);
-    h_counters.reserve(1);
     iter_total.reserve(1);
     h_iter_total.reserve(1);
 }
 
 fi_frame::~fi_frame()
 {
+    for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+void fi_frame::invalidate_graphs()
+{
+    for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
+    graphs.clear();
 }
 
 void fi_frame::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
@@ -491,6 +494,13 @@ void fi_frame::set_pois(const fi_poi* pois, size_t count)
     reset_sequence();
 }
 
+void fi_frame::set_self_adaptive(bool on)
+{
+    self_adaptive = on;
+    prepared = false;
+    reset_sequence();
+}
+
 void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12)
 {
     DeviceGuard g(device);
@@ -507,16 +517,59 @@ void fi_frame::reset_sequence()
     baseline_ready = false;
     baseline_bad = NAN;
     frame_index = 0;
+    invalidate_graphs();
 }
 
 // prepareRef of both engines for the template (cli:4095-4104): reference f32,
-// gradients, FFTCC spectra, per-POI Hessian records.
+// gradients, FFTCC spectra, per-POI Hessian records.  With self_adaptive the
+// template is laid out on the device sorted by subset radius (stable), one
+// radius class per distinct radius.
 void fi_frame::prepare()
 {
     if (!ref_set) fail(FI_E_LOGIC, "frame driver needs a reference image");
     if (!pois_set) fail(FI_E_LOGIC, "frame driver needs a POI template");
     const int w = ref_raw.view.width, h = ref_raw.view.height;
     const size_t px = (size_t)w * h;
+
+    // radius classes
+    std::vector<int> rad(n, r);
+    order.clear();
+    if (self_adaptive) {
+        const size_t lds = max_lds_per_block(device);
+        for (size_t i = 0; i < n; i++) {
+            const int rx = safe_int(tmpl_host[i].subset_radius[0]), ry = safe_int(tmpl_host[i].subset_radius[1]);
+            if (rx != ry)
+                fail(FI_E_UNSUPPORTED, "frame driver subsets are square (FFTCC2D): POI " + std::to_string(i) +
+                                           " has subset_radius " + std::to_string(rx) + "x" + std::to_string(ry));
+            if (rx < 1 || rx > kIcgnMaxRadius)
+                fail(FI_E_UNSUPPORTED, "self-adaptive subset radius outside 1.." + std::to_string(kIcgnMaxRadius) +
+                                           " at POI " + std::to_string(i));
+            if (fftcc_lds_bytes(rx, kFftccCached) > lds)
+                fail(FI_E_UNSUPPORTED, "FFTCC window of radius " + std::to_string(rx) + " exceeds the LDS per block");
+            rad[i] = rx;
+        }
+        order.resize(n);
+        std::iota(order.begin(), order.end(), 0u);
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rad[a] < rad[b]; });
+    }
+    classes.clear();
+    for (size_t i = 0; i < n; i++) {
+        const int ri = order.empty() ? rad[i] : rad[order[i]];
+        if (classes.empty() || classes.back()->r != ri) {
+            classes.push_back(std::make_unique<FrameClass>());
+            classes.back()->r = ri;
+            classes.back()->off = i;
+        }
+        classes.back()->n++;
+    }
+    std::vector<fi_poi> layout;
+    const fi_poi* lay = tmpl_host.data();
+    if (!order.empty()) {
+        layout.resize(n);
+        for (size_t i = 0; i < n; i++) layout[i] = tmpl_host[order[i]];
+        lay = layout.data();
+    }
+
     ref_f.reserve(px);
     gx.reserve(px);
     gy.reserve(px);
@@ -532,22 +585,30 @@ void fi_frame::prepare()
     repair.reserve(cap);
     cand.reserve(cap);
     prep.reserve(cap);
-    slots.reserve(cap);
-    spectra.reserve(cap * (size_t)2 * r * (r + 1) * 2);
-    if (n > 0) {
-        FI_HIP(hipMemcpyAsync(tmpl.p, tmpl_host.data(), n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    counters.reserve(nclass() + 1);
+    h_counters.reserve(nclass() + 1);
+    queue.reserve(std::max<size_t>(8 * nclass(), 8));
+    if (n > 0) FI_HIP(hipMemcpyAsync(tmpl.p, lay, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    for (auto& cp : classes) {
+        FrameClass& c = *cp;
+        std::vector<float> t(4 * c.r);
+        dft_twiddles(2 * c.r, t.data());
+        c.tw.reserve(t.size());
+        FI_HIP(hipMemcpyAsync(c.tw.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+        c.spectra.reserve(c.n * (size_t)2 * c.r * (c.r + 1) * 2);
+        c.slots.reserve(c.n);
         FftccParams fp{};
         fp.ref = ref_f.p;
         fp.tar = ref_f.p;
         fp.w = w;
         fp.h = h;
-        fp.r = r;
+        fp.r = c.r;
         fp.mode = kFftccPrepare;
-        fp.pois = tmpl.p;
-        fp.n_pois = (int)n;
-        fp.spectra = spectra.p;
-        fp.slots = slots.p;
-        fp.twiddles = tw.p;
+        fp.pois = tmpl.p + c.off;
+        fp.n_pois = (int)c.n;
+        fp.spectra = c.spectra.p;
+        fp.slots = c.slots.p;
+        fp.twiddles = c.tw.p;
         launch_fftcc(fp, stream);
         IcgnParams ip{};
         ip.ref = ref_f.p;
@@ -555,72 +616,129 @@ void fi_frame::prepare()
         ip.gy = gy.p;
         ip.w = w;
         ip.h = h;
-        ip.pois = tmpl.p;
-        ip.n = (int)n;
-        ip.prep = prep.p;
-        ip.rx = r;
-        ip.ry = r;
+        ip.pois = tmpl.p + c.off;
+        ip.n = (int)c.n;
+        ip.prep = prep.p + c.off;
+        ip.rx = c.r;
+        ip.ry = c.r;
         launch_icgn_prep(ip, stream);
+        // the twiddle staging vector must outlive its copy
+        FI_HIP(hipStreamSynchronize(stream));
     }
     FI_HIP(hipGetLastError());
     FI_HIP(hipStreamSynchronize(stream));
+    invalidate_graphs();
     prepared = true;
 }
 
-void fi_frame::fftcc_pass(fi_poi* queue, int count, const uint32_t* slot_dev)
+void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* slot_dev)
 {
     FftccParams fp{};
     fp.ref = ref_f.p;
     fp.tar = tar_f.p;
     fp.w = ref_raw.view.width;
     fp.h = ref_raw.view.height;
-    fp.r = r;
+    fp.r = c.r;
     fp.mode = kFftccCached;
-    fp.pois = queue;
+    fp.pois = queue_base;
     fp.n_pois = count;
     fp.slot = slot_dev;
-    fp.spectra = spectra.p;
-    fp.slots = slots.p;
-    fp.twiddles = tw.p;
+    fp.spectra = c.spectra.p;
+    fp.slots = c.slots.p;
+    fp.twiddles = c.tw.p;
     launch_fftcc(fp, stream);
 }
 
-void fi_frame::icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index)
+void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* prep_index)
 {
+    if (count <= 0) return;
+    const int mode = c.sampling >= 0 ? c.sampling : sampling;
     IcgnParams ip{};
     ip.ref = ref_f.p;
     ip.gx = gx.p;
     ip.gy = gy.p;
     ip.coef = coef.p;
-    ip.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+    ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
-    ip.pois = queue;
+    ip.pois = queue_base;
     ip.n = count;
-    ip.prep = prep.p;
+    ip.prep = prep.p + c.off;
     ip.prep_index = prep_index;
-    ip.rx = r;
-    ip.ry = r;
+    ip.rx = c.r;
+    ip.ry = c.r;
     ip.conv = conv;
     ip.stop = stop;
-    ip.iters = iters.p;
-    icgn_launch(ip, shape_waves, shape_spl, stream);
-    launch_sum_iters(iters.p, count, iter_total.p, stream);
+    ip.iters = iters.p + c.off;
+    int waves = c.waves, spl = c.spl;
+    if (waves == 0 && shape_waves && icgn_shape_covers(shape_waves, shape_spl, c.r, c.r)) {
+        waves = shape_waves;
+        spl = shape_spl;
+    }
+    const int qb = c.queue_blocks >= 0 ? c.queue_blocks : persistent_blocks;
+    if (qb > 0) {
+        int ci = 0;
+        while (classes[ci].get() != &c) ci++;
+        ip.queue = queue.p + 8 * ci;
+        ip.queue_blocks = qb;
+        launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), 8, stream);
+    }
+    icgn_launch(ip, waves, spl, stream);
+    launch_sum_iters(ip.iters, count, iter_total.p, stream);
+}
+
+void fi_frame::icgn_all(fi_poi* base)
+{
+    for (auto& c : classes) icgn_pass(*c, base + c->off, (int)c->n, nullptr);
+}
+
+void fi_frame::fftcc_all(fi_poi* base)
+{
+    for (auto& c : classes) fftcc_pass(*c, base + c->off, (int)c->n, nullptr);
+}
+
+// POI-level repair of class c's m candidates (cand + off): FFTCC on the cached
+// spectra, IC-GN on the resident records, merge (cli:2955-3022)
+void fi_frame::repair_class(const FrameClass& c, int m)
+{
+    if (m <= 0) return;
+    int ci = 0;
+    while (classes[ci].get() != &c) ci++;
+    launch_gather(tmpl.p + c.off, cand.p + c.off, m, repair.p + c.off, stream);
+    fftcc_pass(c, repair.p + c.off, m, cand.p + c.off);
+    icgn_pass(c, repair.p + c.off, m, cand.p + c.off);
+    launch_merge_repair(cur.p + c.off, repair.p + c.off, cand.p + c.off, m, counters.p + ci, stream);
+    FI_HIP(hipGetLastError());
 }
 
 void fi_frame::zero_counters()
 {
-    FI_HIP(hipMemsetAsync(counters.p, 0, sizeof(FrameCounters), stream));
+    // a fill kernel, not hipMemsetAsync: this runs inside captured frame graphs
+    launch_zero_words(reinterpret_cast<uint32_t*>(counters.p), (nclass() + 1) * sizeof(FrameCounters) / 4, stream);
 }
 
-FrameCounters fi_frame::read_counters()
+FrameCounters fi_frame::read_counters(FrameCounters* per_class)
 {
-    FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, sizeof(FrameCounters), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (nclass() + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
+                          stream));
     FI_HIP(hipStreamSynchronize(stream));
-    return *h_counters.p;
+    if (per_class) std::copy(h_counters.p, h_counters.p + nclass() + 1, per_class);
+    return h_counters.p[nclass()];
 }
 
-// processFrame, opencorr_cli_cuda_mvp.cpp:3024-3210
+void fi_frame::prep_target()
+{
+    const int w = ref_raw.view.width, h = ref_raw.view.height;
+    check_table_size(w, h);
+    coef.reserve(bspline_table_floats(w, h));
+    tar_f.reserve((size_t)w * h);
+}
+
+// processFrame, opencorr_cli_cuda_mvp.cpp:3024-3210.  The device work up to
+// the frame's decisions (target prefilter, seeding, IC-GN of every radius
+// class, fallback scan, QA, counter read-back) is one segment, run eagerly or
+// replayed as a captured hipGraph; the rare fallback and repair steps after it
+// run eagerly.
 void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 {
     DeviceGuard g(device);
@@ -628,67 +746,111 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     if (!tar_ready) fail(FI_E_LOGIC, "frame driver needs an uploaded target");
     const int w = ref_raw.view.width, h = ref_raw.view.height;
     const int cnt = (int)n;
+    const int C = (int)nclass();
     fi_frame_stats st{};
     st.frame_index = frame_index;
     st.baseline_bad_ratio = NAN;
     st.fallback_threshold = NAN;
-
-    FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
-    FI_HIP(hipEventRecord(ev[0], stream));
-    check_table_size(w, h);
-    coef.reserve(bspline_table_floats(w, h));
-    tar_f.reserve((size_t)w * h);
-    launch_bspline(tar.view, coef.p, stream);
-    launch_to_float(tar.view, tar_f.p, stream);
-    FI_HIP(hipEventRecord(ev[1], stream));
+    prep_target();
 
     const bool skip = (policy & FI_POLICY_SKIP_FFTCC) && baseline_ready && have_prev;
-    float bad_ratio = 0.f;
-    double fftcc_ms = 0.0, icgn_ms = 0.0;
     const bool learned = !skip && (policy & FI_POLICY_LEARNED_SEED) && seeder != nullptr;
-    if (learned) {
-        // learned seeds (fused inference), IC-GN, FFTCC fallback for the POIs
-        // whose result has a low ZNCC or failed, per-POI merge
-        FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
-        seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, seed_ok.p, stream);
-        FI_HIP(hipEventRecord(ev[2], stream));
-        icgn_pass(cur.p, cnt, nullptr);
-        FI_HIP(hipEventRecord(ev[3], stream));
+    const int collect = (policy & FI_POLICY_POI_FALLBACK) ? 1 : 0;
+    const int mode = learned ? 0 : (skip ? 2 : 1);
+
+    auto segment = [&](bool timed) {
+        launch_zero_words(reinterpret_cast<uint32_t*>(iter_total.p), 2, stream);
         zero_counters();
-        launch_seed_scan(cur.p, seed_ok.p, cnt, seed_threshold, counters.p, cand.p, stream);
-        FI_HIP(hipGetLastError());
-        const FrameCounters c = read_counters();
-        st.seed_pois = cnt;
-        st.seed_fallback_count = c.candidates;
-        if (c.candidates > 0) {
-            const int m = (int)c.candidates;
-            launch_gather(tmpl.p, cand.p, m, repair.p, stream);
-            fftcc_pass(repair.p, m, cand.p);
-            icgn_pass(repair.p, m, cand.p);
-            zero_counters();
-            launch_merge_repair(cur.p, repair.p, cand.p, m, counters.p, stream);
-            FI_HIP(hipGetLastError());
+        if (timed) FI_HIP(hipEventRecord(ev[0], stream));
+        launch_bspline(tar.view, coef.p, stream);
+        launch_to_float(tar.view, tar_f.p, stream);
+        if (timed) FI_HIP(hipEventRecord(ev[1], stream));
+        if (mode == 0) {
+            FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+            seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, seed_ok.p, stream);
+        } else if (mode == 1) {
+            FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+            fftcc_all(cur.p);
+        } else {
+            launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);
         }
-        st.fftcc_pois = c.candidates;
-    } else if (!skip) {
-        FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
-        fftcc_pass(cur.p, cnt, nullptr);
-        FI_HIP(hipEventRecord(ev[2], stream));
-        icgn_pass(cur.p, cnt, nullptr);
-        FI_HIP(hipEventRecord(ev[3], stream));
+        if (timed) FI_HIP(hipEventRecord(ev[2], stream));
+        icgn_all(cur.p);
+        if (timed) FI_HIP(hipEventRecord(ev[3], stream));
+        for (int ci = 0; ci < C; ci++) {
+            const FrameClass& c = *classes[ci];
+            if (mode == 0)
+                launch_seed_scan(cur.p + c.off, seed_ok.p + c.off, (int)c.n, seed_threshold, counters.p + ci,
+                                 cand.p + c.off, stream);
+            else if (mode == 2)
+                launch_scan(cur.p + c.off, prev.p + c.off, (int)c.n, collect, counters.p + ci, cand.p + c.off,
+                            stream);
+        }
+        launch_qa(cur.p, cnt, counters.p + C, stream);
+        FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
+                              stream));
+        FI_HIP(hipMemcpyAsync(h_iter_total.p, iter_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              stream));
         FI_HIP(hipGetLastError());
+    };
+
+    if (use_graph) {
+        const uint64_t key[8] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
+                                  (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tar.view.data,
+                                  (uint64_t)(uintptr_t)coef.p, (uint64_t)(uintptr_t)tar_f.p,
+                                  (uint64_t)(uintptr_t)seeder };
+        hipGraphExec_t exec = nullptr;
+        for (auto& gr : graphs)
+            if (std::memcmp(gr.key, key, sizeof(key)) == 0) exec = gr.exec;
+        if (!exec) {
+            FI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                segment(false);
+            } catch (...) {
+                hipGraph_t broken = nullptr;
+                (void)hipStreamEndCapture(stream, &broken);
+                if (broken) (void)hipGraphDestroy(broken);
+                throw;
+            }
+            hipGraph_t graph = nullptr;
+            FI_HIP(hipStreamEndCapture(stream, &graph));
+            const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            FI_HIP(e);
+            Graph gr;
+            std::memcpy(gr.key, key, sizeof(key));
+            gr.exec = exec;
+            graphs.push_back(gr);
+        }
+        FI_HIP(hipEventRecord(ev[0], stream));
+        FI_HIP(hipGraphLaunch(exec, stream));
+    } else {
+        segment(true);
+    }
+    FI_HIP(hipStreamSynchronize(stream));
+    std::vector<FrameCounters> pc(h_counters.p, h_counters.p + C + 1);
+    FrameCounters qa = pc[C];
+    bool redo_qa = false;
+
+    if (mode == 0) {
+        unsigned long long m_all = 0;
+        for (int ci = 0; ci < C; ci++) m_all += pc[ci].candidates;
+        st.seed_pois = cnt;
+        st.seed_fallback_count = (int64_t)m_all;
+        st.fftcc_pois = (int64_t)m_all;
+        if (m_all > 0) {
+            for (int ci = 0; ci < C; ci++) repair_class(*classes[ci], (int)pc[ci].candidates);
+            redo_qa = true;
+        }
+    } else if (mode == 1) {
         st.fftcc_pois = cnt;
     } else {
-        launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);
-        FI_HIP(hipEventRecord(ev[2], stream));
-        icgn_pass(cur.p, cnt, nullptr);
-        FI_HIP(hipEventRecord(ev[3], stream));
-        zero_counters();
-        const int collect = (policy & FI_POLICY_POI_FALLBACK) ? 1 : 0;
-        launch_scan(cur.p, prev.p, cnt, collect, counters.p, cand.p, stream);
-        FI_HIP(hipGetLastError());
-        const FrameCounters c = read_counters();
-        const float ratio = cnt ? (float)c.bad / (float)cnt : 0.f;
+        unsigned long long bad = 0, m_all = 0;
+        for (int ci = 0; ci < C; ci++) {
+            bad += pc[ci].bad;
+            m_all += pc[ci].candidates;
+        }
+        const float ratio = cnt ? (float)bad / (float)cnt : 0.f;
         st.skip_fftcc_used = 1;
         st.pre_repair_bad_ratio = ratio;
         const float threshold = baseline_bad + kBadRatioMargin;
@@ -697,33 +859,37 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             st.fallback_used = 1;
             FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
-            fftcc_pass(cur.p, cnt, nullptr);
-            icgn_pass(cur.p, cnt, nullptr);
+            fftcc_all(cur.p);
+            icgn_all(cur.p);
             launch_merge_frame_fallback(save.p, cur.p, cnt, stream);
             FI_HIP(hipGetLastError());
             st.fftcc_pois = cnt;
-        } else if (collect && c.candidates > 0) {
+            redo_qa = true;
+        } else if (collect && m_all > 0) {
             // POI-level repair on cached spectra and resident Hessians (:2955-3022)
-            const int m = (int)c.candidates;
-            launch_gather(tmpl.p, cand.p, m, repair.p, stream);
-            fftcc_pass(repair.p, m, cand.p);
-            icgn_pass(repair.p, m, cand.p);
-            zero_counters();
-            launch_merge_repair(cur.p, repair.p, cand.p, m, counters.p, stream);
-            FI_HIP(hipGetLastError());
-            const FrameCounters rc = read_counters();
-            st.poi_fallback_count = m;
-            st.poi_fallback_recovered = (int32_t)rc.recovered;
-            st.fftcc_pois = m;
+            for (int ci = 0; ci < C; ci++) repair_class(*classes[ci], (int)pc[ci].candidates);
+            std::vector<FrameCounters> rc(C + 1);
+            read_counters(rc.data());
+            unsigned long long rec = 0;
+            for (int ci = 0; ci < C; ci++) rec += rc[ci].recovered;
+            st.poi_fallback_count = (int32_t)m_all;
+            st.poi_fallback_recovered = (int32_t)rec;
+            st.fftcc_pois = (int64_t)m_all;
+            redo_qa = true;
         }
     }
+    if (redo_qa) {
+        launch_zero_words(reinterpret_cast<uint32_t*>(counters.p + C), sizeof(FrameCounters) / 4, stream);
+        launch_qa(cur.p, cnt, counters.p + C, stream);
+        FI_HIP(hipGetLastError());
+        FI_HIP(hipMemcpyAsync(h_iter_total.p, iter_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              stream));
+        qa = read_counters(nullptr);
+    }
     FI_HIP(hipEventRecord(ev[4], stream));
-    zero_counters();
-    launch_qa(cur.p, cnt, counters.p, stream);
-    FI_HIP(hipGetLastError());
-    FI_HIP(hipMemcpyAsync(h_iter_total.p, iter_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream));
-    const FrameCounters q = read_counters();
-    bad_ratio = cnt ? (float)q.bad / (float)cnt : 0.f;
+    FI_HIP(hipEventSynchronize(ev[4]));
+
+    const float bad_ratio = cnt ? (float)qa.bad / (float)cnt : 0.f;
     if (!skip) st.pre_repair_bad_ratio = bad_ratio;
     if ((policy & FI_POLICY_SKIP_FFTCC) && !baseline_ready && !skip) {
         baseline_bad = bad_ratio;
@@ -734,29 +900,120 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         st.fallback_threshold = baseline_bad + kBadRatioMargin;
     }
     st.bad_ratio = bad_ratio;
-    st.valid_count = (int64_t)q.valid;
-    st.valid_ratio = cnt ? (float)q.valid / (float)cnt : 0.f;
+    st.valid_count = (int64_t)qa.valid;
+    st.valid_ratio = cnt ? (float)qa.valid / (float)cnt : 0.f;
     st.iteration_sum = (int64_t)*h_iter_total.p;
-    st.iteration_mean = q.iter_finite ? (float)((double)q.iter_sum / (double)q.iter_finite) : 0.f;
-    st.iteration_max = (int32_t)q.iter_max;
+    st.iteration_mean = qa.iter_finite ? (float)((double)qa.iter_sum / (double)qa.iter_finite) : 0.f;
+    st.iteration_max = (int32_t)qa.iter_max;
     float ms = 0.f;
     FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
     st.device_ms = ms;
-    FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-    st.bspline_ms = ms;
-    FI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
-    if (learned) st.seed_ms = ms;
-    else fftcc_ms = ms;
-    FI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-    icgn_ms = ms;
-    st.fftcc_ms = fftcc_ms;
-    st.icgn_ms = icgn_ms;
+    if (use_graph) {
+        // stages inside a graph are not timed separately
+        st.bspline_ms = st.fftcc_ms = st.icgn_ms = NAN;
+        st.seed_ms = mode == 0 ? NAN : 0.0;
+    } else {
+        FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        st.bspline_ms = ms;
+        FI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
+        if (mode == 0)
+            st.seed_ms = ms;
+        else
+            st.fftcc_ms = mode == 1 ? ms : 0.0;
+        FI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        st.icgn_ms = ms;
+    }
 
     std::swap(cur.p, prev.p);
     std::swap(cur.cap, prev.cap);
     have_prev = true;
     frame_index++;
     if (stats) *stats = st;
+}
+
+// Autotuner: per radius class, every covering lane shape x sampling mode x
+// launch mode (one block per POI / persistent work queue), timed on the
+// FFTCC-seeded POIs of the uploaded target (median of three runs); the fastest
+// is kept.  The sequence state (previous frame) is not touched.
+int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_n)
+{
+    DeviceGuard g(device);
+    if (!prepared) prepare();
+    if (!tar_ready) fail(FI_E_LOGIC, "autotune needs an uploaded target");
+    prep_target();
+    launch_bspline(tar.view, coef.p, stream);
+    launch_to_float(tar.view, tar_f.p, stream);
+    // FFTCC seeds of the template in `save`
+    FI_HIP(hipMemcpyAsync(save.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+    fftcc_all(save.p);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
+    int cus = 0;
+    FI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    size_t m = 0;
+    int timed = 0;
+    for (auto& cp : classes) {
+        FrameClass& c = *cp;
+        if (c.n == 0) continue;
+        int ws[64], ss[64];
+        const int ns = icgn_shape_candidates(c.r, c.r, ws, ss, 64);
+        double best = 1e30;
+        int bw = 0, bs = 0, bsamp = -1, bq = 0;
+        size_t best_entry = SIZE_MAX;
+        for (int si = 0; si < ns; si++) {
+            for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE }) {
+                // resident blocks of the queue launch: 12 waves per CU at the
+                // kernel's occupancy target of 3 waves per SIMD
+                for (int qb : { 0, cus * 12 / ws[si] }) {
+                    c.waves = ws[si];
+                    c.spl = ss[si];
+                    c.sampling = samp;
+                    c.queue_blocks = qb;
+                    float t[3];
+                    for (int rep = 0; rep < 3; rep++) {
+                        FI_HIP(hipMemcpyAsync(repair.p + c.off, save.p + c.off, c.n * sizeof(fi_poi),
+                                              hipMemcpyDeviceToDevice, stream));
+                        FI_HIP(hipEventRecord(ev[5], stream));
+                        icgn_pass(c, repair.p + c.off, (int)c.n, nullptr);
+                        FI_HIP(hipEventRecord(ev[6], stream));
+                        FI_HIP(hipEventSynchronize(ev[6]));
+                        FI_HIP(hipEventElapsedTime(&t[rep], ev[5], ev[6]));
+                    }
+                    std::sort(t, t + 3);
+                    timed++;
+                    if (report && m < report_len) {
+                        fi_tune_entry& e = report[m];
+                        e.radius = c.r;
+                        e.waves = ws[si];
+                        e.rows_per_band = ss[si];
+                        e.sampling = samp;
+                        e.persistent = qb;
+                        e.chosen = 0;
+                        e.ms = t[1];
+                    }
+                    if (t[1] < best) {
+                        best = t[1];
+                        bw = ws[si];
+                        bs = ss[si];
+                        bsamp = samp;
+                        bq = qb;
+                        best_entry = m;
+                    }
+                    m++;
+                }
+            }
+        }
+        c.waves = bw;
+        c.spl = bs;
+        c.sampling = bsamp;
+        c.queue_blocks = bq;
+        if (report && best_entry < report_len) report[best_entry].chosen = 1;
+    }
+    FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
+    FI_HIP(hipStreamSynchronize(stream));
+    invalidate_graphs();
+    if (report_n) *report_n = std::min(m, report_len);
+    return timed;
 }
 
 void fi_frame::download(fi_poi* out, size_t count)
@@ -766,6 +1023,13 @@ void fi_frame::download(fi_poi* out, size_t count)
     if (count == 0) return;
     if (out == nullptr) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");
     DeviceGuard g(device);
-    FI_HIP(hipMemcpyAsync(out, prev.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
+    if (order.empty()) {
+        FI_HIP(hipMemcpyAsync(out, prev.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
+        FI_HIP(hipStreamSynchronize(stream));
+        return;
+    }
+    h_out.reserve(n);
+    FI_HIP(hipMemcpyAsync(h_out.p, prev.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
     FI_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < n; i++) out[order[i]] = h_out.p[i];
 }

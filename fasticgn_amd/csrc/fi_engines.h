@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <memory>
 #include <vector>
 
 #include "fi_host.h"
@@ -78,47 +79,84 @@ struct fi_icgn {
     void compute(fi_poi* pois, size_t n);
 };
 
+// One radius class of the frame driver's template: POIs [off, off + n) of the
+// radius-sorted device layout, with their FFTCC state and IC-GN launch
+// configuration (lane shape, sampling, persistent queue).
+struct FrameClass {
+    int r = 0;
+    size_t off = 0, n = 0;
+    int waves = 0, spl = 0;  // IC-GN lane shape (0, 0 = the driver's)
+    int sampling = -1;       // -1 = the driver's sampling
+    int queue_blocks = -1;   // persistent work-queue blocks (0 = one block per POI, -1 = the driver's)
+    fi::DevArray<float> tw, spectra;
+    fi::DevArray<fi::FftccSlot> slots;
+};
+
 struct fi_frame {
     int device = 0;
-    int r = 0;
+    int r = 0;                     // subset radius of every POI unless self_adaptive
+    bool self_adaptive = false;    // per-POI subset_radius (square) instead of r
     float conv = 0.001f, stop = 10.f;
     int shape_waves = 0, shape_spl = 0;
     int sampling = FI_SAMPLING_AUTO;
+    bool use_graph = false;        // capture each frame's device work as a hipGraph
+    int persistent_blocks = 0;     // IC-GN persistent work-queue blocks (0 = one block per POI)
     fi_seeder* seeder = nullptr;   // learned seeder (FI_POLICY_LEARNED_SEED)
     float seed_threshold = 0.8f;   // ZNCC below which a learned seed falls back to FFTCC
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw, tar;
-    fi::DevArray<float> ref_f, gx, gy, coef, tw, spectra, tar_f;
-    fi::DevArray<fi::FftccSlot> slots;
+    fi::DevArray<float> ref_f, gx, gy, coef, tar_f;
     fi::DevArray<fi::PoiPrep> prep;
-    std::vector<fi_poi> tmpl_host;
+    std::vector<fi_poi> tmpl_host;        // template in the caller's order
+    std::vector<uint32_t> order;          // layout position -> template index (empty: identity)
+    std::vector<std::unique_ptr<FrameClass>> classes;
     fi::DevArray<fi_poi> tmpl, cur, prev, save, repair;
     fi::DevArray<uint32_t> cand, seed_ok;
+    // counters[c] for the scans and repairs of class c, counters[classes] for QA
     fi::DevArray<fi::FrameCounters> counters;
     fi::PinnedArray<fi::FrameCounters> h_counters;
     fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry
     fi::DevArray<unsigned long long> iter_total;
     fi::PinnedArray<unsigned long long> h_iter_total;
+    fi::DevArray<int> queue;       // persistent-queue claim counters, 8 per class
+    fi::PinnedArray<fi_poi> h_out;
     size_t n = 0;
     bool ref_set = false, pois_set = false, prepared = false, tar_ready = false;
     bool have_prev = false, baseline_ready = false;
     float baseline_bad = NAN;
     int frame_index = 0;
     hipEvent_t ev[8] = {};
+    struct Graph {
+        uint64_t key[8];
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;     // instantiated frame graphs (cleared on any change)
 
     fi_frame(int radius, float conv, float stop, int dev);
     ~fi_frame();
     void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
     void set_pois(const fi_poi* pois, size_t n);
+    void set_self_adaptive(bool on);
     void upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12);
     void run(unsigned policy, fi_frame_stats* stats);
     void download(fi_poi* out, size_t n);
     void reset_sequence();
+    void invalidate_graphs();
+    // Times the IC-GN lane shapes, sampling modes and launch modes on the
+    // uploaded target per radius class and keeps the fastest; returns the
+    // number of configurations timed.
+    int autotune(fi_tune_entry* report, size_t report_len, size_t* report_n);
 
 private:
     void prepare();
-    void fftcc_pass(fi_poi* queue, int count, const uint32_t* slot_dev);
-    void icgn_pass(fi_poi* queue, int count, const uint32_t* prep_index);
-    fi::FrameCounters read_counters();
+    void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev);
+    void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index);
+    void icgn_all(fi_poi* base);
+    void fftcc_all(fi_poi* base);
+    void repair_class(const FrameClass& c, int m);
+    void prep_target();
+    fi::FrameCounters read_counters(fi::FrameCounters* per_class);
     void zero_counters();
+    size_t nclass() const { return classes.size(); }
 };
+

@@ -124,5 +124,7 @@ void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float thresh
                       uint32_t* candidates, hipStream_t s);
 // *total += sum of iters[0..n)
 void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s);
+// p[0..n) = 0 (32-bit words), by a kernel
+void launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
 
 }  // namespace fi

@@ -23,6 +23,8 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s);
 bool icgn_shape_supported(int waves, int spl);
 bool icgn_shape_covers(int waves, int spl, int rx, int ry);
 void icgn_default_shape(int rx, int ry, int& waves, int& spl);
+// Every instantiated shape that covers the subset (at most cap); returns the count.
+int icgn_shape_candidates(int rx, int ry, int* waves, int* spl, int cap);
 bool launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s);
 bool launch_icgn(const IcgnParams& p, hipStream_t s);
 

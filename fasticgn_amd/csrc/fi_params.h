@@ -82,6 +82,10 @@ struct IcgnParams {
     uint32_t* iters;          // optional: executed iterations of launch entry b
     // per-POI LDS tile of the target (set by the launcher; tile_w = 0: none)
     int tile_w, tile_h, tile_stride;
+    // persistent work queue (nullptr: one block per entry): 8 zeroed claim
+    // counters, and the number of resident blocks to launch
+    int* queue;
+    int queue_blocks;
 };
 
 }  // namespace fi

@@ -244,6 +244,27 @@ bool icgn_shape_covers(int waves, int spl, int rx, int ry)
     return (R == spl && exact_shape(waves, spl)) || generic_shape(waves, spl);
 }
 
+int icgn_shape_candidates(int rx, int ry, int* waves, int* spl, int cap)
+{
+    int m = 0;
+    auto add = [&](int W, int S) {
+        if (!icgn_shape_covers(W, S, rx, ry)) return;
+        for (int i = 0; i < m; i++)
+            if (waves[i] == W && spl[i] == S) return;
+        if (m < cap) {
+            waves[m] = W;
+            spl[m] = S;
+            m++;
+        }
+    };
+#define X(W, S) add(W, S);
+    FI_ICGN_EXACT_A(X)
+    FI_ICGN_EXACT_B(X)
+    FI_ICGN_GENERIC(X)
+#undef X
+    return m;
+}
+
 void icgn_default_shape(int rx, int ry, int& waves, int& spl)
 {
     for (int W : { 1, 2, 4 }) {

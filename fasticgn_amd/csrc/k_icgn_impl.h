@@ -32,6 +32,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fi_device.h"
 #include "fi_launch.h"
 
@@ -343,18 +345,13 @@ constexpr int kMaxOverlap = 4;
 #endif
 
 // One block of WAVES wavefronts per POI; each lane owns SPL rows of one subset
-// column.  EXACT: the host guarantees R == SPL, so no slot is skipped.
+// column.  EXACT: the host guarantees R == SPL, so no slot is skipped.  Every
+// early return is taken by the whole block.
 template <int WAVES, int SPL, bool EXACT>
-__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
-void k_icgn(IcgnParams p)
+__device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, float* red, int* flags, float* tile)
 {
     constexpr int NT = 64 * WAVES;
-    __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
-    __shared__ int flags[WAVES];
-    extern __shared__ float tile[];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b >= p.n) return;
     const int i = p.index ? (int)p.index[b] : b;
     fi_poi* poi = p.pois + i;
     const float x = poi->x, y = poi->y;
@@ -622,6 +619,54 @@ void k_icgn(IcgnParams p)
     }
 }
 
+// Persistent work queue: entries [0, n) split into kQueueParts contiguous
+// ranges, one per XCD (workgroups are dealt to the XCDs round-robin, so block
+// b runs on XCD b % 8 and neighbouring POIs share that XCD's L2); a block
+// claims kQueueChunk entries at a time from its own range, then from the
+// others' until every range is drained.  p.queue holds kQueueParts zeroed
+// counters.
+constexpr int kQueueParts = 8;
+constexpr int kQueueChunk = 4;
+
+template <int WAVES, int SPL, bool EXACT>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
+void k_icgn(IcgnParams p)
+{
+    __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
+    __shared__ int flags[WAVES];
+    extern __shared__ float tile[];
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b < p.n) icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+}
+
+template <int WAVES, int SPL, bool EXACT>
+__global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
+void k_icgn_queue(IcgnParams p)
+{
+    __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
+    __shared__ int flags[WAVES];
+    __shared__ int claim;
+    extern __shared__ float tile[];
+    const int home = (int)(blockIdx.x % kQueueParts);
+    for (int k = 0; k < kQueueParts; k++) {
+        const int part = (home + k) % kQueueParts;
+        const int lo = (int)((long long)p.n * part / kQueueParts);
+        const int hi = (int)((long long)p.n * (part + 1) / kQueueParts);
+        for (;;) {
+            if (threadIdx.x == 0) claim = lo + atomicAdd(p.queue + part, kQueueChunk);
+            __syncthreads();
+            const int b0 = claim;
+            __syncthreads();  // every thread has read claim before it is rewritten
+            if (b0 >= hi) break;
+            const int b1 = min(b0 + kQueueChunk, hi);
+            for (int b = b0; b < b1; b++) {
+                icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+                __syncthreads();  // shared state is reused by the next entry
+            }
+        }
+    }
+}
+
 // Instantiated lane shapes (waves, rows per band).  EXACT_A/B are compiled in
 // k_icgn_xa.hip / k_icgn_xb.hip, GENERIC in k_icgn.hip.
 #define FI_ICGN_EXACT_A(X) \
@@ -635,7 +680,11 @@ template <int WAVES, int SPL, bool EXACT>
 void launch_icgn_t(const IcgnParams& p, hipStream_t s)
 {
     const size_t lds = p.tile_w > 0 ? (size_t)p.tile_stride * p.tile_h * sizeof(float) : 0;
-    hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), lds, s, p);
+    if (p.queue)
+        hipLaunchKernelGGL((k_icgn_queue<WAVES, SPL, EXACT>), dim3(std::max(1, std::min(p.n, p.queue_blocks))),
+                           dim3(64 * WAVES), lds, s, p);
+    else
+        hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), lds, s, p);
 }
 
 }  // namespace fi

@@ -223,6 +223,21 @@ __global__ __launch_bounds__(kPolicyThreads) void k_sum_iters(const uint32_t* __
 
 static dim3 grid_for(int n) { return dim3((n + kPolicyThreads - 1) / kPolicyThreads); }
 
+// Zero n 32-bit words (a kernel rather than hipMemsetAsync, so that frame
+// graphs capture no memset node).
+__global__ void k_zero_words(uint32_t* __restrict__ p, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+void launch_zero_words(uint32_t* p, size_t n, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_zero_words, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
+                           p, n);
+}
+
 void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, fi_poi* out, int n,
                                hipStream_t s)
 {

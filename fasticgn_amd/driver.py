@@ -75,6 +75,31 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_set_seeder", self._h, seeder.handle if seeder is not None else None,
                  float(zncc_threshold))
 
+    def set_self_adaptive(self, on: bool = True) -> None:
+        """Mixed subset radii: every template POI's (square) subset_radius is
+        its FFTCC and IC-GN radius (DIC::setSelfAdaptive); call before
+        set_pois takes effect on the next frame."""
+        nat.call("fi_frame_set_self_adaptive", self._h, int(bool(on)))
+
+    def set_graph(self, on: bool = True) -> None:
+        """Replay each frame's device work as a captured hipGraph."""
+        nat.call("fi_frame_set_graph", self._h, int(bool(on)))
+
+    def set_persistent(self, blocks: int) -> None:
+        """IC-GN persistent work-queue launch with `blocks` resident blocks
+        (0 = one block per POI) for every radius class."""
+        nat.call("fi_frame_set_persistent", self._h, int(blocks))
+
+    def autotune(self) -> list:
+        """Time every IC-GN lane shape, sampling mode and launch mode per
+        radius class on the uploaded target and keep the fastest; returns the
+        timed configurations (dicts, 'chosen' marks the kept ones)."""
+        cap = 512
+        rep = (nat.TuneEntry * cap)()
+        m = ctypes.c_size_t(0)
+        nat.call("fi_frame_autotune", self._h, ctypes.cast(rep, ctypes.c_void_p), cap, ctypes.byref(m))
+        return [rep[i].as_dict() for i in range(m.value)]
+
     def reset_sequence(self) -> None:
         nat.call("fi_frame_reset_sequence", self._h)
 

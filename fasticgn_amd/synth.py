@@ -206,6 +206,71 @@ def config1(seed: int = 20261, size: int = 2048, grid: int = 5, radius: int = 15
     return Scene(f"config1_{size}", ref, tar, pois, radius, truth, meta)
 
 
+RADII_MIXED = (10, 15, 20)
+
+
+@dataclass
+class Sequence:
+    """A fixed reference and a family of targets: frame k is the reference
+    warped by the affine map of step k (frame(k) renders it)."""
+    name: str
+    ref: np.ndarray
+    pois: np.ndarray
+    frames: int
+    frame: Callable[[int], np.ndarray]
+    truth: Callable[[int, np.ndarray, np.ndarray], tuple]  # (k, X, Y) -> (u, v, ux, uy, vx, vy)
+    meta: dict = field(default_factory=dict)
+    depth12: str = "right"
+
+
+def config2(size: int = 1024, frames: int = 500, grid: int = 5, seed: int = 20262,
+            radii=RADII_MIXED) -> Sequence:
+    """Batched multi-frame sequence: size^2 u16 holding 12-bit values
+    (background 640, peak 2880, noise sigma 24 grey levels), one fixed
+    reference; target k is the reference resampled (as every target here)
+    under the map of frame k (0 <= k < frames) translates by t_k = 15 k / 499 px
+    along 30 degrees, (u, v) = t_k (cos 30, sin 30), plus the equibiaxial strain
+    eps_k = 5e-3 k / 499 about the image centre (ux = vy = eps_k).  POIs on a
+    `grid` px grid at margin max(radii) + 2; POI i has the square subset radius
+    radii[i % len(radii)] (self-adaptive subsets)."""
+    rng = np.random.default_rng(seed)
+    n_dots = int(round(6000 * size * size / (512 * 512)))
+    cx, cy, s = dot_field(size, size, n_dots, rng)
+    ref = quantize(render_dots(cx, cy, s, size, size, 640.0, 2240.0), rng, 24.0, np.uint16, 4095.0)
+    c = ((size - 1) / 2.0, (size - 1) / 2.0)
+    dirx, diry = math.cos(math.pi / 6), math.sin(math.pi / 6)
+
+    def params(k: int):
+        t = 15.0 * k / 499.0
+        eps = 5e-3 * k / 499.0
+        return t * dirx, t * diry, eps
+
+    def frame(k: int) -> np.ndarray:
+        tu, tv, eps = params(k)
+
+        def inverse_map(x, y):
+            return c[0] + (x - c[0] - tu) / (1.0 + eps), c[1] + (y - c[1] - tv) / (1.0 + eps)
+
+        return warp_target(ref, inverse_map, np.uint16, 4095.0)
+
+    def truth(k: int, X, Y):
+        tu, tv, eps = params(k)
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        one = np.ones_like(X)
+        return tu + eps * (X - c[0]), tv + eps * (Y - c[1]), eps * one, 0 * one, 0 * one, eps * one
+
+    pois = grid_queue(size, size, max(radii), grid)
+    r = np.array([radii[i % len(radii)] for i in range(len(pois))], dtype=np.float32)
+    pois["subset_radius"][:, 0] = r
+    pois["subset_radius"][:, 1] = r
+    meta = {"seed": seed, "n_dots": n_dots, "frames": frames, "grid": grid, "margin": max(radii) + 2,
+            "radii": list(radii), "radius_rule": "radii[i % 3]", "direction_deg": 30.0,
+            "translation": "15 k / 499 px", "strain": "5e-3 k / 499 (ux = vy)", "noise_sigma": 24.0,
+            "background": 640, "peak": 2880, "bit_depth": 12, "resample": "keys-cubic a=-0.5"}
+    return Sequence(f"config2_{size}", ref, pois, frames, frame, truth, meta)
+
+
 def lcg_pair(width: int = 192, height: int = 128, seed: int = 12345):
     """The reference's FFTCC known-answer images
     (smoke/library_boundary_probe.cpp:41-47): LCG pixels, target = reference

@@ -193,6 +193,37 @@ int fi_frame_reset_sequence(fi_frame* h);
 int fi_frame_set_shape(fi_frame* h, int waves, int rows_per_band);
 /* IC-GN target sampling for this driver (see fi_icgn_set_sampling). */
 int fi_frame_set_sampling(fi_frame* h, int mode);
+/* Mixed subset radii (DIC::setSelfAdaptive, src/oc_dic.h:44-65, with the
+ * per-POI subset_radius of ICGN2D1, src/oc_icgn.cpp:176-182): each template
+ * POI carries a square subset_radius (1..31) used by both FFTCC and IC-GN;
+ * POIs are grouped into radius classes on the device (downloads keep the
+ * template order).  Off: every POI uses the driver's radius. */
+int fi_frame_set_self_adaptive(fi_frame* h, int on);
+/* Capture each frame's device work (target prefilter, seeding, IC-GN of every
+ * radius class, fallback scan, QA) as a hipGraph and replay it; the rare
+ * fallback and repair steps still run eagerly.  Results are byte-identical to
+ * the eager frame; per-stage times in fi_frame_stats are NaN. */
+int fi_frame_set_graph(fi_frame* h, int on);
+/* IC-GN launch of every radius class: blocks > 0 runs the persistent
+ * work-queue kernel with that many resident blocks claiming POIs from
+ * per-XCD queues; 0 (default) launches one block per POI.  Replaces tuned
+ * launch modes. */
+int fi_frame_set_persistent(fi_frame* h, int blocks);
+/* Autotuner: on the uploaded target (FFTCC-seeded), time every IC-GN lane
+ * shape, sampling mode and launch mode (one block per POI, or the persistent
+ * work queue) for each radius class and keep the fastest.  report (may be
+ * NULL) receives up to report_len timed configurations; *report_n (may be
+ * NULL) the number timed.  Does not advance the frame sequence. */
+typedef struct fi_tune_entry {
+    int32_t radius;
+    int32_t waves;
+    int32_t rows_per_band;
+    int32_t sampling;     /* FI_SAMPLING_* */
+    int32_t persistent;   /* resident blocks of the work-queue launch, 0 = one block per POI */
+    int32_t chosen;       /* 1: kept for the radius class */
+    double ms;            /* median IC-GN time of the class */
+} fi_tune_entry;
+int fi_frame_autotune(fi_frame* h, fi_tune_entry* report, size_t report_len, size_t* report_n);
 
 /* ---- learned seeder --------------------------------------------------------
  * No reference counterpart: the seeder that replaces FFTCC2D as the IC-GN

@@ -1,0 +1,158 @@
+"""Batched multi-frame path on the device (through the C ABI): mixed subset
+radii (one radius class per radius), frames replayed as captured hipGraphs,
+the persistent work-queue IC-GN launch and the autotuner.
+
+Bars: the first frame of every radius class meets the parity gate against the
+CPU oracle run with that radius (validity flags identical; u, v median
+<= 1e-3, p99 <= 1e-2, max <= 0.5 px); a mixed-radius sequence is
+byte-identical to one single-radius driver per class; graph replay and the
+persistent launch are byte-identical to the eager, one-block-per-POI frames;
+the tuned configuration meets the parity gate against the default one.
+"""
+import numpy as np
+import pytest
+
+import fasticgn_amd as fa
+from fasticgn_amd import synth
+from tests.parity import compare
+
+pytestmark = pytest.mark.gpu
+
+STEPS = [0, 8, 16, 24, 32, 40]  # frames of the 500-frame motion (0.24 px apart)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def seq():
+    return synth.config2(size=512, frames=500)
+
+
+@pytest.fixture(scope="module")
+def targets(seq):
+    return {k: seq.frame(k) for k in STEPS}
+
+
+def img(a):
+    return fa.Image2D(a, depth12="right")
+
+
+def mixed_driver(seq, **kw):
+    d = fa.FrameDriver(15)
+    d.set_self_adaptive(True)
+    d.set_reference(img(seq.ref))
+    d.set_pois(seq.pois)
+    if kw.get("graph"):
+        d.set_graph(True)
+    if kw.get("persistent"):
+        d.set_persistent(kw["persistent"])
+    return d
+
+
+def run_sequence(d, targets, policy=fa.driver.DEFAULT_POLICY):
+    out = []
+    for k in STEPS:
+        res, st = d.process(img(targets[k]), policy=policy)
+        out.append((res, st))
+    return out
+
+
+def same_bytes(a, b):
+    return a.tobytes() == b.tobytes()
+
+
+def test_mixed_radii_first_frame_matches_oracle(oracle, seq, targets):
+    k = STEPS[-1]
+    with mixed_driver(seq) as d:
+        res, st = d.process(img(targets[k]), policy=0)
+    say("mixed first frame", {x: st[x] for x in ("valid_count", "iteration_mean", "fftcc_pois")})
+    assert st["fftcc_pois"] == len(seq.pois)
+    ref32 = oracle.as_f32(seq.ref, "right")
+    tar32 = oracle.as_f32(targets[k], "right")
+    r = seq.pois["subset_radius"][:, 0].astype(int)
+    assert np.array_equal(res["subset_radius"][:, 0].astype(int), r)
+    for rad in synth.RADII_MIXED:
+        idx = np.nonzero(r == rad)[0][::3]  # a third of each class
+        o = seq.pois[idx].copy()
+        oracle.fftcc(ref32, tar32, rad, o)
+        g = res[idx]
+        # FFTCC integer seeds are bit-exact: IC-GN starts from them (result[0:2])
+        assert np.array_equal(o["deformation"][:, 0], g["result"][:, 0])
+        assert np.array_equal(o["deformation"][:, 6], g["result"][:, 1])
+        oracle.icgn(ref32, tar32, rad, o)
+        cmp = compare(o, g)
+        say("radius", rad, {x: cmp.get(x) for x in ("n", "status_mismatch", "p99_du", "p99_dv", "max_du")})
+        assert cmp["pass"], cmp
+    u, v, *_ = seq.truth(k, res["x"], res["y"])
+    ok = res["result"][:, 2] >= 0
+    err = np.maximum(np.abs(res["deformation"][ok, 0] - u[ok]), np.abs(res["deformation"][ok, 6] - v[ok]))
+    say("truth p99", float(np.percentile(err, 99)))
+    assert ok.mean() > 0.99 and np.percentile(err, 99) < 0.05
+
+
+def test_mixed_radii_sequence_matches_per_class_drivers(seq, targets):
+    with mixed_driver(seq) as d:
+        mixed = run_sequence(d, targets)
+    r = seq.pois["subset_radius"][:, 0].astype(int)
+    for rad in synth.RADII_MIXED:
+        idx = np.nonzero(r == rad)[0]
+        with fa.FrameDriver(rad) as d:
+            d.set_reference(img(seq.ref))
+            d.set_pois(seq.pois[idx].copy())
+            single = run_sequence(d, targets)
+        for f, ((mres, mst), (sres, sst)) in enumerate(zip(mixed, single)):
+            assert same_bytes(mres[idx], sres), (rad, f)
+            assert mst["skip_fftcc_used"] == sst["skip_fftcc_used"]
+            assert not mst["fallback_used"] and not sst["fallback_used"]
+    say("per-frame", [{x: st[x] for x in ("skip_fftcc_used", "poi_fallback_count", "iteration_mean",
+                                           "valid_ratio")} for _, st in mixed])
+
+
+def test_graph_replay_matches_eager(seq, targets):
+    with mixed_driver(seq) as d:
+        eager = run_sequence(d, targets)
+    with mixed_driver(seq, graph=True) as d:
+        graph = run_sequence(d, targets)
+        # the graphs replay again after a reset, with eager frames between
+        d.reset_sequence()
+        again = run_sequence(d, targets)
+    keys = ("skip_fftcc_used", "fallback_used", "poi_fallback_count", "poi_fallback_recovered", "valid_count",
+            "iteration_sum", "iteration_max", "fftcc_pois", "bad_ratio")
+    for f, ((er, es), (gr, gs), (ar, as_)) in enumerate(zip(eager, graph, again)):
+        assert same_bytes(er, gr) and same_bytes(er, ar), f
+        for key in keys:
+            assert es[key] == gs[key] == as_[key], (f, key)
+        assert np.isnan(gs["icgn_ms"]) and gs["device_ms"] > 0
+    say("graph frames", [round(st["device_ms"], 3) for _, st in graph], "eager",
+        [round(st["device_ms"], 3) for _, st in eager])
+
+
+def test_persistent_queue_matches_block_per_poi(seq, targets):
+    with mixed_driver(seq) as d:
+        base = run_sequence(d, targets)
+    with mixed_driver(seq, persistent=96) as d:
+        pers = run_sequence(d, targets)
+    for f, ((br, bs), (pr, ps)) in enumerate(zip(base, pers)):
+        assert same_bytes(br, pr), f
+        assert bs["iteration_sum"] == ps["iteration_sum"]
+
+
+def test_autotune_keeps_one_configuration_per_class(seq, targets):
+    k = STEPS[-1]
+    with mixed_driver(seq) as d:
+        base, _ = d.process(img(targets[k]), policy=0)
+    with mixed_driver(seq) as d:
+        d.upload_target(img(targets[k]))
+        rep = d.autotune()
+        tuned, st = d.process(img(targets[k]), policy=0)
+    for rad in synth.RADII_MIXED:
+        rows = [e for e in rep if e["radius"] == rad]
+        assert len(rows) >= 4 and sum(e["chosen"] for e in rows) == 1
+        assert {e["sampling"] for e in rows} == {0, 1} and {e["persistent"] > 0 for e in rows} == {False, True}
+        best = min(rows, key=lambda e: e["ms"])
+        assert best["chosen"] == 1
+        say("radius", rad, "chosen", [e for e in rows if e["chosen"]][0], "of", len(rows))
+    cmp = compare(base, tuned)
+    assert cmp["pass"], cmp
