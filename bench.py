@@ -103,6 +103,185 @@ def copy_calibration(mib: int, repeats: int = 3) -> int:
     return 0
 
 
+_C2 = None
+
+
+def _c2_frame(job):
+    """Render frame k of the config-2 sequence (process-pool worker)."""
+    global _C2
+    size, frames, k = job
+    if _C2 is None:
+        from fasticgn_amd import synth
+        _C2 = synth.config2(size=size, frames=frames)
+    return _C2.frame(k)
+
+
+def bench_config2(args) -> int:
+    """500 frames of the mixed-radius u16 sequence through the frame driver:
+    hipGraph per frame, autotuned launches, default policy (seed from the
+    previous frame, bad-ratio guard, POI-level repair).  value = converged
+    POIs per second over the timed frames, each frame's target uploaded from
+    host memory inside the timed region (the sequence streams in)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import fasticgn_amd as fa
+    from fasticgn_amd import synth
+    from fasticgn_amd._native import synchronize as sync
+
+    size, F = 1024, args.frames
+    sq = synth.config2(size=size, frames=F)
+    t0 = time.perf_counter()
+    with ProcessPoolExecutor(min(16, os.cpu_count() or 1)) as ex:
+        frames = list(ex.map(_c2_frame, [(size, F, k) for k in range(F)], chunksize=8))
+    log(f"[bench] config2: {F} frames of {size}^2 u16 rendered in {time.perf_counter() - t0:.1f} s, "
+        f"{len(sq.pois)} POIs, radii {sq.meta['radii']}")
+    img = lambda a: fa.Image2D(a, depth12="right")  # noqa: E731
+    drv = fa.FrameDriver(15, 0.001, 10)
+    drv.set_self_adaptive(True)
+    drv.set_reference(img(sq.ref))
+    drv.set_pois(sq.pois)
+    drv.set_graph(True)
+    drv.upload_target(img(frames[F // 2]))
+    tune = drv.autotune()
+    chosen = [e for e in tune if e["chosen"]]
+    log(f"[bench] autotune: {len(tune)} configurations, chosen {chosen}")
+    drv.reset_sequence()
+    W = max(1, min(args.warmup, F - 1))
+    for k in range(W):
+        drv.process(img(frames[k]))
+    sync()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(W, F):
+        drv.upload_target(img(frames[k]))
+        stats.append(drv.run())
+    sync()
+    wall = time.perf_counter() - t0
+    res = drv.download()
+    K = F - W
+    valid = float(sum(s["valid_count"] for s in stats))
+    dev_ms = [s["device_ms"] for s in stats]
+    u, v, *_ = sq.truth(F - 1, res["x"], res["y"])
+    ok = (res["result"][:, 2] >= 0) & np.isfinite(res["result"][:, 2])
+    r = sq.pois["subset_radius"][:, 0].astype(int)
+    acc = {}
+    for rad in sq.meta["radii"]:
+        m = ok & (r == rad)
+        e = np.maximum(np.abs(res["deformation"][m, 0] - u[m]), np.abs(res["deformation"][m, 6] - v[m]))
+        acc[str(rad)] = {"valid": int(m.sum()), "of": int((r == rad).sum()),
+                         "p99_err_px": round(float(np.percentile(e, 99)), 5), "max_err_px": round(float(e.max()), 5)}
+    out = {
+        "metric": "converged POIs/sec, mixed radii 10/15/20, 500-frame streaming sequence",
+        "value": round(valid / wall, 1), "unit": "POIs/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{size}x{size} u16 12-bit speckle, {F} frames (t = 15 k/499 px at 30 deg, "
+                               f"strain 5e-3 k/499), {len(sq.pois)} POIs on a 5 px grid, radii 10/15/20 "
+                               "(POI index mod 3), seeded from the previous frame, hipGraph per frame, autotuned",
+                   "pois": len(sq.pois), "parallelism": "single-gpu"},
+        "detail": {"device_ms_median": round(float(np.median(dev_ms)), 4),
+                   "device_only_pois_per_s": round(valid / (sum(dev_ms) / 1e3), 1),
+                   "upload_inclusive": True,
+                   "iterations_per_poi": round(float(np.mean([s["iteration_mean"] for s in stats])), 3),
+                   "poi_fallback_total": int(sum(s["poi_fallback_count"] for s in stats)),
+                   "frame_fallbacks": int(sum(s["fallback_used"] for s in stats)),
+                   "autotune_chosen": chosen, "last_frame_accuracy": acc},
+    }
+    print(json.dumps(out), flush=True)
+    drv.close()
+    return 0
+
+
+def bench_config4(args) -> int:
+    """The 4096^2 crack field, 1.03 M POIs at r = 20, one frame: learned seed
+    with FFTCC fallback (and the FFTCC-seeded path beside it), autotuned
+    IC-GN launch.  Reports accuracy against the truth away from the crack,
+    parity against the FP64 oracle on a POI sample, and the fraction of POIs
+    within 2r of the crack sent to FFTCC."""
+    import fasticgn_amd as fa
+    from fasticgn_amd import seeder as sd
+    from fasticgn_amd import synth
+    from fasticgn_amd._native import synchronize as sync
+
+    t0 = time.perf_counter()
+    sc = synth.config4()
+    log(f"[bench] config4: {sc.ref.shape} u16, {len(sc.pois)} POIs ({time.perf_counter() - t0:.1f} s)")
+    drv = fa.FrameDriver(sc.radius, 0.001, 10)
+    drv.set_reference(fa.Image2D(sc.ref))
+    drv.set_pois(sc.pois)
+    drv.upload_target(fa.Image2D(sc.tar))
+    drv.run(policy=0)
+    tune = drv.autotune()
+    chosen = [e for e in tune if e["chosen"]]
+    log(f"[bench] autotune chosen {chosen}")
+    seeder = sd.LearnedSeeder()
+    drv.set_seeder(seeder, args.seed_zncc)
+    runs = {}
+    steps = max(1, min(args.steps, 10))
+    for name, pol in (("fftcc", 0), ("learned", fa.driver.LEARNED_SEED)):
+        drv.run(policy=pol)
+        sync()
+        st = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.append(drv.run(policy=pol))
+        sync()
+        wall = time.perf_counter() - t0
+        runs[name] = {"ms": wall * 1e3 / steps, "stats": st[-1], "res": drv.download(), "redo": drv.download_redo()}
+        log(f"[bench] {name}: {runs[name]['ms']:.3f} ms/frame, valid {st[-1]['valid_count']}")
+    r = runs["learned"]
+    res = r["res"]
+    dist = sc.crack_distance(res["x"], res["y"])
+    near = dist <= 2 * sc.radius
+    ok = (res["result"][:, 2] >= 0) & np.isfinite(res["result"][:, 2])
+    u, *_ = sc.truth(res["x"], res["y"])
+    far = ok & ~near
+    err = np.abs(res["deformation"][far, 0] - u[far])
+    errv = np.abs(res["deformation"][far, 6])
+    # FP64 oracle on a sample away from the crack, FFTCC-seeded
+    from oracle import pyoracle
+    from tests.parity import compare
+    rng = np.random.default_rng(4)
+    pick = np.sort(rng.choice(np.nonzero(~near)[0], 2000, replace=False))
+    ref32 = pyoracle.as_f32(sc.ref)
+    tar32 = pyoracle.as_f32(sc.tar)
+    o = sc.pois[pick].copy()
+    pyoracle.fftcc(ref32, tar32, sc.radius, o)
+    pyoracle.icgn(ref32, tar32, sc.radius, o, f64=True)
+    par = compare(o, res[pick])
+    keys = ("n", "status_mismatch", "median_du", "p99_du", "max_du", "median_dv", "p99_dv", "max_dv", "pass")
+    st = r["stats"]
+    value = st["valid_count"] / (r["ms"] / 1e3)
+    out = {
+        "metric": "converged POIs/sec (41x41 subset), learned seed with FFTCC fallback",
+        "value": round(value, 1), "unit": "POIs/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": round(r["ms"], 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"4096x4096 u16 speckle (noise 2), crack at 30 deg through the centre with a 1.5 px "
+                               f"jump of u, u = 1e-3 x; grid 4, r = 20, {len(sc.pois)} POIs; learned seed + FFTCC "
+                               "fallback + ICGN2D1 conv 1e-3 max 10 it, autotuned",
+                   "pois": len(sc.pois), "parallelism": "single-gpu"},
+        "detail": {"fftcc_path_pois_per_s": round(runs["fftcc"]["stats"]["valid_count"] / (runs["fftcc"]["ms"] / 1e3), 1),
+                   "learned_vs_fftcc": round(value / (runs["fftcc"]["stats"]["valid_count"] / (runs["fftcc"]["ms"] / 1e3)), 4),
+                   "seed_fallback_fraction": round(st["seed_fallback_count"] / len(sc.pois), 5),
+                   "near_crack_pois": int(near.sum()),
+                   "near_crack_fallback_fraction": round(float(r["redo"][near].mean()), 5),
+                   "far_fallback_fraction": round(float(r["redo"][~near].mean()), 5),
+                   "valid_fraction_near_crack": round(float(ok[near].mean()), 5),
+                   "valid_fraction_far": round(float(ok[~near].mean()), 5),
+                   "truth_far_p99_du_px": round(float(np.percentile(err, 99)), 5),
+                   "truth_far_p99_dv_px": round(float(np.percentile(errv, 99)), 5),
+                   "iterations_per_poi": round(st["iteration_sum"] / len(sc.pois), 3),
+                   "seed_ms": round(st["seed_ms"], 4), "icgn_ms": round(st["icgn_ms"], 4),
+                   "device_ms": round(st["device_ms"], 4),
+                   "parity_vs_fp64_oracle_far_sample": {k: par.get(k) for k in keys},
+                   "autotune_chosen": chosen},
+    }
+    print(json.dumps(out), flush=True)
+    drv.close()
+    seeder.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +297,10 @@ def main() -> int:
                     help="learned seeder weights (.npy), 'auto' = the shipped weights, 'off' = FFTCC path only")
     ap.add_argument("--seed-zncc", type=float, default=0.8,
                     help="ZNCC below which a learned-seed POI is redone from FFTCC")
+    ap.add_argument("--workload", default="config1", choices=["config1", "config2", "config4"],
+                    help="config1: the 2048^2 single pair (default); config2: the 500-frame mixed-radius "
+                         "u16 sequence; config4: the 4096^2 crack field (1.03 M POIs, r = 20)")
+    ap.add_argument("--frames", type=int, default=500, help="config2: frames of the sequence")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -127,6 +310,10 @@ def main() -> int:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
               "(the path shards over POIs; see DESIGN.md)", file=sys.stderr)
         return 2
+    if args.workload == "config2":
+        return bench_config2(args)
+    if args.workload == "config4":
+        return bench_config4(args)
 
     import fasticgn_amd as fa
     from fasticgn_amd import synth

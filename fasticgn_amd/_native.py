@@ -153,6 +153,8 @@ SIGNATURES = {
     "fi_frame_set_self_adaptive": [_P, _I],
     "fi_frame_set_graph": [_P, _I],
     "fi_frame_set_persistent": [_P, _I],
+    "fi_frame_download_redo": [_P, _P, _S],
+    "fi_frame_set_fused": [_P, _I],
     "fi_frame_autotune": [_P, _P, _S, ctypes.POINTER(_S)],
     "fi_seeder_create": [_I, ctypes.POINTER(_P)],
     "fi_seeder_destroy": [_P],

@@ -277,6 +277,11 @@ int fi_frame_download(fi_frame* h, fi_poi* out, size_t n)
     return guarded([&] { need(h); h->download(out, n); });
 }
 
+int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n)
+{
+    return guarded([&] { need(h); h->download_redo(out, n); });
+}
+
 int fi_frame_reset_sequence(fi_frame* h)
 {
     return guarded([&] { need(h); h->reset_sequence(); });
@@ -326,6 +331,18 @@ int fi_frame_set_persistent(fi_frame* h, int blocks)
         if (blocks < 0) fail(FI_E_INVALID_ARGUMENT, "persistent block count must be >= 0");
         h->persistent_blocks = blocks;
         for (auto& c : h->classes) c->queue_blocks = -1;
+        h->invalidate_graphs();
+    });
+}
+
+int fi_frame_set_fused(fi_frame* h, int on)
+{
+    return guarded([&] {
+        need(h);
+        h->fuse = on != 0;
+        h->fused.waves = h->fused.spl = 0;
+        h->fused.sampling = -1;
+        h->fused.queue_blocks = -1;
         h->invalidate_graphs();
     });
 }

@@ -553,6 +553,9 @@ void fi_frame::prepare()
         std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rad[a] < rad[b]; });
     }
     classes.clear();
+    fused.waves = fused.spl = 0;  // tuned for the previous classes
+    fused.sampling = -1;
+    fused.queue_blocks = -1;
     for (size_t i = 0; i < n; i++) {
         const int ri = order.empty() ? rad[i] : rad[order[i]];
         if (classes.empty() || classes.back()->r != ri) {
@@ -584,6 +587,7 @@ void fi_frame::prepare()
     save.reserve(cap);
     repair.reserve(cap);
     cand.reserve(cap);
+    redo.reserve(cap);
     prep.reserve(cap);
     counters.reserve(nclass() + 1);
     h_counters.reserve(nclass() + 1);
@@ -687,8 +691,61 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     launch_sum_iters(ip.iters, count, iter_total.p, stream);
 }
 
+// The fewest-waves generic shape whose bands cover every class radius.
+bool fi_frame::fused_shape(int& waves, int& spl) const
+{
+    for (int W : { 1, 2, 4 }) {
+        bool all = true;
+        for (auto& c : classes) all = all && icgn_shape_covers(W, 16, c->r, c->r);
+        if (all) {
+            waves = W;
+            spl = 16;
+            return true;
+        }
+    }
+    return false;
+}
+
+// One launch over all classes: the kernel reads each POI's subset_radius
+// (self-adaptive); the LDS tile and shape are those of the largest radius.
+void fi_frame::icgn_fused(fi_poi* base)
+{
+    const int mode = fused.sampling >= 0 ? fused.sampling : sampling;
+    IcgnParams ip{};
+    ip.ref = ref_f.p;
+    ip.gx = gx.p;
+    ip.gy = gy.p;
+    ip.coef = coef.p;
+    ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+    ip.w = ref_raw.view.width;
+    ip.h = ref_raw.view.height;
+    ip.pois = base;
+    ip.n = (int)n;
+    ip.prep = prep.p;
+    ip.rx = ip.ry = classes.back()->r;
+    ip.self_adaptive = 1;
+    ip.conv = conv;
+    ip.stop = stop;
+    ip.iters = iters.p;
+    int waves = fused.waves, spl = fused.spl;
+    if (waves == 0 && !fused_shape(waves, spl))
+        fail(FI_E_UNSUPPORTED, "no IC-GN lane shape covers every radius class");
+    const int qb = fused.queue_blocks >= 0 ? fused.queue_blocks : persistent_blocks;
+    if (qb > 0) {
+        ip.queue = queue.p;
+        ip.queue_blocks = qb;
+        launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), 8, stream);
+    }
+    icgn_launch(ip, waves, spl, stream);
+    launch_sum_iters(ip.iters, (int)n, iter_total.p, stream);
+}
+
 void fi_frame::icgn_all(fi_poi* base)
 {
+    if (fuse && nclass() > 1) {
+        icgn_fused(base);
+        return;
+    }
     for (auto& c : classes) icgn_pass(*c, base + c->off, (int)c->n, nullptr);
 }
 
@@ -705,6 +762,7 @@ void fi_frame::repair_class(const FrameClass& c, int m)
     int ci = 0;
     while (classes[ci].get() != &c) ci++;
     launch_gather(tmpl.p + c.off, cand.p + c.off, m, repair.p + c.off, stream);
+    launch_mark(cand.p + c.off, m, redo.p + c.off, stream);
     fftcc_pass(c, repair.p + c.off, m, cand.p + c.off);
     icgn_pass(c, repair.p + c.off, m, cand.p + c.off);
     launch_merge_repair(cur.p + c.off, repair.p + c.off, cand.p + c.off, m, counters.p + ci, stream);
@@ -761,6 +819,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     auto segment = [&](bool timed) {
         launch_zero_words(reinterpret_cast<uint32_t*>(iter_total.p), 2, stream);
         zero_counters();
+        launch_fill_bytes(redo.p, n, mode == 1 ? 1 : 0, stream);
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
         launch_bspline(tar.view, coef.p, stream);
         launch_to_float(tar.view, tar_f.p, stream);
@@ -862,6 +921,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             fftcc_all(cur.p);
             icgn_all(cur.p);
             launch_merge_frame_fallback(save.p, cur.p, cnt, stream);
+            launch_fill_bytes(redo.p, n, 1, stream);
             FI_HIP(hipGetLastError());
             st.fftcc_pois = cnt;
             redo_qa = true;
@@ -1009,11 +1069,108 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
         c.queue_blocks = bq;
         if (report && best_entry < report_len) report[best_entry].chosen = 1;
     }
+    // one launch over every class against the per-class launches
+    fuse = false;
+    if (nclass() > 1) {
+        double per_class = 0.0;
+        {
+            float t[3];
+            for (int rep = 0; rep < 3; rep++) {
+                FI_HIP(hipMemcpyAsync(repair.p, save.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+                FI_HIP(hipEventRecord(ev[5], stream));
+                for (auto& c : classes) icgn_pass(*c, repair.p + c->off, (int)c->n, nullptr);
+                FI_HIP(hipEventRecord(ev[6], stream));
+                FI_HIP(hipEventSynchronize(ev[6]));
+                FI_HIP(hipEventElapsedTime(&t[rep], ev[5], ev[6]));
+            }
+            std::sort(t, t + 3);
+            per_class = t[1];
+        }
+        double best = per_class;
+        size_t best_entry = SIZE_MAX;
+        FrameClass keep;
+        for (int W : { 1, 2, 4 }) {
+            bool all = true;
+            for (auto& c : classes) all = all && icgn_shape_covers(W, 16, c->r, c->r);
+            if (!all) continue;
+            for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE }) {
+                for (int qb : { 0, cus * 12 / W }) {
+                    fused.waves = W;
+                    fused.spl = 16;
+                    fused.sampling = samp;
+                    fused.queue_blocks = qb;
+                    float t[3];
+                    for (int rep = 0; rep < 3; rep++) {
+                        FI_HIP(hipMemcpyAsync(repair.p, save.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice,
+                                              stream));
+                        FI_HIP(hipEventRecord(ev[5], stream));
+                        icgn_fused(repair.p);
+                        FI_HIP(hipEventRecord(ev[6], stream));
+                        FI_HIP(hipEventSynchronize(ev[6]));
+                        FI_HIP(hipEventElapsedTime(&t[rep], ev[5], ev[6]));
+                    }
+                    std::sort(t, t + 3);
+                    timed++;
+                    if (report && m < report_len) {
+                        fi_tune_entry& e = report[m];
+                        e.radius = 0;  // every class in one launch
+                        e.waves = W;
+                        e.rows_per_band = 16;
+                        e.sampling = samp;
+                        e.persistent = qb;
+                        e.chosen = 0;
+                        e.ms = t[1];
+                    }
+                    if (t[1] < best) {
+                        best = t[1];
+                        best_entry = m;
+                        keep.waves = W;
+                        keep.spl = 16;
+                        keep.sampling = samp;
+                        keep.queue_blocks = qb;
+                    }
+                    m++;
+                }
+            }
+        }
+        if (best_entry != SIZE_MAX) {
+            fuse = true;
+            fused.waves = keep.waves;
+            fused.spl = keep.spl;
+            fused.sampling = keep.sampling;
+            fused.queue_blocks = keep.queue_blocks;
+            // the per-class choices are no longer used
+            if (report)
+                for (size_t i = 0; i < std::min(m, report_len); i++)
+                    if (report[i].radius != 0) report[i].chosen = 0;
+            if (report && best_entry < report_len) report[best_entry].chosen = 1;
+        } else {
+            fused.waves = fused.spl = 0;
+            fused.sampling = -1;
+            fused.queue_blocks = -1;
+        }
+    }
     FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
     FI_HIP(hipStreamSynchronize(stream));
     invalidate_graphs();
     if (report_n) *report_n = std::min(m, report_len);
     return timed;
+}
+
+void fi_frame::download_redo(uint8_t* out, size_t count)
+{
+    if (!have_prev) fail(FI_E_LOGIC, "no frame has been run");
+    if (count != n) fail(FI_E_INVALID_ARGUMENT, "download length must equal the template length");
+    if (count == 0) return;
+    if (out == nullptr) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");
+    DeviceGuard g(device);
+    std::vector<uint8_t> h(n);
+    FI_HIP(hipMemcpyAsync(h.data(), redo.p, n, hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipStreamSynchronize(stream));
+    if (order.empty())
+        std::copy(h.begin(), h.end(), out);
+    else
+        for (size_t i = 0; i < n; i++) out[order[i]] = h[i];
 }
 
 void fi_frame::download(fi_poi* out, size_t count)

@@ -101,6 +101,10 @@ struct fi_frame {
     int sampling = FI_SAMPLING_AUTO;
     bool use_graph = false;        // capture each frame's device work as a hipGraph
     int persistent_blocks = 0;     // IC-GN persistent work-queue blocks (0 = one block per POI)
+    // One IC-GN launch over every radius class (per-POI radius, a shape that
+    // covers all of them) instead of one launch per class
+    bool fuse = false;
+    FrameClass fused;              // its launch configuration (r = largest radius)
     fi_seeder* seeder = nullptr;   // learned seeder (FI_POLICY_LEARNED_SEED)
     float seed_threshold = 0.8f;   // ZNCC below which a learned seed falls back to FFTCC
     hipStream_t stream = nullptr;
@@ -120,6 +124,7 @@ struct fi_frame {
     fi::PinnedArray<unsigned long long> h_iter_total;
     fi::DevArray<int> queue;       // persistent-queue claim counters, 8 per class
     fi::PinnedArray<fi_poi> h_out;
+    fi::DevArray<uint8_t> redo;    // per POI: re-run from FFTCC in the last frame
     size_t n = 0;
     bool ref_set = false, pois_set = false, prepared = false, tar_ready = false;
     bool have_prev = false, baseline_ready = false;
@@ -140,6 +145,7 @@ struct fi_frame {
     void upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12);
     void run(unsigned policy, fi_frame_stats* stats);
     void download(fi_poi* out, size_t n);
+    void download_redo(uint8_t* out, size_t n);
     void reset_sequence();
     void invalidate_graphs();
     // Times the IC-GN lane shapes, sampling modes and launch modes on the
@@ -151,6 +157,8 @@ private:
     void prepare();
     void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev);
     void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index);
+    void icgn_fused(fi_poi* base);
+    bool fused_shape(int& waves, int& spl) const;
     void icgn_all(fi_poi* base);
     void fftcc_all(fi_poi* base);
     void repair_class(const FrameClass& c, int m);

@@ -91,6 +91,7 @@ struct PinnedArray {
 // range (src/oc_image.cpp:172-178).
 struct DeviceImage {
     DevArray<uint8_t> raw;
+    PinnedArray<uint8_t> stage;  // host staging: one contiguous DMA per upload
     DevImage view;
     bool set = false;
     void upload(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12,
@@ -126,5 +127,8 @@ void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float thresh
 void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s);
 // p[0..n) = 0 (32-bit words), by a kernel
 void launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
+// flags[idx[j]] = 1, j < n; p[0..n) = v (bytes)
+void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s);
+void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 
 }  // namespace fi

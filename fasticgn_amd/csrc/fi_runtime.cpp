@@ -1,4 +1,5 @@
 // Host runtime: last-error storage, image upload, DFT twiddles.
+#include <cstring>
 #include <cmath>
 #include <string>
 
@@ -39,22 +40,27 @@ void DeviceImage::upload(const void* px, int width, int height, size_t stride_by
                          int depth12, hipStream_t s)
 {
     check_image_args(px, width, height, stride_bytes, pixel, depth12);
-    if (depth12 == FI_DEPTH12_RIGHT) {
-        // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
-        for (int y = 0; y < height; y++) {
-            const uint16_t* row = reinterpret_cast<const uint16_t*>(
-                static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes);
-            for (int x = 0; x < width; x++)
-                if (row[x] > 4095)
-                    fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
-        }
-    }
     const size_t bpp = bytes_per_pixel(pixel);
     const size_t row_bytes = (size_t)width * bpp;
-    raw.reserve(row_bytes * height);
-    FI_HIP(hipMemcpy2DAsync(raw.p, row_bytes, px, stride_bytes, row_bytes, height,
-                            hipMemcpyHostToDevice, s));
-    FI_HIP(hipStreamSynchronize(s));
+    const size_t total = row_bytes * height;
+    // rows packed into pinned memory (the copy out of pageable memory is a
+    // plain memcpy; the DMA then runs at the link rate)
+    stage.reserve(total);
+    uint16_t mx = 0;
+    for (int y = 0; y < height; y++) {
+        const uint8_t* src = static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes;
+        std::memcpy(stage.p + (size_t)y * row_bytes, src, row_bytes);
+        if (depth12 == FI_DEPTH12_RIGHT) {
+            const uint16_t* row = reinterpret_cast<const uint16_t*>(stage.p + (size_t)y * row_bytes);
+            for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
+        }
+    }
+    // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
+    if (depth12 == FI_DEPTH12_RIGHT && mx > 4095)
+        fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
+    raw.reserve(total);
+    FI_HIP(hipMemcpyAsync(raw.p, stage.p, total, hipMemcpyHostToDevice, s));
+    FI_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next upload
     view.data = raw.p;
     view.width = width;
     view.height = height;

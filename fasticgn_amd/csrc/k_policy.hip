@@ -231,6 +231,31 @@ __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n)
         p[i] = 0u;
 }
 
+// flags[idx[j]] = 1 for j < n
+__global__ void k_mark(const uint32_t* __restrict__ idx, int n, uint8_t* __restrict__ flags)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) flags[idx[j]] = 1u;
+}
+
+__global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_mark, grid_for(n), dim3(256), 0, s, idx, n, flags);
+}
+
+void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_fill_bytes, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
+                           p, n, v);
+}
+
 void launch_zero_words(uint32_t* p, size_t n, hipStream_t s)
 {
     if (n > 0)

@@ -54,6 +54,12 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))
         return out
 
+    def download_redo(self) -> np.ndarray:
+        """Per POI of the last frame: True where it went through FFTCC."""
+        out = np.zeros(self.n, dtype=np.uint8)
+        nat.call("fi_frame_download_redo", self._h, nat.ptr(out), len(out))
+        return out.astype(bool)
+
     def set_shape(self, waves: int, rows_per_band: int) -> None:
         """IC-GN lane-to-POI shape: waves per POI and rows per band (0, 0 =
         default); it must fit the driver's subset."""
@@ -89,6 +95,11 @@ class FrameDriver(_Handle):
         """IC-GN persistent work-queue launch with `blocks` resident blocks
         (0 = one block per POI) for every radius class."""
         nat.call("fi_frame_set_persistent", self._h, int(blocks))
+
+    def set_fused(self, on: bool = True) -> None:
+        """One IC-GN launch over every radius class (per-POI radius) instead
+        of one launch per class."""
+        nat.call("fi_frame_set_fused", self._h, int(bool(on)))
 
     def autotune(self) -> list:
         """Time every IC-GN lane shape, sampling mode and launch mode per

@@ -271,6 +271,50 @@ def config2(size: int = 1024, frames: int = 500, grid: int = 5, seed: int = 2026
     return Sequence(f"config2_{size}", ref, pois, frames, frame, truth, meta)
 
 
+def config4(size: int = 4096, grid: int = 4, radius: int = 20, seed: int = 20264, jump: float = 1.5,
+            angle_deg: float = 30.0, slope: float = 1e-3) -> Scene:
+    """Large field with a discontinuity: size^2 u16 speckle (16-bit range,
+    background 10280, peak 46260, noise sigma 2 grey levels), a straight crack
+    through the image centre at angle_deg, u = slope * X + jump on the crack's
+    positive side (v = 0).  POIs on a `grid` px grid, subset radius `radius`,
+    margin radius + 2; guess 0.  A target pixel in the opening (no preimage)
+    takes the mean of its two one-sided preimages."""
+    rng = np.random.default_rng(seed)
+    n_dots = int(round(6000 * size * size / (512 * 512)))
+    cx, cy, s = dot_field(size, size, n_dots, rng)
+    ref = quantize(render_dots(cx, cy, s, size, size, 10280.0, 35980.0), rng, 2.0, np.uint16, 65535.0)
+    c = ((size - 1) / 2.0, (size - 1) / 2.0)
+    nx, ny = -math.sin(math.radians(angle_deg)), math.cos(math.radians(angle_deg))
+
+    def side(X, Y):
+        return (X - c[0]) * nx + (Y - c[1]) * ny
+
+    def inverse_map(x, y):
+        xa = x / (1.0 + slope)             # preimage on the negative side
+        xb = (x - jump) / (1.0 + slope)    # on the positive side
+        on_b = side(xb, y) > 0
+        on_a = side(xa, y) <= 0
+        X = np.where(on_b & ~on_a, xb, np.where(on_a & ~on_b, xa, 0.5 * (xa + xb)))
+        return X, y
+
+    tar = warp_target(ref, inverse_map, np.uint16, 65535.0)
+
+    def truth(X, Y):
+        X = np.asarray(X, dtype=np.float64)
+        Y = np.asarray(Y, dtype=np.float64)
+        u = slope * X + jump * (side(X, Y) > 0)
+        z = np.zeros_like(X)
+        return u, z, slope + z, z, z, z
+
+    pois = grid_queue(size, size, radius, grid)
+    meta = {"seed": seed, "n_dots": n_dots, "grid": grid, "margin": radius + 2, "bit_depth": 16,
+            "noise_sigma": 2.0, "crack": {"angle_deg": angle_deg, "through": "centre", "jump_u_px": jump},
+            "smooth_field": f"u = {slope} * x", "resample": "keys-cubic a=-0.5"}
+    sc = Scene(f"config4_{size}", ref, tar, pois, radius, truth, meta)
+    sc.crack_distance = lambda X, Y: np.abs(side(np.asarray(X, np.float64), np.asarray(Y, np.float64)))
+    return sc
+
+
 def lcg_pair(width: int = 192, height: int = 128, seed: int = 12345):
     """The reference's FFTCC known-answer images
     (smoke/library_boundary_probe.cpp:41-47): LCG pixels, target = reference

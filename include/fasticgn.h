@@ -187,6 +187,10 @@ int fi_frame_upload_target(fi_frame* h, const void* px, int width, int height,
  * device (fi_frame_download copies them out).  stats may be NULL. */
 int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats);
 int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
+/* Per POI of the last frame (template order): 1 if it was run through FFTCC
+ * (an FFTCC-seeded frame, the whole-frame fallback, POI-level repair or the
+ * learned seed's fallback), else 0. */
+int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n);
 /* Forget the previous frames (next run seeds by FFTCC again). */
 int fi_frame_reset_sequence(fi_frame* h);
 /* IC-GN lane-to-POI shape for this driver (see fi_icgn_set_shape). */
@@ -209,9 +213,16 @@ int fi_frame_set_graph(fi_frame* h, int on);
  * per-XCD queues; 0 (default) launches one block per POI.  Replaces tuned
  * launch modes. */
 int fi_frame_set_persistent(fi_frame* h, int blocks);
+/* With several radius classes: on = one IC-GN launch over all of them (each
+ * POI at its own radius, the lane shape and LDS tile of the largest radius),
+ * off (default) = one launch per class.  The launch mode follows
+ * fi_frame_set_persistent.  The autotuner chooses between the two. */
+int fi_frame_set_fused(fi_frame* h, int on);
 /* Autotuner: on the uploaded target (FFTCC-seeded), time every IC-GN lane
  * shape, sampling mode and launch mode (one block per POI, or the persistent
- * work queue) for each radius class and keep the fastest.  report (may be
+ * work queue) for each radius class and keep the fastest; with several
+ * classes, also one launch over all of them (report rows with radius 0),
+ * kept when faster than the best per-class launches together.  report (may be
  * NULL) receives up to report_len timed configurations; *report_n (may be
  * NULL) the number timed.  Does not advance the frame sequence. */
 typedef struct fi_tune_entry {
