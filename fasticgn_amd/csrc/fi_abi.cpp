@@ -1,8 +1,10 @@
 // extern "C" boundary (include/fasticgn.h): every entry point catches
 // everything, records the message for fi_last_error and returns a status.
+#include <cstdio>
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "fi_engines.h"
 #include "fi_seeder_engine.h"
@@ -460,6 +462,26 @@ int fi_format_raw(const fi_poi* pois, size_t n, void* out, size_t out_len, size_
             c += 36;
         }
         if (written) *written = n * 36;
+    });
+}
+
+int fi_write_raw(const fi_poi* pois, size_t n, const char* path, size_t* bytes)
+{
+    return guarded([&] {
+        if (!path) fail(FI_E_INVALID_ARGUMENT, "null path");
+        if (n > 0 && !pois) fail(FI_E_INVALID_ARGUMENT, "null POI buffer");
+        std::vector<unsigned char> buf(n * 36);
+        size_t w = 0;
+        if (n > 0) {
+            const int rc = fi_format_raw(pois, n, buf.data(), buf.size(), &w);
+            if (rc != FI_OK) fail(rc, last_error());
+        }
+        FILE* f = std::fopen(path, "wb");
+        if (!f) fail(FI_E_INVALID_ARGUMENT, std::string("cannot open ") + path + " for writing");
+        const size_t put = w ? std::fwrite(buf.data(), 1, w, f) : 0;
+        const bool ok = put == w && std::fflush(f) == 0;
+        if (std::fclose(f) != 0 || !ok) fail(FI_E_DEVICE, std::string("short write to ") + path);
+        if (bytes) *bytes = w;
     });
 }
 

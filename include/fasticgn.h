@@ -275,6 +275,12 @@ int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold);
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */
 int fi_format_raw(const fi_poi* pois, size_t n, void* out, size_t out_len, size_t* written);
+/* One raw frame file (frame_NNNNNN.bin of the opencorr-gpu-raw-v1 tree,
+ * cli:3700-3720): the n records of fi_format_raw written to path (created or
+ * truncated); *bytes (may be NULL) = bytes written.  The manifest
+ * (writeRawManifest, cli:2615-2732) is written by the host layer
+ * (fasticgn_amd/rawio.py). */
+int fi_write_raw(const fi_poi* pois, size_t n, const char* path, size_t* bytes);
 
 #ifdef __cplusplus
 }
