@@ -83,6 +83,7 @@ constexpr int kInferWaves = 4;
 constexpr int kInferThreads = 64 * kInferWaves;
 constexpr int kInferGroup = 16;  // POIs per wave pass
 constexpr int kInferBlocksPerCu = 2;
+constexpr int kXcds = 8;
 // packed fragments: 64 lanes x 8 BF16 (1 KiB) each
 constexpr int kFragOff2 = 1, kFragOff3 = kFragOff2 + 4, kFragOff4 = kFragOff3 + 16, kFragTotal = kFragOff4 + 64;
 constexpr int kLdsFrags = kFragTotal - kFragOff3;  // W3 (16) + W4 (64)
@@ -126,7 +127,19 @@ __global__ __launch_bounds__(kInferThreads, 2) void k_seed_infer(SeedInferParams
         return;
     }
 
-    for (int grp = blockIdx.x * kInferWaves + wave; grp < groups; grp += gridDim.x * kInferWaves) {
+    // XCD-aware: workgroups are dealt to the 8 XCDs round-robin, so with a grid
+    // that is a multiple of 8 XCD x = blockIdx % 8 takes the contiguous
+    // eighth x of the groups, and neighbouring POIs (overlapping crops) share
+    // that XCD's L2
+    int g0 = 0, g1 = groups, first = blockIdx.x * kInferWaves + wave, stride = gridDim.x * kInferWaves;
+    if (gridDim.x % kXcds == 0) {
+        const int xcd = blockIdx.x % kXcds, per = gridDim.x / kXcds;
+        g0 = (int)((long long)groups * xcd / kXcds);
+        g1 = (int)((long long)groups * (xcd + 1) / kXcds);
+        first = g0 + (int)(blockIdx.x / kXcds) * kInferWaves + wave;
+        stride = per * kInferWaves;
+    }
+    for (int grp = first; grp < g1; grp += stride) {
         const int b = grp * kInferGroup + s;
         const int qi = b < p.n ? (p.index ? (int)p.index[b] : b) : 0;
         int ok = 0, x0 = 0, y0 = 0;
@@ -313,7 +326,8 @@ void launch_seed_infer(const SeedInferParams& p, hipStream_t s)
     if (p.n <= 0) return;
     const int cus = std::max(1, p.cus);
     const int groups = (p.n + kInferGroup - 1) / kInferGroup;
-    const int blocks = std::max(1, std::min((groups + kInferWaves - 1) / kInferWaves, kInferBlocksPerCu * cus));
+    int blocks = std::max(1, std::min((groups + kInferWaves - 1) / kInferWaves, kInferBlocksPerCu * cus));
+    if (blocks >= 8 * kXcds) blocks -= blocks % kXcds;  // XCD-aware split (see the kernel)
     hipLaunchKernelGGL(k_seed_infer, dim3(blocks), dim3(kInferThreads), 0, s, p);
 }
 
