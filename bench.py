@@ -103,6 +103,17 @@ def copy_calibration(mib: int, repeats: int = 3) -> int:
     return 0
 
 
+def tune_summary(rows: list) -> dict:
+    """Best autotuner time per (radius class, launch kind); radius 0 = one
+    launch over every class."""
+    out = {}
+    for e in rows:
+        key = f"r{e['radius'] or 'all'}_{'queue' if e['persistent'] else 'blocks'}"
+        if key not in out or e["ms"] < out[key]:
+            out[key] = round(e["ms"], 4)
+    return out
+
+
 _C2 = None
 
 
@@ -184,7 +195,8 @@ def bench_config2(args) -> int:
                    "iterations_per_poi": round(float(np.mean([s["iteration_mean"] for s in stats])), 3),
                    "poi_fallback_total": int(sum(s["poi_fallback_count"] for s in stats)),
                    "frame_fallbacks": int(sum(s["fallback_used"] for s in stats)),
-                   "autotune_chosen": chosen, "last_frame_accuracy": acc},
+                   "autotune_chosen": chosen, "autotune_best_ms": tune_summary(tune),
+                   "last_frame_accuracy": acc},
     }
     print(json.dumps(out), flush=True)
     drv.close()
@@ -274,7 +286,7 @@ def bench_config4(args) -> int:
                    "seed_ms": round(st["seed_ms"], 4), "icgn_ms": round(st["icgn_ms"], 4),
                    "device_ms": round(st["device_ms"], 4),
                    "parity_vs_fp64_oracle_far_sample": {k: par.get(k) for k in keys},
-                   "autotune_chosen": chosen},
+                   "autotune_chosen": chosen, "autotune_best_ms": tune_summary(tune)},
     }
     print(json.dumps(out), flush=True)
     drv.close()

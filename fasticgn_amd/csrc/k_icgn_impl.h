@@ -653,7 +653,11 @@ void k_icgn_queue(IcgnParams p)
         const int lo = (int)((long long)p.n * part / kQueueParts);
         const int hi = (int)((long long)p.n * (part + 1) / kQueueParts);
         for (;;) {
-            if (threadIdx.x == 0) claim = lo + atomicAdd(p.queue + part, kQueueChunk);
+            if (threadIdx.x == 0) {
+                // a drained range is seen with a load, not another atomic
+                const int seen = __hip_atomic_load(p.queue + part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                claim = lo + seen >= hi ? hi : lo + atomicAdd(p.queue + part, kQueueChunk);
+            }
             __syncthreads();
             const int b0 = claim;
             __syncthreads();  // every thread has read claim before it is rewritten

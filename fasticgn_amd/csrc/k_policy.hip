@@ -186,18 +186,19 @@ __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restric
 __global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict__ cur, int n,
                                                         FrameCounters* counters)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // grid-stride: a few hundred blocks, so the five counters see one atomic
+    // per block (same-address atomics serialise)
     unsigned v[4] = { 0u, 0u, 0u, 0u }, mx[1] = { 0u };
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const fi_poi& p = cur[i];
         const float z = p.result[2];
-        v[0] = (finite_f(z) && z >= 0.f) ? 1u : 0u;
-        v[1] = is_bad(p) ? 1u : 0u;
+        v[0] += (finite_f(z) && z >= 0.f) ? 1u : 0u;
+        v[1] += is_bad(p) ? 1u : 0u;
         const float it = p.result[3];
         if (finite_f(it)) {
-            v[2] = 1u;
-            v[3] = (unsigned)(long long)it;
-            mx[0] = (unsigned)(int)(it + 0.5f);
+            v[2] += 1u;
+            v[3] += (unsigned)(long long)it;
+            mx[0] = max(mx[0], (unsigned)(int)(it + 0.5f));
         }
     }
     block_reduce<4>(v, mx);
@@ -222,6 +223,7 @@ __global__ __launch_bounds__(kPolicyThreads) void k_sum_iters(const uint32_t* __
 }
 
 static dim3 grid_for(int n) { return dim3((n + kPolicyThreads - 1) / kPolicyThreads); }
+constexpr int kReduceBlocks = 256;  // grid of the counting kernels
 
 // Zero n 32-bit words (a kernel rather than hipMemsetAsync, so that frame
 // graphs capture no memset node).
@@ -305,13 +307,15 @@ void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* id
 
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_qa, grid_for(n), dim3(kPolicyThreads), 0, s, cur, n, counters);
+    if (n > 0)
+        hipLaunchKernelGGL(k_qa, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks)),
+                           dim3(kPolicyThreads), 0, s, cur, n, counters);
 }
 
 void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s)
 {
     if (n > 0)
-        hipLaunchKernelGGL(k_sum_iters, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, 1024)),
+        hipLaunchKernelGGL(k_sum_iters, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks)),
                            dim3(kPolicyThreads), 0, s, iters, n, total);
 }
 
