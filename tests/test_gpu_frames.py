@@ -179,3 +179,36 @@ def test_autotune_keeps_one_configuration(seq, targets):
     assert sum(e["chosen"] for e in rep) == (1 if fused_chosen else len(synth.RADII_MIXED))
     cmp = compare(base, tuned)
     assert cmp["pass"], cmp
+
+
+def test_frame_option_errors(seq, targets):
+    """Error behaviour of the multi-frame options (status codes through the ABI)."""
+    q = seq.pois[:200].copy()
+    q["subset_radius"][5, 1] = 12.0  # not square
+    with fa.FrameDriver(15) as d:
+        d.set_self_adaptive(True)
+        d.set_reference(img(seq.ref))
+        d.set_pois(q)
+        d.upload_target(img(targets[0]))
+        with pytest.raises(NotImplementedError):
+            d.run(0)
+        q["subset_radius"][5] = 40.0  # beyond the device limit
+        d.set_pois(q)
+        with pytest.raises(NotImplementedError):
+            d.run(0)
+    with fa.FrameDriver(15) as d:
+        d.set_reference(img(seq.ref))
+        d.set_pois(seq.pois[:200].copy())
+        with pytest.raises(fa.LogicError):
+            d.autotune()            # no target uploaded
+        with pytest.raises(fa.LogicError):
+            d.download_redo()       # no frame run
+        with pytest.raises(ValueError):
+            d.set_persistent(-1)
+        d.upload_target(img(targets[0]))
+        d.set_fused(True)           # one class: the fused launch is the per-class one
+        res, st = d.process(img(targets[0]))  # first frame: FFTCC, sets the bad-ratio baseline
+        assert st["skip_fftcc_used"] == 0 and d.download_redo().all()
+        res2, st2 = d.process(img(targets[8]))
+        assert st2["skip_fftcc_used"] == 1
+        assert d.download_redo().sum() == st2["poi_fallback_count"]
