@@ -455,17 +455,97 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
     if (fftcc_lds_bytes(r, kFftccCached) > max_lds_per_block(device))
         fail(FI_E_UNSUPPORTED, "FFTCC window exceeds the LDS per block");
     stream = make_stream();
+    copy_stream = make_stream();
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
+    for (auto& t : tslot) FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
     iter_total.reserve(1);
     h_iter_total.reserve(1);
+    stager = std::thread([this] { stage_loop(); });
 }
 
 fi_frame::~fi_frame()
 {
+    {
+        std::lock_guard<std::mutex> lk(stage_mu);
+        stage_stop = true;
+    }
+    stage_cv.notify_all();
+    if (stager.joinable()) stager.join();
     for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& t : tslot)
+        if (t.ready) (void)hipEventDestroy(t.ready);
+    if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+// Staging thread: packs a queued target into its slot's pinned buffer and
+// starts the DMA on copy_stream.  The frame that last read the slot has
+// finished: run() and autotune() return only after their frame completes, and
+// at most two targets are queued, so a slot is refilled only after the run
+// that took it returned.
+void fi_frame::stage_loop()
+{
+    for (;;) {
+        StageJob job;
+        {
+            std::unique_lock<std::mutex> lk(stage_mu);
+            stage_cv.wait(lk, [&] { return stage_stop || !stage_jobs.empty(); });
+            if (stage_stop) return;
+            job = stage_jobs.front();
+            stage_jobs.pop_front();
+        }
+        TargetSlot& t = tslot[job.slot];
+        int code = 0;
+        std::string msg;
+        try {
+            DeviceGuard g(device);
+            t.img.stage(job.px, job.w, job.h, job.stride, job.pixel, job.depth12);
+            t.img.dma(copy_stream);
+            FI_HIP(hipEventRecord(t.ready, copy_stream));
+        } catch (const Error& e) {
+            code = e.code;
+            msg = e.what();
+        } catch (const std::exception& e) {
+            code = FI_E_DEVICE;
+            msg = e.what();
+        }
+        {
+            std::lock_guard<std::mutex> lk(stage_mu);
+            t.err_code = code;
+            t.err = msg;
+            t.state = 2;
+        }
+        stage_cv.notify_all();
+    }
+}
+
+// The slot holding the next frame's target once its rows are staged and its
+// DMA is queued; consume: take it off the queue.
+int fi_frame::take_target(bool consume)
+{
+    int s;
+    if (!pending.empty()) {
+        s = pending.front();
+        if (consume) pending.pop_front();
+    } else {
+        s = last_slot;
+    }
+    if (s < 0) fail(FI_E_LOGIC, "frame driver needs an uploaded target");
+    TargetSlot& t = tslot[s];
+    std::unique_lock<std::mutex> lk(stage_mu);
+    stage_cv.wait(lk, [&] { return t.state != 1; });
+    if (t.err_code) {
+        const int c = t.err_code;
+        const std::string m = t.err;
+        t.err_code = 0;
+        t.state = 0;
+        if (s == last_slot) last_slot = -1;
+        lk.unlock();
+        fail(c, m);
+    }
+    return s;
 }
 
 void fi_frame::invalidate_graphs()
@@ -501,13 +581,31 @@ void fi_frame::set_self_adaptive(bool on)
     reset_sequence();
 }
 
+// Queues one target (host pixels) for the next fi_frame_run: the rows are
+// copied to pinned memory and sent to the device by the staging thread, so the
+// copy overlaps a frame that is running.  The caller's pixels must stay valid
+// until the next run (or upload) returns.
 void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12)
 {
     DeviceGuard g(device);
     if (ref_set && (w != ref_raw.view.width || h != ref_raw.view.height))
         fail(FI_E_INVALID_ARGUMENT, "reference " + size_str(ref_raw.view.width, ref_raw.view.height) +
                                         " and target " + size_str(w, h) + " sizes differ");
-    tar.upload(px, w, h, stride, pixel, depth12, stream);
+    check_image_args(px, w, h, stride, pixel, depth12);
+    check_depth12(px, w, h, stride, pixel, depth12);
+    if (pending.size() >= 2) fail(FI_E_LOGIC, "two targets are already queued; run a frame first");
+    const int s = next_slot;
+    next_slot ^= 1;
+    {
+        std::unique_lock<std::mutex> lk(stage_mu);
+        stage_cv.wait(lk, [&] { return tslot[s].state != 1; });
+        tslot[s].state = 1;
+        tslot[s].err_code = 0;
+        stage_jobs.push_back({ s, px, w, h, stride, pixel, depth12 });
+    }
+    stage_cv.notify_all();
+    pending.push_back(s);
+    if (last_slot == s) last_slot = -1;
     tar_ready = true;
 }
 
@@ -810,6 +908,9 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     st.baseline_bad_ratio = NAN;
     st.fallback_threshold = NAN;
     prep_target();
+    const int ts = take_target(true);
+    const DevImage tview = target(ts);
+    FI_HIP(hipStreamWaitEvent(stream, tslot[ts].ready, 0));
 
     const bool skip = (policy & FI_POLICY_SKIP_FFTCC) && baseline_ready && have_prev;
     const bool learned = !skip && (policy & FI_POLICY_LEARNED_SEED) && seeder != nullptr;
@@ -821,8 +922,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         zero_counters();
         launch_fill_bytes(redo.p, n, mode == 1 ? 1 : 0, stream);
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
-        launch_bspline(tar.view, coef.p, stream);
-        launch_to_float(tar.view, tar_f.p, stream);
+        launch_bspline(tview, coef.p, stream);
+        launch_to_float(tview, tar_f.p, stream);
         if (timed) FI_HIP(hipEventRecord(ev[1], stream));
         if (mode == 0) {
             FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
@@ -855,7 +956,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 
     if (use_graph) {
         const uint64_t key[8] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
-                                  (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tar.view.data,
+                                  (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tview.data,
                                   (uint64_t)(uintptr_t)coef.p, (uint64_t)(uintptr_t)tar_f.p,
                                   (uint64_t)(uintptr_t)seeder };
         hipGraphExec_t exec = nullptr;
@@ -947,6 +1048,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         qa = read_counters(nullptr);
     }
     FI_HIP(hipEventRecord(ev[4], stream));
+    last_slot = ts;
     FI_HIP(hipEventSynchronize(ev[4]));
 
     const float bad_ratio = cnt ? (float)qa.bad / (float)cnt : 0.f;
@@ -1001,8 +1103,13 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
     if (!prepared) prepare();
     if (!tar_ready) fail(FI_E_LOGIC, "autotune needs an uploaded target");
     prep_target();
-    launch_bspline(tar.view, coef.p, stream);
-    launch_to_float(tar.view, tar_f.p, stream);
+    // the queued target is taken as a frame takes it (a run without a new
+    // upload processes it again)
+    const int ts = take_target(true);
+    FI_HIP(hipStreamWaitEvent(stream, tslot[ts].ready, 0));
+    launch_bspline(target(ts), coef.p, stream);
+    launch_to_float(target(ts), tar_f.p, stream);
+    last_slot = ts;
     // FFTCC seeds of the template in `save`
     FI_HIP(hipMemcpyAsync(save.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
     fftcc_all(save.p);

@@ -6,8 +6,13 @@
 #pragma once
 
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
+#include <deque>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "fi_host.h"
@@ -108,7 +113,33 @@ struct fi_frame {
     fi_seeder* seeder = nullptr;   // learned seeder (FI_POLICY_LEARNED_SEED)
     float seed_threshold = 0.8f;   // ZNCC below which a learned seed falls back to FFTCC
     hipStream_t stream = nullptr;
-    fi::DeviceImage ref_raw, tar;
+    fi::DeviceImage ref_raw;
+    // Targets: two slots filled by a staging thread (host copy into pinned
+    // memory, DMA on copy_stream) while frames run; run() takes the oldest
+    // uploaded slot, or re-runs the last one when none is queued.
+    struct TargetSlot {
+        fi::DeviceImage img;
+        hipEvent_t ready = nullptr;  // DMA done (copy_stream)
+        int state = 0;               // 0 idle, 1 staging, 2 staged
+        int err_code = 0;
+        std::string err;
+    };
+    TargetSlot tslot[2];
+    std::deque<int> pending;
+    int next_slot = 0, last_slot = -1;
+    hipStream_t copy_stream = nullptr;
+    std::thread stager;
+    std::mutex stage_mu;
+    std::condition_variable stage_cv;
+    struct StageJob {
+        int slot;
+        const void* px;
+        int w, h;
+        size_t stride;
+        int pixel, depth12;
+    };
+    std::deque<StageJob> stage_jobs;
+    bool stage_stop = false;
     fi::DevArray<float> ref_f, gx, gy, coef, tar_f;
     fi::DevArray<fi::PoiPrep> prep;
     std::vector<fi_poi> tmpl_host;        // template in the caller's order
@@ -166,5 +197,8 @@ private:
     fi::FrameCounters read_counters(fi::FrameCounters* per_class);
     void zero_counters();
     size_t nclass() const { return classes.size(); }
+    void stage_loop();
+    int take_target(bool consume);   // slot of the next frame's target, staged
+    const fi::DevImage& target(int slot) const { return tslot[slot].img.view; }
 };
 

@@ -36,38 +36,53 @@ void check_image_args(const void* px, int width, int height, size_t stride_bytes
                                         " is shorter than a row");
 }
 
-void DeviceImage::upload(const void* px, int width, int height, size_t stride_bytes, int pixel,
-                         int depth12, hipStream_t s)
+void check_depth12(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12)
+{
+    if (depth12 != FI_DEPTH12_RIGHT || pixel != FI_PIXEL_U16) return;
+    uint16_t mx = 0;
+    for (int y = 0; y < height; y++) {
+        const uint16_t* row = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes);
+        for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
+    }
+    // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
+    if (mx > 4095) fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
+}
+
+void DeviceImage::stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12)
 {
     check_image_args(px, width, height, stride_bytes, pixel, depth12);
-    const size_t bpp = bytes_per_pixel(pixel);
-    const size_t row_bytes = (size_t)width * bpp;
+    const size_t row_bytes = (size_t)width * bytes_per_pixel(pixel);
     const size_t total = row_bytes * height;
     // rows packed into pinned memory (the copy out of pageable memory is a
     // plain memcpy; the DMA then runs at the link rate)
-    stage.reserve(total);
-    uint16_t mx = 0;
-    for (int y = 0; y < height; y++) {
-        const uint8_t* src = static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes;
-        std::memcpy(stage.p + (size_t)y * row_bytes, src, row_bytes);
-        if (depth12 == FI_DEPTH12_RIGHT) {
-            const uint16_t* row = reinterpret_cast<const uint16_t*>(stage.p + (size_t)y * row_bytes);
-            for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
-        }
-    }
-    // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
-    if (depth12 == FI_DEPTH12_RIGHT && mx > 4095)
-        fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
+    this->stage_buf.reserve(total);
+    for (int y = 0; y < height; y++)
+        std::memcpy(stage_buf.p + (size_t)y * row_bytes, static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes,
+                    row_bytes);
     raw.reserve(total);
-    FI_HIP(hipMemcpyAsync(raw.p, stage.p, total, hipMemcpyHostToDevice, s));
-    FI_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next upload
+    staged_bytes = total;
     view.data = raw.p;
     view.width = width;
     view.height = height;
     view.pixel = pixel;
     view.pitch = (size_t)width;
     view.scale = (depth12 == FI_DEPTH12_LEFT) ? (1.f / 16.f) : 1.f;
+}
+
+void DeviceImage::dma(hipStream_t s)
+{
+    FI_HIP(hipMemcpyAsync(raw.p, stage_buf.p, staged_bytes, hipMemcpyHostToDevice, s));
     set = true;
+}
+
+void DeviceImage::upload(const void* px, int width, int height, size_t stride_bytes, int pixel,
+                         int depth12, hipStream_t s)
+{
+    check_image_args(px, width, height, stride_bytes, pixel, depth12);
+    check_depth12(px, width, height, stride_bytes, pixel, depth12);
+    stage(px, width, height, stride_bytes, pixel, depth12);
+    dma(s);
+    FI_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next upload
 }
 
 void dft_twiddles(int n, float* out)

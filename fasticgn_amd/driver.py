@@ -42,7 +42,11 @@ class FrameDriver(_Handle):
         self.n = len(q)
 
     def upload_target(self, tar: Image2D) -> None:
+        """Queue a target for the next run(); the copy to the device proceeds
+        in the background (at most two targets queued)."""
         nat.call("fi_frame_upload_target", self._h, *tar.args())
+        # the library reads the pixels until the frame that uses them starts
+        self._queued = (getattr(self, "_queued", ()) + (tar,))[-3:]
 
     def run(self, policy: int = DEFAULT_POLICY) -> dict:
         st = FrameStats()

@@ -180,7 +180,11 @@ int fi_frame_set_ref(fi_frame* h, const void* px, int width, int height,
                      size_t stride_bytes, int pixel, int depth12);
 /* POI template (buildPoiQueue, :2439-2472, or any explicit list). */
 int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n);
-/* Upload one target (host memory) into the device staging buffer. */
+/* Queue one target (host memory) for the next fi_frame_run: a staging thread
+ * copies it to pinned memory and the device while a running frame computes
+ * (two slots; a third queued target is FI_E_LOGIC).  The pixels must stay
+ * valid until the next fi_frame_run / fi_frame_autotune returns.  A run with
+ * no new target processes the last one again. */
 int fi_frame_upload_target(fi_frame* h, const void* px, int width, int height,
                            size_t stride_bytes, int pixel, int depth12);
 /* Process the uploaded target with the given policy; results stay on the
@@ -222,7 +226,8 @@ int fi_frame_set_fused(fi_frame* h, int on);
  * shape, sampling mode and launch mode (one block per POI, or the persistent
  * work queue) for each radius class and keep the fastest; with several
  * classes, also one launch over all of them (report rows with radius 0),
- * kept when faster than the best per-class launches together.  report (may be
+ * kept when faster than the best per-class launches together.  Takes the
+ * queued target as a frame does.  report (may be
  * NULL) receives up to report_len timed configurations; *report_n (may be
  * NULL) the number timed.  Does not advance the frame sequence. */
 typedef struct fi_tune_entry {

@@ -212,3 +212,30 @@ def test_frame_option_errors(seq, targets):
         res2, st2 = d.process(img(targets[8]))
         assert st2["skip_fftcc_used"] == 1
         assert d.download_redo().sum() == st2["poi_fallback_count"]
+
+
+def test_streamed_uploads_match_serial_frames(seq, targets):
+    """Target k+1 queued while frame k runs (two slots, background copy):
+    byte-identical to uploading and running one frame at a time; a third
+    queued target is refused; a 12-bit image above 4095 is refused at upload."""
+    with mixed_driver(seq) as d:
+        serial = run_sequence(d, targets)
+    with mixed_driver(seq, graph=True) as d:
+        streamed = []
+        d.upload_target(img(targets[STEPS[0]]))
+        for i, k in enumerate(STEPS):
+            if i + 1 < len(STEPS):
+                d.upload_target(img(targets[STEPS[i + 1]]))
+            st = d.run()
+            streamed.append((d.download(), st))
+        d.upload_target(img(targets[STEPS[0]]))
+        d.upload_target(img(targets[STEPS[1]]))
+        with pytest.raises(fa.LogicError):
+            d.upload_target(img(targets[STEPS[2]]))
+        bad = targets[STEPS[0]].copy()
+        bad[3, 3] = 5000
+        d.run()
+        with pytest.raises(ValueError):
+            d.upload_target(img(bad))
+    for f, ((a, _), (b, _)) in enumerate(zip(serial, streamed)):
+        assert same_bytes(a, b), f
