@@ -239,3 +239,29 @@ def test_streamed_uploads_match_serial_frames(seq, targets):
             d.upload_target(img(bad))
     for f, ((a, _), (b, _)) in enumerate(zip(serial, streamed)):
         assert same_bytes(a, b), f
+
+
+def test_graph_replay_with_persistent_queue_and_learned_seed(seq, targets):
+    """Graph replay byte-identical to eager frames with the persistent work
+    queue (its counters zeroed by a captured fill kernel) and with the
+    learned seeder inside the graph."""
+    with mixed_driver(seq, persistent=96) as d:
+        eager = run_sequence(d, targets)
+    with mixed_driver(seq, persistent=96, graph=True) as d:
+        graph = run_sequence(d, targets)
+    for f, ((er, _), (gr, _)) in enumerate(zip(eager, graph)):
+        assert same_bytes(er, gr), f
+    from fasticgn_amd import seeder as sd
+    sc = synth.config0()
+    outs = []
+    for use_graph in (False, True):
+        with sd.LearnedSeeder() as s, fa.FrameDriver(15) as d:
+            d.set_reference(fa.Image2D(sc.ref))
+            d.set_pois(sc.pois)
+            d.set_seeder(s, 0.8)
+            d.set_graph(use_graph)
+            runs = [d.process(fa.Image2D(sc.tar), policy=fa.driver.LEARNED_SEED) for _ in range(3)]
+        outs.append(runs)
+    for (er, es), (gr, gs) in zip(*outs):
+        assert same_bytes(er, gr)
+        assert es["seed_fallback_count"] == gs["seed_fallback_count"]
