@@ -456,6 +456,9 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
         fail(FI_E_UNSUPPORTED, "FFTCC window exceeds the LDS per block");
     stream = make_stream();
     copy_stream = make_stream();
+    aux_stream = make_stream();
+    FI_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    FI_HIP(hipEventCreate(&join_ev));
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
     for (auto& t : tslot) FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
     iter_total.reserve(1);
@@ -476,6 +479,9 @@ fi_frame::~fi_frame()
         if (e) (void)hipEventDestroy(e);
     for (auto& t : tslot)
         if (t.ready) (void)hipEventDestroy(t.ready);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (aux_stream) (void)hipStreamDestroy(aux_stream);
     if (copy_stream) (void)hipStreamDestroy(copy_stream);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -922,7 +928,13 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         zero_counters();
         launch_fill_bytes(redo.p, n, mode == 1 ? 1 : 0, stream);
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
-        launch_bspline(tview, coef.p, stream);
+        // the B-spline table (HBM-write-bound) on a second stream, beside the
+        // seeding (FFTCC or the seeder read only the f32 images); joined
+        // before IC-GN
+        FI_HIP(hipEventRecord(fork_ev, stream));
+        FI_HIP(hipStreamWaitEvent(aux_stream, fork_ev, 0));
+        launch_bspline(tview, coef.p, aux_stream);
+        FI_HIP(hipEventRecord(join_ev, aux_stream));
         launch_to_float(tview, tar_f.p, stream);
         if (timed) FI_HIP(hipEventRecord(ev[1], stream));
         if (mode == 0) {
@@ -934,6 +946,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         } else {
             launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);
         }
+        FI_HIP(hipStreamWaitEvent(stream, join_ev, 0));
         if (timed) FI_HIP(hipEventRecord(ev[2], stream));
         icgn_all(cur.p);
         if (timed) FI_HIP(hipEventRecord(ev[3], stream));
@@ -1075,7 +1088,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         st.bspline_ms = st.fftcc_ms = st.icgn_ms = NAN;
         st.seed_ms = mode == 0 ? NAN : 0.0;
     } else {
-        FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        FI_HIP(hipEventElapsedTime(&ms, ev[0], join_ev));  // table on its stream (overlapping the seeding)
         st.bspline_ms = ms;
         FI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
         if (mode == 0)

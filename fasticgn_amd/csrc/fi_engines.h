@@ -128,6 +128,8 @@ struct fi_frame {
     std::deque<int> pending;
     int next_slot = 0, last_slot = -1;
     hipStream_t copy_stream = nullptr;
+    hipStream_t aux_stream = nullptr;      // B-spline table beside the seeding
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     std::thread stager;
     std::mutex stage_mu;
     std::condition_variable stage_cv;

@@ -81,7 +81,7 @@ typedef struct fi_frame_stats {
     int64_t iteration_sum;        /* sum of executed IC-GN iterations */
     int64_t fftcc_pois;           /* POIs run through FFTCC this frame */
     double device_ms;             /* device time of the frame (HIP events) */
-    double bspline_ms;            /* target B-spline table */
+    double bspline_ms;            /* target B-spline table (on its own stream, beside the seeding) */
     double fftcc_ms;              /* FFTCC seeding (or temporal seed copy) */
     double icgn_ms;               /* first IC-GN pass over the frame */
     int64_t seed_pois;            /* POIs seeded by the learned seeder */
