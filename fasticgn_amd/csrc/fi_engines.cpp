@@ -924,9 +924,9 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     const int mode = learned ? 0 : (skip ? 2 : 1);
 
     auto segment = [&](bool timed) {
-        launch_zero_words(reinterpret_cast<uint32_t*>(iter_total.p), 2, stream);
-        zero_counters();
-        launch_fill_bytes(redo.p, n, mode == 1 ? 1 : 0, stream);
+        // a fill kernel, not memsets: this runs inside captured frame graphs
+        launch_frame_reset(iter_total.p, reinterpret_cast<uint32_t*>(counters.p),
+                           (C + 1) * sizeof(FrameCounters) / 4, redo.p, n, mode == 1 ? 1 : 0, stream);
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
         // the B-spline table (HBM-write-bound) on a second stream, beside the
         // seeding (FFTCC or the seeder read only the f32 images); joined

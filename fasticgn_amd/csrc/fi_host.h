@@ -139,5 +139,8 @@ void launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
 // flags[idx[j]] = 1, j < n; p[0..n) = v (bytes)
 void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s);
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
+// *total = 0, words[0..n_words) = 0, flags[0..n) = v: one launch
+void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
+                        uint8_t v, hipStream_t s);
 
 }  // namespace fi

@@ -246,6 +246,25 @@ __global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v)
         p[i] = v;
 }
 
+// Start of a frame in one launch: zero the iteration total and the counters,
+// set every per-POI FFTCC-redo flag to v.
+__global__ void k_frame_reset(unsigned long long* __restrict__ total, uint32_t* __restrict__ words, size_t n_words,
+                              uint8_t* __restrict__ flags, size_t n, uint8_t v)
+{
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    if (t == 0) *total = 0ull;
+    for (size_t i = t; i < n_words; i += stride) words[i] = 0u;
+    for (size_t i = t; i < n; i += stride) flags[i] = v;
+}
+
+void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
+                        uint8_t v, hipStream_t s)
+{
+    const size_t m = std::max(n_words, n);
+    hipLaunchKernelGGL(k_frame_reset, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + 255) / 256, 1024))),
+                       dim3(256), 0, s, total, words, n_words, flags, n, v);
+}
+
 void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_mark, grid_for(n), dim3(256), 0, s, idx, n, flags);
