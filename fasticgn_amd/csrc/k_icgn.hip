@@ -284,19 +284,19 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 }
 
 // The per-POI LDS tile: the subset's neighbourhood (2r+1 + 3 pixels) plus
-// kTileMargin on each side, when the image is that large and the f32 target
-// is given.  (A wave64 two-dword LDS read occupies the 32 banks for four
-// cycles whatever the row stride; padding the stride buys nothing.)
+// kTileMargin on each side, rows kTileStride words apart, when the image is
+// that large, the tile at most kTileStride wide (radius <= 28) and the f32
+// target is given; otherwise the kernel samples the B-spline table.
 static IcgnParams with_tile(const IcgnParams& p)
 {
     IcgnParams q = p;
     q.tile_w = q.tile_h = q.tile_stride = 0;
     if (p.tar == nullptr) return q;
     const int tw = 2 * p.rx + 4 + 2 * kTileMargin, th = 2 * p.ry + 4 + 2 * kTileMargin;
-    if (tw > p.w || th > p.h) return q;
+    if (tw > p.w || th > p.h || tw > kTileStride) return q;
     q.tile_w = tw;
     q.tile_h = th;
-    q.tile_stride = tw;
+    q.tile_stride = kTileStride;
     return q;
 }
 

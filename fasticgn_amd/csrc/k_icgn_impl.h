@@ -138,18 +138,24 @@ __device__ __forceinline__ float row4(const float* q, float w0, float w1, float 
     return __builtin_fmaf(w3, q[3], __builtin_fmaf(w2, q[2], __builtin_fmaf(w1, q[1], w0 * q[0])));
 }
 
+// LDS row stride of the per-POI tile (tiles up to 64 pixels wide, radius <=
+// 28): a neighbourhood's four rows lie 0, 64, 128 and 192 words from one
+// address, so its 16 pixels are eight ds_read2_b32 with immediate offsets.
+constexpr int kTileStride = 64;
+
 // Sample at (gx, gy) from a tile whose element (0, 0) is image pixel
-// (tx0, ty0); cbase = (ty0 + 1) * stride + tx0 + 1 places the neighbourhood's
-// first row and column.  The caller guarantees the neighbourhood lies in the
-// tile.
-__device__ __forceinline__ float tile_sample(const float* tile, int stride, int cbase, float gx, float gy)
+// (tx0, ty0); cbase = (ty0 + 1) * kTileStride + tx0 + 1 places the
+// neighbourhood's first row and column.  The caller guarantees the
+// neighbourhood lies in the tile.
+__device__ __forceinline__ float tile_sample(const float* tile, int cbase, float gx, float gy)
 {
     const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
-    const int base = (int)__umul24((uint32_t)(int)gy, (uint32_t)stride) + (int)gx - cbase;
+    const int base = ((int)gy << 6) + (int)gx - cbase;
+    static_assert(kTileStride == 64, "base uses a shift by 6");
     const float* q0 = tile + base;
-    const float* q1 = q0 + stride;
-    const float* q2 = q1 + stride;
-    const float* q3 = q2 + stride;
+    const float* q1 = q0 + kTileStride;
+    const float* q2 = q1 + kTileStride;
+    const float* q3 = q2 + kTileStride;
     float x0, x1, x2, x3, y0, y1, y2, y3;
     spline_weights(dx, x0, x1, x2, x3);
     spline_weights(dy, y0, y1, y2, y3);
@@ -522,7 +528,7 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
             }
         };
         if (use_tile)
-            pass1([&](float gx, float gy, int) { return tile_sample(tile, tile_stride, cbase, gx, gy); });
+            pass1([&](float gx, float gy, int) { return tile_sample(tile, cbase, gx, gy); });
         else
             pass1([&](float gx, float gy, int) { return bspline_sample(table, row_bytes, plane_bytes, gx, gy); });
         // lanes mirroring band 0 drop their t' sums (their r is zero)
