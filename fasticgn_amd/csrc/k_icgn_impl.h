@@ -455,8 +455,10 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     float conv = 0.f, zncc = 0.f;
     do {
         it++;
-        const float bx = __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
-        const float by = __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
+        // sample k of the lane sits at (xb + k W01, yb + k W11): the POI
+        // centre, the warp of the lane's column and first row, then its steps
+        const float xb = x + __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
+        const float yb = y + __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
         // The samples are an affine image of the grid: they lie within the
         // warped bounding box x + W02 +- (|W00| rx + |W01| ry), so all are in
         // the interpolatable range (:39-43) when the box is, by a margin.  A NaN
@@ -474,8 +476,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
 #pragma unroll
             for (int k = 0; k < SPL; k++) {
                 if (EXACT || k < R) {
-                    const float gxp = x + __builtin_fmaf((float)k, W01, bx);
-                    const float gyp = y + __builtin_fmaf((float)k, W11, by);
+                    const float gxp = __builtin_fmaf((float)k, W01, xb);
+                    const float gyp = __builtin_fmaf((float)k, W11, yb);
                     oob |= (gxp < 1.f) | (gyp < 1.f) | (gxp >= wm2) | (gyp >= hm2);
                     nan |= (gxp != gxp) | (gyp != gyp);
                 }
@@ -516,8 +518,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
             for (int k = 0; k < SPL; k++) {
                 tpv[k] = 0.f;
                 if (EXACT || k < R) {
-                    const float gxp = x + __builtin_fmaf((float)k, W01, bx);
-                    const float gyp = y + __builtin_fmaf((float)k, W11, by);
+                    const float gxp = __builtin_fmaf((float)k, W01, xb);
+                    const float gyp = __builtin_fmaf((float)k, W11, yb);
                     const float tp = sample(gxp, gyp, k) - ref_mean;
                     const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
                     tpv[k] = tp;
