@@ -190,3 +190,103 @@ private:
 };
 
 }  // namespace opencorr
+
+// The multi-frame driver and the learned seeder: no OpenCorr class exists for
+// them (the reference's sequence loop lives in opencorr_cli_cuda_mvp.cpp:
+// processFrame :3024-3210, runMultiFrame :3834-4509), so they live in their
+// own namespace, over the same POI2D and Image2D.
+namespace fasticgn {
+
+class LearnedSeeder {
+public:
+    explicit LearnedSeeder(int device = 0) { opencorr::detail::check(fi_seeder_create(device, &h_)); }
+    ~LearnedSeeder() { fi_seeder_destroy(h_); }
+    LearnedSeeder(const LearnedSeeder&) = delete;
+    LearnedSeeder& operator=(const LearnedSeeder&) = delete;
+    static size_t paramCount()
+    {
+        size_t n = 0;
+        opencorr::detail::check(fi_seeder_param_count(&n));
+        return n;
+    }
+    void setParams(const std::vector<float>& p) { opencorr::detail::check(fi_seeder_set_params(h_, p.data(), p.size())); }
+    std::vector<float> params() const
+    {
+        std::vector<float> p(paramCount());
+        opencorr::detail::check(fi_seeder_get_params(h_, p.data(), p.size()));
+        return p;
+    }
+    fi_seeder* handle() const { return h_; }
+
+private:
+    fi_seeder* h_ = nullptr;
+};
+
+class FrameDriver {
+public:
+    FrameDriver(int subset_radius, float conv_criterion = 0.001f, float stop_condition = 10.f, int device = 0)
+    {
+        opencorr::detail::check(fi_frame_create(subset_radius, conv_criterion, stop_condition, device, &h_));
+    }
+    ~FrameDriver() { fi_frame_destroy(h_); }
+    FrameDriver(const FrameDriver&) = delete;
+    FrameDriver& operator=(const FrameDriver&) = delete;
+
+    void setReference(const opencorr::Image2D& ref)
+    {
+        opencorr::detail::check(
+            fi_frame_set_ref(h_, ref.data, ref.width, ref.height, ref.stride_bytes, ref.pixel, ref.depth12));
+    }
+    void setPois(const std::vector<opencorr::POI2D>& tmpl)
+    {
+        opencorr::detail::check(fi_frame_set_pois(h_, opencorr::detail::raw(tmpl), tmpl.size()));
+        n_ = tmpl.size();
+    }
+    void setSelfAdaptive(bool on) { opencorr::detail::check(fi_frame_set_self_adaptive(h_, on ? 1 : 0)); }
+    void setGraph(bool on) { opencorr::detail::check(fi_frame_set_graph(h_, on ? 1 : 0)); }
+    void setPersistent(int blocks) { opencorr::detail::check(fi_frame_set_persistent(h_, blocks)); }
+    void setSeeder(LearnedSeeder* s, float zncc_threshold = 0.8f)
+    {
+        opencorr::detail::check(fi_frame_set_seeder(h_, s ? s->handle() : nullptr, zncc_threshold));
+    }
+    // Queues a target; its pixels must stay valid until the next run() returns.
+    void uploadTarget(const opencorr::Image2D& tar)
+    {
+        opencorr::detail::check(
+            fi_frame_upload_target(h_, tar.data, tar.width, tar.height, tar.stride_bytes, tar.pixel, tar.depth12));
+    }
+    fi_frame_stats run(unsigned policy = FI_POLICY_DEFAULT)
+    {
+        fi_frame_stats st{};
+        opencorr::detail::check(fi_frame_run(h_, policy, &st));
+        return st;
+    }
+    void download(std::vector<opencorr::POI2D>& out)
+    {
+        if (out.size() != n_) out.assign(n_, opencorr::POI2D(0.f, 0.f));
+        opencorr::detail::check(fi_frame_download(h_, opencorr::detail::raw(out), out.size()));
+    }
+    fi_frame_stats process(const opencorr::Image2D& tar, std::vector<opencorr::POI2D>& out,
+                           unsigned policy = FI_POLICY_DEFAULT)
+    {
+        uploadTarget(tar);
+        const fi_frame_stats st = run(policy);
+        download(out);
+        return st;
+    }
+    std::vector<fi_tune_entry> autotune()
+    {
+        std::vector<fi_tune_entry> rep(512);
+        size_t m = 0;
+        opencorr::detail::check(fi_frame_autotune(h_, rep.data(), rep.size(), &m));
+        rep.resize(m);
+        return rep;
+    }
+    void resetSequence() { opencorr::detail::check(fi_frame_reset_sequence(h_)); }
+
+private:
+    fi_frame* h_ = nullptr;
+    size_t n_ = 0;
+};
+
+}  // namespace fasticgn

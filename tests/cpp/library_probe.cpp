@@ -61,5 +61,27 @@ int main()
     int close = 0;
     for (const auto& p : q) close += std::fabs(p.deformation.u - 2.f) < 0.01f && std::fabs(p.deformation.v + 1.f) < 0.01f;
     check("icgn-integer-shift", close == (int)q.size());
+
+    // multi-frame driver and learned seeder (namespace fasticgn)
+    fasticgn::FrameDriver drv(7);
+    drv.setReference(R);
+    drv.setPois(layout);
+    drv.setGraph(true);
+    std::vector<POI2D> res;
+    const fi_frame_stats s0 = drv.process(T, res);
+    const fi_frame_stats s1 = drv.process(T, res);
+    close = 0;
+    for (const auto& p : res) close += std::fabs(p.deformation.u - 2.f) < 0.01f && std::fabs(p.deformation.v + 1.f) < 0.01f;
+    check("frame-driver-sequence", close == (int)layout.size() && s0.skip_fftcc_used == 0 && s1.skip_fftcc_used == 1);
+    fasticgn::LearnedSeeder seeder;
+    const std::vector<float> params = seeder.params();
+    check("seeder-param-count", params.size() == fasticgn::LearnedSeeder::paramCount() && params.size() == 44534);
+    drv.setSeeder(&seeder);
+    drv.resetSequence();
+    const fi_frame_stats s2 = drv.process(T, res, FI_POLICY_LEARNED_SEED);
+    close = 0;
+    for (const auto& p : res) close += std::fabs(p.deformation.u - 2.f) < 0.01f && std::fabs(p.deformation.v + 1.f) < 0.01f;
+    check("frame-driver-learned-seed", close == (int)layout.size() && s2.seed_pois == (int64_t)layout.size());
+    drv.setSeeder(nullptr);
     return failures ? 1 : 0;
 }
