@@ -150,6 +150,7 @@ SIGNATURES = {
     "fi_frame_set_shape": [_P, _I, _I],
     "fi_frame_set_sampling": [_P, _I],
     "fi_frame_set_seeder": [_P, _P, _F],
+    "fi_frame_set_seed_passes": [_P, _I],
     "fi_frame_set_self_adaptive": [_P, _I],
     "fi_frame_set_graph": [_P, _I],
     "fi_frame_set_persistent": [_P, _I],

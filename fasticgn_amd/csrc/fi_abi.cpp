@@ -349,6 +349,16 @@ int fi_frame_set_fused(fi_frame* h, int on)
     });
 }
 
+int fi_frame_set_seed_passes(fi_frame* h, int passes)
+{
+    return guarded([&] {
+        need(h);
+        if (passes < 1 || passes > 4) fail(FI_E_INVALID_ARGUMENT, "seeder passes must be 1..4");
+        h->seed_passes = passes;
+        h->invalidate_graphs();
+    });
+}
+
 int fi_frame_set_graph(fi_frame* h, int on)
 {
     return guarded([&] {

@@ -695,7 +695,7 @@ void fi_frame::prepare()
     prep.reserve(cap);
     counters.reserve(nclass() + 1);
     h_counters.reserve(nclass() + 1);
-    queue.reserve(std::max<size_t>(8 * nclass(), 8));
+    queue.reserve((size_t)kQueueWords * std::max<size_t>(nclass(), 1));
     if (n > 0) FI_HIP(hipMemcpyAsync(tmpl.p, lay, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
     for (auto& cp : classes) {
         FrameClass& c = *cp;
@@ -787,9 +787,9 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     if (qb > 0) {
         int ci = 0;
         while (classes[ci].get() != &c) ci++;
-        ip.queue = queue.p + 8 * ci;
+        ip.queue = queue.p + kQueueWords * ci;
         ip.queue_blocks = qb;
-        launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), 8, stream);
+        launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), kQueueWords, stream);
     }
     icgn_launch(ip, waves, spl, stream);
     launch_sum_iters(ip.iters, count, iter_total.p, stream);
@@ -838,7 +838,7 @@ void fi_frame::icgn_fused(fi_poi* base)
     if (qb > 0) {
         ip.queue = queue.p;
         ip.queue_blocks = qb;
-        launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), 8, stream);
+        launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), kQueueWords, stream);
     }
     icgn_launch(ip, waves, spl, stream);
     launch_sum_iters(ip.iters, (int)n, iter_total.p, stream);
@@ -939,7 +939,12 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         if (timed) FI_HIP(hipEventRecord(ev[1], stream));
         if (mode == 0) {
             FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+            // each pass crops the target at the previous pass's rounded u, v; a
+            // POI a later pass cannot crop keeps the earlier seed, and only the
+            // first pass's flag decides the FFTCC fallback for an unseeded POI
             seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, seed_ok.p, stream);
+            for (int pass = 1; pass < seed_passes; pass++)
+                seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, nullptr, stream);
         } else if (mode == 1) {
             FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             fftcc_all(cur.p);
@@ -1130,6 +1135,20 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
     FI_HIP(hipStreamSynchronize(stream));
     int cus = 0;
     FI_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    // Resident-block counts tried for the queue launch: 0 (one block per POI),
+    // 12 waves per CU, and as many blocks as the kernel's registers and LDS
+    // let every CU hold.
+    auto queue_counts = [&](int r, int waves, int spl, int samp) {
+        IcgnParams op{};
+        op.tar = samp == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+        op.w = ref_raw.view.width;
+        op.h = ref_raw.view.height;
+        op.rx = op.ry = r;
+        std::vector<int> q = { 0, cus * 12 / waves };
+        const int fill = cus * icgn_queue_per_cu(op, waves, spl);
+        if (fill > 0 && fill != q[1]) q.push_back(fill);
+        return q;
+    };
     size_t m = 0;
     int timed = 0;
     for (auto& cp : classes) {
@@ -1144,7 +1163,7 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
             for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE }) {
                 // resident blocks of the queue launch: 12 waves per CU at the
                 // kernel's occupancy target of 3 waves per SIMD
-                for (int qb : { 0, cus * 12 / ws[si] }) {
+                for (int qb : queue_counts(c.r, ws[si], ss[si], samp)) {
                     c.waves = ws[si];
                     c.spl = ss[si];
                     c.sampling = samp;
@@ -1214,7 +1233,7 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
             for (auto& c : classes) all = all && icgn_shape_covers(W, 16, c->r, c->r);
             if (!all) continue;
             for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE }) {
-                for (int qb : { 0, cus * 12 / W }) {
+                for (int qb : queue_counts(classes.back()->r, W, 16, samp)) {
                     fused.waves = W;
                     fused.spl = 16;
                     fused.sampling = samp;

@@ -112,6 +112,7 @@ struct fi_frame {
     FrameClass fused;              // its launch configuration (r = largest radius)
     fi_seeder* seeder = nullptr;   // learned seeder (FI_POLICY_LEARNED_SEED)
     float seed_threshold = 0.8f;   // ZNCC below which a learned seed falls back to FFTCC
+    int seed_passes = 1;           // seeder passes, each re-centred on the previous one's rounded u, v
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw;
     // Targets: two slots filled by a staging thread (host copy into pinned
@@ -155,7 +156,7 @@ struct fi_frame {
     fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry
     fi::DevArray<unsigned long long> iter_total;
     fi::PinnedArray<unsigned long long> h_iter_total;
-    fi::DevArray<int> queue;       // persistent-queue claim counters, 8 per class
+    fi::DevArray<int> queue;       // persistent-queue claim counters, kQueueWords per class
     fi::PinnedArray<fi_poi> h_out;
     fi::DevArray<uint8_t> redo;    // per POI: re-run from FFTCC in the last frame
     size_t n = 0;

@@ -26,6 +26,10 @@ void icgn_default_shape(int rx, int ry, int& waves, int& spl);
 // Every instantiated shape that covers the subset (at most cap); returns the count.
 int icgn_shape_candidates(int rx, int ry, int* waves, int* spl, int cap);
 bool launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s);
+// Resident blocks per CU of the persistent-queue launch of shape (waves, spl)
+// for p's radius, image size and sampling (p.tar null = table); 0 if the shape
+// does not cover the radius.
+int icgn_queue_per_cu(const IcgnParams& p, int waves, int spl);
 bool launch_icgn(const IcgnParams& p, hipStream_t s);
 
 // Frame-policy helpers (k_policy.hip).

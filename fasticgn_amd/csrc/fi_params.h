@@ -64,6 +64,13 @@ struct FftccParams {
     const float* twiddles;    // 2n floats
 };
 
+// Claim counters of one persistent-queue IC-GN launch: kQueueParts counters
+// (one per XCD range), each on its own 256-byte line, since device-scope
+// atomics from every XCD to one line serialize.
+constexpr int kQueueParts = 8;
+constexpr int kQueueStride = 64;  // ints between counters
+constexpr int kQueueWords = kQueueParts * kQueueStride;
+
 struct IcgnParams {
     const float* ref;         // reference, f32 row-major
     const float* gx;

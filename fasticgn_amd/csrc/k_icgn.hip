@@ -201,7 +201,9 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
 // 1..31 at the fewest waves, plus alternatives for radii 10, 15 and 20; the
 // generic ones (spl = 16, slots past R skipped) cover the rest.
 // ---------------------------------------------------------------------------
-#define X(W, S) extern template void launch_icgn_t<W, S, true>(const IcgnParams&, hipStream_t);
+#define X(W, S)                                                                 \
+    extern template void launch_icgn_t<W, S, true>(const IcgnParams&, hipStream_t); \
+    extern template int icgn_queue_per_cu_t<W, S, true>(const IcgnParams&);
 FI_ICGN_EXACT_A(X)
 FI_ICGN_EXACT_B(X)
 #undef X
@@ -324,6 +326,25 @@ bool launch_icgn_shape(const IcgnParams& p0, int waves, int spl, hipStream_t s)
     FI_ICGN_GENERIC(X)
 #undef X
     return false;
+}
+
+int icgn_queue_per_cu(const IcgnParams& p0, int waves, int spl)
+{
+    const IcgnParams p = with_tile(p0);
+    const int R = band_rows(waves, p.rx, p.ry);
+    if (R == 0 || R > spl) return 0;
+    if (R == spl && exact_shape(waves, spl)) {
+#define X(W, S)                                                                 \
+        if (waves == W && spl == S) return icgn_queue_per_cu_t<W, S, true>(p);
+        FI_ICGN_EXACT_A(X)
+        FI_ICGN_EXACT_B(X)
+#undef X
+    }
+#define X(W, S)                                                                 \
+    if (waves == W && spl == S) return icgn_queue_per_cu_t<W, S, false>(p);
+    FI_ICGN_GENERIC(X)
+#undef X
+    return 0;
 }
 
 bool launch_icgn(const IcgnParams& p, hipStream_t s)

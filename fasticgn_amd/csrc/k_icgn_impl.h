@@ -631,9 +631,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
 // ranges, one per XCD (workgroups are dealt to the XCDs round-robin, so block
 // b runs on XCD b % 8 and neighbouring POIs share that XCD's L2); a block
 // claims kQueueChunk entries at a time from its own range, then from the
-// others' until every range is drained.  p.queue holds kQueueParts zeroed
-// counters.
-constexpr int kQueueParts = 8;
+// others' until every range is drained.  p.queue holds the kQueueParts
+// counters, kQueueStride ints apart, zeroed.
 constexpr int kQueueChunk = 4;
 
 template <int WAVES, int SPL, bool EXACT>
@@ -653,29 +652,48 @@ void k_icgn_queue(IcgnParams p)
 {
     __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
     __shared__ int flags[WAVES];
-    __shared__ int claim;
+    // claim[0]: the entry the block runs next, -1 once every range is
+    // drained; claim[1]: end of the chunk it came from; claim[2]: ranges
+    // thread 0 has passed since its own.  The walk state lives here rather
+    // than in registers, so that only the entry index stays live across an
+    // entry, as in k_icgn (the entry needs every scalar register it can get).
+    __shared__ int claim[3];
     extern __shared__ float tile[];
-    const int home = (int)(blockIdx.x % kQueueParts);
-    for (int k = 0; k < kQueueParts; k++) {
-        const int part = (home + k) % kQueueParts;
-        const int lo = (int)((long long)p.n * part / kQueueParts);
-        const int hi = (int)((long long)p.n * (part + 1) / kQueueParts);
-        for (;;) {
-            if (threadIdx.x == 0) {
-                // a drained range is seen with a load, not another atomic
-                const int seen = __hip_atomic_load(p.queue + part, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                claim = lo + seen >= hi ? hi : lo + atomicAdd(p.queue + part, kQueueChunk);
+    if (threadIdx.x == 0) {
+        claim[0] = -1;
+        claim[1] = 0;
+        claim[2] = 0;
+    }
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int b = claim[0] + 1;
+            if (claim[0] < 0 || b >= claim[1]) {
+                const int home = (int)(blockIdx.x % kQueueParts);
+                int k = claim[2];
+                b = -1;
+                for (; k < kQueueParts; k++) {
+                    const int part = (home + k) % kQueueParts;
+                    const int lo = (int)((long long)p.n * part / kQueueParts);
+                    const int hi = (int)((long long)p.n * (part + 1) / kQueueParts);
+                    // a drained range is seen with a load, not another atomic
+                    if (lo + __hip_atomic_load(p.queue + part * kQueueStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= hi)
+                        continue;
+                    const int c = lo + atomicAdd(p.queue + part * kQueueStride, kQueueChunk);
+                    if (c >= hi) continue;
+                    b = c;
+                    claim[1] = min(c + kQueueChunk, hi);
+                    break;
+                }
+                claim[2] = k;
             }
-            __syncthreads();
-            const int b0 = claim;
-            __syncthreads();  // every thread has read claim before it is rewritten
-            if (b0 >= hi) break;
-            const int b1 = min(b0 + kQueueChunk, hi);
-            for (int b = b0; b < b1; b++) {
-                icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
-                __syncthreads();  // shared state is reused by the next entry
-            }
+            claim[0] = b;
         }
+        __syncthreads();
+        const int b = __builtin_amdgcn_readfirstlane(claim[0]);
+        __syncthreads();  // every thread has read the entry before it is rewritten
+        if (b < 0) break;
+        icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+        __syncthreads();  // shared state is reused by the next entry
     }
 }
 
@@ -697,6 +715,20 @@ void launch_icgn_t(const IcgnParams& p, hipStream_t s)
                            dim3(64 * WAVES), lds, s, p);
     else
         hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), lds, s, p);
+}
+
+// Blocks of k_icgn_queue<WAVES, SPL, EXACT> resident on one CU at p's tile
+// size (register, LDS and wave limits, as the runtime computes them); 0 on
+// error.
+template <int WAVES, int SPL, bool EXACT>
+int icgn_queue_per_cu_t(const IcgnParams& p)
+{
+    const size_t lds = p.tile_w > 0 ? (size_t)p.tile_stride * p.tile_h * sizeof(float) : 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void*>(&k_icgn_queue<WAVES, SPL, EXACT>), 64 * WAVES, lds) != hipSuccess)
+        return 0;
+    return nb;
 }
 
 }  // namespace fi

@@ -4,7 +4,9 @@
 
 namespace fi {
 
-#define X(W, S) template void launch_icgn_t<W, S, true>(const IcgnParams&, hipStream_t);
+#define X(W, S)                                                                 \
+    template void launch_icgn_t<W, S, true>(const IcgnParams&, hipStream_t);     \
+    template int icgn_queue_per_cu_t<W, S, true>(const IcgnParams&);
 FI_ICGN_EXACT_B(X)
 #undef X
 

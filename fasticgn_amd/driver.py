@@ -85,6 +85,11 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_set_seeder", self._h, seeder.handle if seeder is not None else None,
                  float(zncc_threshold))
 
+    def set_seed_passes(self, passes: int) -> None:
+        """Seeder passes per frame (1..4): each re-crops the target at the
+        previous pass's rounded u, v."""
+        nat.call("fi_frame_set_seed_passes", self._h, int(passes))
+
     def set_self_adaptive(self, on: bool = True) -> None:
         """Mixed subset radii: every template POI's (square) subset_radius is
         its FFTCC and IC-GN radius (DIC::setSelfAdaptive); call before

@@ -224,7 +224,8 @@ int fi_frame_set_persistent(fi_frame* h, int blocks);
 int fi_frame_set_fused(fi_frame* h, int on);
 /* Autotuner: on the uploaded target (FFTCC-seeded), time every IC-GN lane
  * shape, sampling mode and launch mode (one block per POI, or the persistent
- * work queue) for each radius class and keep the fastest; with several
+ * work queue with 12 waves per CU or with as many blocks as every CU holds)
+ * for each radius class and keep the fastest; with several
  * classes, also one launch over all of them (report rows with radius 0),
  * kept when faster than the best per-class launches together.  Takes the
  * queued target as a frame does.  report (may be
@@ -276,6 +277,10 @@ int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, 
  * from FFTCC and merged (better ZNCC wins).  The seeder must outlive its use. */
 #define FI_POLICY_LEARNED_SEED 4u
 int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold);
+/* Seeder passes per frame (1..4, default 1): pass k crops the target at the
+ * rounded u, v of pass k-1, so a POI that starts far from its displacement
+ * (the seeder's +-8 px range) is seeded from a near crop by the second pass. */
+int fi_frame_set_seed_passes(fi_frame* h, int passes);
 
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */

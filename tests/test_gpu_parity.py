@@ -436,3 +436,30 @@ def test_learned_seed_bad_seeds_fall_back_to_fftcc(oracle, c0):
     assert same.mean() >= 0.95
     cmp = compare(plain, res)
     assert cmp["pass"], cmp
+
+
+def test_learned_seed_two_passes_parity(oracle, c0):
+    """Config 0 starts every POI at guess 0 with u = 2.35, v = -1.70: a second
+    seeder pass (crops re-centred on the first pass's rounded u, v) keeps the
+    parity gate against the CPU oracle and needs no more IC-GN iterations."""
+    from fasticgn_amd import seeder as sd
+    out = {}
+    for passes in (1, 2):
+        with sd.LearnedSeeder() as s, fa.FrameDriver(15) as d:
+            d.set_reference(fa.Image2D(c0.ref))
+            d.set_pois(c0.pois)
+            d.set_seeder(s, 0.8)
+            d.set_seed_passes(passes)
+            out[passes] = d.process(fa.Image2D(c0.tar), policy=fa.driver.LEARNED_SEED)
+    o = c0.pois.copy()
+    ref32, tar32 = c0.ref.astype(np.float32), c0.tar.astype(np.float32)
+    oracle.fftcc(ref32, tar32, 15, o)
+    oracle.icgn(ref32, tar32, 15, o)
+    cmp = compare(o, out[2][0])
+    report("learned 2 passes config0", cmp)
+    assert cmp["pass"], cmp
+    say("iterations 1 pass", out[1][1]["iteration_sum"], "2 passes", out[2][1]["iteration_sum"])
+    assert out[2][1]["iteration_sum"] <= out[1][1]["iteration_sum"]
+    with fa.FrameDriver(15) as d:
+        with pytest.raises(ValueError):
+            d.set_seed_passes(0)
