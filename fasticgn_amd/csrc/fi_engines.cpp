@@ -765,7 +765,9 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     ip.ref = ref_f.p;
     ip.gx = gx.p;
     ip.gy = gy.p;
-    ip.coef = coef.p;
+    // the table exists only while a class samples it; the LDS-tile mode
+    // samples the f32 target when a warp outgrows the tile
+    ip.coef = mode == FI_SAMPLING_TABLE ? coef.p : nullptr;
     ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
@@ -819,7 +821,9 @@ void fi_frame::icgn_fused(fi_poi* base)
     ip.ref = ref_f.p;
     ip.gx = gx.p;
     ip.gy = gy.p;
-    ip.coef = coef.p;
+    // the table exists only while a class samples it; the LDS-tile mode
+    // samples the f32 target when a warp outgrows the tile
+    ip.coef = mode == FI_SAMPLING_TABLE ? coef.p : nullptr;
     ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
@@ -888,12 +892,24 @@ FrameCounters fi_frame::read_counters(FrameCounters* per_class)
     return h_counters.p[nclass()];
 }
 
-void fi_frame::prep_target()
+void fi_frame::prep_target(bool table)
 {
     const int w = ref_raw.view.width, h = ref_raw.view.height;
-    check_table_size(w, h);
-    coef.reserve(bspline_table_floats(w, h));
+    if (table) {
+        check_table_size(w, h);
+        coef.reserve(bspline_table_floats(w, h));
+    }
     tar_f.reserve((size_t)w * h);
+}
+
+// The B-spline table is built only when some launch samples it (table mode
+// set for the frame, a class or the fused launch).
+bool fi_frame::table_needed() const
+{
+    if (sampling == FI_SAMPLING_TABLE || fused.sampling == FI_SAMPLING_TABLE) return true;
+    for (auto& c : classes)
+        if (c->sampling == FI_SAMPLING_TABLE) return true;
+    return false;
 }
 
 // processFrame, opencorr_cli_cuda_mvp.cpp:3024-3210.  The device work up to
@@ -913,7 +929,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     st.frame_index = frame_index;
     st.baseline_bad_ratio = NAN;
     st.fallback_threshold = NAN;
-    prep_target();
+    const bool table = table_needed();
+    prep_target(table);
     const int ts = take_target(true);
     const DevImage tview = target(ts);
     FI_HIP(hipStreamWaitEvent(stream, tslot[ts].ready, 0));
@@ -928,13 +945,15 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         launch_frame_reset(iter_total.p, reinterpret_cast<uint32_t*>(counters.p),
                            (C + 1) * sizeof(FrameCounters) / 4, redo.p, n, mode == 1 ? 1 : 0, stream);
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
-        // the B-spline table (HBM-write-bound) on a second stream, beside the
-        // seeding (FFTCC or the seeder read only the f32 images); joined
-        // before IC-GN
-        FI_HIP(hipEventRecord(fork_ev, stream));
-        FI_HIP(hipStreamWaitEvent(aux_stream, fork_ev, 0));
-        launch_bspline(tview, coef.p, aux_stream);
-        FI_HIP(hipEventRecord(join_ev, aux_stream));
+        // the B-spline table, when a launch samples it (HBM-write-bound), on a
+        // second stream beside the seeding (FFTCC or the seeder read only the
+        // f32 images); joined before IC-GN
+        if (table) {
+            FI_HIP(hipEventRecord(fork_ev, stream));
+            FI_HIP(hipStreamWaitEvent(aux_stream, fork_ev, 0));
+            launch_bspline(tview, coef.p, aux_stream);
+            FI_HIP(hipEventRecord(join_ev, aux_stream));
+        }
         launch_to_float(tview, tar_f.p, stream);
         if (timed) FI_HIP(hipEventRecord(ev[1], stream));
         if (mode == 0) {
@@ -951,7 +970,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         } else {
             launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);
         }
-        FI_HIP(hipStreamWaitEvent(stream, join_ev, 0));
+        if (table) FI_HIP(hipStreamWaitEvent(stream, join_ev, 0));
         if (timed) FI_HIP(hipEventRecord(ev[2], stream));
         icgn_all(cur.p);
         if (timed) FI_HIP(hipEventRecord(ev[3], stream));
@@ -975,7 +994,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     if (use_graph) {
         const uint64_t key[8] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
                                   (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tview.data,
-                                  (uint64_t)(uintptr_t)coef.p, (uint64_t)(uintptr_t)tar_f.p,
+                                  (uint64_t)(uintptr_t)(table ? coef.p : nullptr), (uint64_t)(uintptr_t)tar_f.p,
                                   (uint64_t)(uintptr_t)seeder };
         hipGraphExec_t exec = nullptr;
         for (auto& gr : graphs)
@@ -1093,7 +1112,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         st.bspline_ms = st.fftcc_ms = st.icgn_ms = NAN;
         st.seed_ms = mode == 0 ? NAN : 0.0;
     } else {
-        FI_HIP(hipEventElapsedTime(&ms, ev[0], join_ev));  // table on its stream (overlapping the seeding)
+        ms = 0.f;
+        if (table) FI_HIP(hipEventElapsedTime(&ms, ev[0], join_ev));  // on its stream, beside the seeding
         st.bspline_ms = ms;
         FI_HIP(hipEventElapsedTime(&ms, ev[1], ev[2]));
         if (mode == 0)
@@ -1120,7 +1140,7 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
     DeviceGuard g(device);
     if (!prepared) prepare();
     if (!tar_ready) fail(FI_E_LOGIC, "autotune needs an uploaded target");
-    prep_target();
+    prep_target(true);  // table-mode candidates are timed too
     // the queued target is taken as a frame takes it (a run without a new
     // upload processes it again)
     const int ts = take_target(true);

@@ -196,7 +196,8 @@ private:
     void icgn_all(fi_poi* base);
     void fftcc_all(fi_poi* base);
     void repair_class(const FrameClass& c, int m);
-    void prep_target();
+    void prep_target(bool table);
+    bool table_needed() const;
     fi::FrameCounters read_counters(fi::FrameCounters* per_class);
     void zero_counters();
     size_t nclass() const { return classes.size(); }

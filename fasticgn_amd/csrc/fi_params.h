@@ -75,7 +75,8 @@ struct IcgnParams {
     const float* ref;         // reference, f32 row-major
     const float* gx;
     const float* gy;
-    const float* coef;        // target B-spline table (bspline_table_floats)
+    const float* coef;        // target B-spline table (bspline_table_floats); nullptr: an
+                              // iteration that outgrows the LDS tile samples tar instead
     const float* tar;         // target, f32 row-major (nullptr: table sampling only)
     int w, h;
     fi_poi* pois;

@@ -125,12 +125,19 @@ __device__ __forceinline__ float bspline_sample(__amdgpu_buffer_rsrc_t r, uint32
 // BC[2][m] d + BC[3][m]: the same polynomial, evaluated from 16 raw pixels
 // instead of 16 stored coefficients.
 // ---------------------------------------------------------------------------
+// The four cubics, factored: with t = 1 - d and P = d t (BC's columns are
+// mirror images, w3(d) = w0(t), w2(d) = w1(t), and every weight but the
+// interpolating part of w1, w2 vanishes at d = 0 and d = 1)
+//     w0 = P (24 d - 33) / 56           w1 = t + P (318 - 384 d) / 336
+//     w3 = -P (24 d + 9) / 56           w2 = d + P (384 d - 66) / 336
+// which equal BC's columns as polynomials: ten operations instead of twelve.
 __device__ __forceinline__ void spline_weights(float d, float& w0, float& w1, float& w2, float& w3)
 {
-    w0 = __builtin_fmaf(__builtin_fmaf(bc(0, 0), d, bc(1, 0)), d, bc(2, 0)) * d;
-    w1 = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(bc(0, 1), d, bc(1, 1)), d, bc(2, 1)), d, bc(3, 1));
-    w2 = __builtin_fmaf(__builtin_fmaf(bc(0, 2), d, bc(1, 2)), d, bc(2, 2)) * d;
-    w3 = __builtin_fmaf(__builtin_fmaf(bc(0, 3), d, bc(1, 3)), d, bc(2, 3)) * d;
+    const float t = 1.f - d, P = d * t;
+    w0 = P * __builtin_fmaf(24.f / 56.f, d, -33.f / 56.f);
+    w3 = P * __builtin_fmaf(-24.f / 56.f, d, -9.f / 56.f);
+    w1 = __builtin_fmaf(P, __builtin_fmaf(-384.f / 336.f, d, 318.f / 336.f), t);
+    w2 = __builtin_fmaf(P, __builtin_fmaf(384.f / 336.f, d, -66.f / 336.f), d);
 }
 
 __device__ __forceinline__ float row4(const float* q, float w0, float w1, float w2, float w3)
@@ -162,6 +169,30 @@ __device__ __forceinline__ float tile_sample(const float* tile, int cbase, float
     const float r0 = row4(q0, x0, x1, x2, x3), r1 = row4(q1, x0, x1, x2, x3);
     const float r2 = row4(q2, x0, x1, x2, x3), r3 = row4(q3, x0, x1, x2, x3);
     return __builtin_fmaf(y3, r3, __builtin_fmaf(y2, r2, __builtin_fmaf(y1, r1, y0 * r0)));
+}
+
+// Sample at (gx, gy) from the f32 target in global memory (rows w floats
+// apart) with the tile path's weights: an iteration whose neighbourhoods
+// outgrow the LDS tile, when no coefficient table was built.  The caller's
+// range test (1 <= gx < w-2, 1 <= gy < h-2) keeps the 4x4 neighbourhood in the
+// image; a row's four pixels are one unaligned 16-byte load.
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+__device__ __forceinline__ float target_sample(const float* tar, uint32_t w, float gx, float gy)
+{
+    const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
+    const float* q0 = tar + ((size_t)((uint32_t)(int)gy - 1u) * w + ((uint32_t)(int)gx - 1u));
+    const f32x4u a = *reinterpret_cast<const f32x4u*>(q0);
+    const f32x4u b = *reinterpret_cast<const f32x4u*>(q0 + w);
+    const f32x4u c = *reinterpret_cast<const f32x4u*>(q0 + 2 * (size_t)w);
+    const f32x4u d = *reinterpret_cast<const f32x4u*>(q0 + 3 * (size_t)w);
+    float x0, x1, x2, x3, y0, y1, y2, y3;
+    spline_weights(dx, x0, x1, x2, x3);
+    spline_weights(dy, y0, y1, y2, y3);
+    auto row = [&](const f32x4u q) {
+        return __builtin_fmaf(x3, q[3], __builtin_fmaf(x2, q[2], __builtin_fmaf(x1, q[1], x0 * q[0])));
+    };
+    return __builtin_fmaf(y3, row(d), __builtin_fmaf(y2, row(c), __builtin_fmaf(y1, row(b), y0 * row(a))));
 }
 
 // Copy image rows ty0.. and columns tx0.. (tw x th pixels) into the tile,
@@ -531,8 +562,10 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         };
         if (use_tile)
             pass1([&](float gx, float gy, int) { return tile_sample(tile, cbase, gx, gy); });
-        else
+        else if (p.coef)
             pass1([&](float gx, float gy, int) { return bspline_sample(table, row_bytes, plane_bytes, gx, gy); });
+        else
+            pass1([&](float gx, float gy, int) { return target_sample(p.tar, (uint32_t)w, gx, gy); });
         // lanes mirroring band 0 drop their t' sums (their r is zero)
         float s1[3] = { st * m_all, stt * m_all, str };
         block_sum<WAVES>(s1, red, wave, lane);

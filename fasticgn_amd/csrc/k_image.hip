@@ -10,16 +10,15 @@
 
 namespace fi {
 
+// One thread per pixel over 64 x 4 pixel tiles (no per-pixel division).
 template <typename T>
 __global__ __launch_bounds__(256) void k_to_float(const T* __restrict__ raw, size_t pitch, int w,
                                                   int h, float scale, float* __restrict__ out)
 {
-    const size_t total = (size_t)w * h;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const size_t y = i / w, x = i - y * w;
-        out[i] = load_px<T>(raw, y * pitch + x, scale);
-    }
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= w || y >= h) return;
+    out[(size_t)y * w + x] = load_px<T>(raw, (size_t)y * pitch + x, scale);
 }
 
 // Gradient2D4::getGradientX/Y, src/oc_gradient.cpp:37-79: taps applied as
@@ -106,8 +105,7 @@ static dim3 tile_grid(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
 
 void launch_to_float(const DevImage& img, float* out, hipStream_t s)
 {
-    const size_t total = (size_t)img.width * img.height;
-    const int blocks = (int)std::min<size_t>((total + 255) / 256, 65536);
+    const dim3 blocks = tile_grid(img.width, img.height);
     switch (img.pixel) {
     case kU8:
         hipLaunchKernelGGL(k_to_float<uint8_t>, dim3(blocks), dim3(256), 0, s,

@@ -81,7 +81,8 @@ typedef struct fi_frame_stats {
     int64_t iteration_sum;        /* sum of executed IC-GN iterations */
     int64_t fftcc_pois;           /* POIs run through FFTCC this frame */
     double device_ms;             /* device time of the frame (HIP events) */
-    double bspline_ms;            /* target B-spline table (on its own stream, beside the seeding) */
+    double bspline_ms;            /* target B-spline table (on its own stream, beside the seeding;
+                                     built only when a launch samples it, else 0) */
     double fftcc_ms;              /* FFTCC seeding (or temporal seed copy) */
     double icgn_ms;               /* first IC-GN pass over the frame */
     int64_t seed_pois;            /* POIs seeded by the learned seeder */
@@ -148,11 +149,12 @@ int fi_icgn_set_self_adaptive(fi_icgn* h, int on);                   /* src/oc_d
 int fi_icgn_set_shape(fi_icgn* h, int waves, int rows_per_band);
 /* Where the IC-GN kernel reads the target: FI_SAMPLING_AUTO stages each
  * subset's raw neighbourhood in LDS and evaluates the B-spline polynomial from
- * the 16 pixels around a sample (the B-spline table serves iterations whose
- * warp outgrows the staged tile); FI_SAMPLING_TABLE reads every sample's 16
- * coefficients from the table (src/oc_cubic_bspline.cpp:134-181).  The two
- * evaluate the same polynomial and differ only in FP32 rounding.  No reference
- * counterpart. */
+ * the 16 pixels around a sample (iterations whose warp outgrows the staged
+ * tile read the B-spline table here, and the same 16 pixels from the target
+ * in device memory in the frame driver, which builds no table for this mode);
+ * FI_SAMPLING_TABLE reads every sample's 16 coefficients from the table
+ * (src/oc_cubic_bspline.cpp:134-181).  All evaluate the same polynomial and
+ * differ only in FP32 rounding.  No reference counterpart. */
 #define FI_SAMPLING_AUTO 0
 #define FI_SAMPLING_TABLE 1
 int fi_icgn_set_sampling(fi_icgn* h, int mode);
