@@ -739,7 +739,8 @@ void fi_frame::prepare()
     prepared = true;
 }
 
-void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* slot_dev)
+void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* slot_dev,
+                          const unsigned* count_dev)
 {
     FftccParams fp{};
     fp.ref = ref_f.p;
@@ -754,10 +755,12 @@ void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, co
     fp.spectra = c.spectra.p;
     fp.slots = c.slots.p;
     fp.twiddles = c.tw.p;
+    fp.count = count_dev;
     launch_fftcc(fp, stream);
 }
 
-void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* prep_index)
+void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* prep_index,
+                         const unsigned* count_dev)
 {
     if (count <= 0) return;
     const int mode = c.sampling >= 0 ? c.sampling : sampling;
@@ -785,8 +788,9 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
         waves = shape_waves;
         spl = shape_spl;
     }
+    ip.count = count_dev;  // bounded on the device: one block per entry
     const int qb = c.queue_blocks >= 0 ? c.queue_blocks : persistent_blocks;
-    if (qb > 0) {
+    if (qb > 0 && !count_dev) {
         int ci = 0;
         while (classes[ci].get() != &c) ci++;
         ip.queue = queue.p + kQueueWords * ci;
@@ -794,7 +798,7 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
         launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), kQueueWords, stream);
     }
     icgn_launch(ip, waves, spl, stream);
-    launch_sum_iters(ip.iters, count, iter_total.p, stream);
+    launch_sum_iters(ip.iters, count, iter_total.p, stream, count_dev);
 }
 
 // The fewest-waves generic shape whose bands cover every class radius.
@@ -864,17 +868,32 @@ void fi_frame::fftcc_all(fi_poi* base)
 
 // POI-level repair of class c's m candidates (cand + off): FFTCC on the cached
 // spectra, IC-GN on the resident records, merge (cli:2955-3022)
-void fi_frame::repair_class(const FrameClass& c, int m)
+// POI-level repair of candidates [lo, m) of class c: FFTCC from the template
+// on the cached spectra, IC-GN on the resident Hessians, merge (better ZNCC
+// wins), redo flags.  With count_dev (lo = 0) the candidate count is read on
+// the device and m is only the capacity the launches are sized for, so the
+// repair can sit in a captured frame graph.
+void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* count_dev)
 {
-    if (m <= 0) return;
+    const int k = m - lo;
+    if (k <= 0) return;
     int ci = 0;
     while (classes[ci].get() != &c) ci++;
-    launch_gather(tmpl.p + c.off, cand.p + c.off, m, repair.p + c.off, stream);
-    launch_mark(cand.p + c.off, m, redo.p + c.off, stream);
-    fftcc_pass(c, repair.p + c.off, m, cand.p + c.off);
-    icgn_pass(c, repair.p + c.off, m, cand.p + c.off);
-    launch_merge_repair(cur.p + c.off, repair.p + c.off, cand.p + c.off, m, counters.p + ci, stream);
+    const uint32_t* idx = cand.p + c.off + lo;
+    fi_poi* rep = repair.p + c.off + lo;
+    launch_gather(tmpl.p + c.off, idx, k, rep, stream, count_dev);
+    launch_mark(idx, k, redo.p + c.off, stream, count_dev);
+    fftcc_pass(c, rep, k, idx, count_dev);
+    icgn_pass(c, rep, k, idx, count_dev);
+    launch_merge_repair(cur.p + c.off, rep, idx, k, counters.p + ci, stream, count_dev);
     FI_HIP(hipGetLastError());
+}
+
+// Fallback candidates a learned-seed frame repairs inside its segment (per
+// class); more are repaired after it.  Fallbacks run at about 1 % of POIs.
+int fi_frame::repair_cap(const FrameClass& c) const
+{
+    return (int)std::min<size_t>(c.n, std::max<size_t>(1024, c.n / 32));
 }
 
 void fi_frame::zero_counters()
@@ -983,6 +1002,13 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
                 launch_scan(cur.p + c.off, prev.p + c.off, (int)c.n, collect, counters.p + ci, cand.p + c.off,
                             stream);
         }
+        // learned seeds: the FFTCC fallback of the scan's candidates, sized for
+        // repair_cap and bounded by the scan's count on the device (no host
+        // round trip; candidates past the cap are repaired after the segment)
+        if (mode == 0)
+            for (int ci = 0; ci < C; ci++)
+                if (classes[ci]->n > 0)
+                    repair_class(*classes[ci], repair_cap(*classes[ci]), 0, &counters.p[ci].candidates);
         launch_qa(cur.p, cnt, counters.p + C, stream);
         FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
                               stream));
@@ -1035,9 +1061,12 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         st.seed_pois = cnt;
         st.seed_fallback_count = (int64_t)m_all;
         st.fftcc_pois = (int64_t)m_all;
-        if (m_all > 0) {
-            for (int ci = 0; ci < C; ci++) repair_class(*classes[ci], (int)pc[ci].candidates);
-            redo_qa = true;
+        for (int ci = 0; ci < C; ci++) {
+            const int m = (int)pc[ci].candidates, cap = repair_cap(*classes[ci]);
+            if (m > cap) {
+                repair_class(*classes[ci], m, cap);
+                redo_qa = true;
+            }
         }
     } else if (mode == 1) {
         st.fftcc_pois = cnt;

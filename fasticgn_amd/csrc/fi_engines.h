@@ -189,13 +189,16 @@ struct fi_frame {
 
 private:
     void prepare();
-    void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev);
-    void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index);
+    void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev,
+                    const unsigned* count_dev = nullptr);
+    void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index,
+                   const unsigned* count_dev = nullptr);
     void icgn_fused(fi_poi* base);
     bool fused_shape(int& waves, int& spl) const;
     void icgn_all(fi_poi* base);
     void fftcc_all(fi_poi* base);
-    void repair_class(const FrameClass& c, int m);
+    void repair_class(const FrameClass& c, int m, int lo = 0, const unsigned* count_dev = nullptr);
+    int repair_cap(const FrameClass& c) const;
     void prep_target(bool table);
     bool table_needed() const;
     fi::FrameCounters read_counters(fi::FrameCounters* per_class);

@@ -62,6 +62,8 @@ struct FftccParams {
     float* spectra;           // slots * n * (r+1) complex, interleaved
     FftccSlot* slots;
     const float* twiddles;    // 2n floats
+    const unsigned* count;    // device POI count, at most n_pois (nullptr: n_pois); set, the
+                              // blocks map to POIs in launch order
 };
 
 // Claim counters of one persistent-queue IC-GN launch: kQueueParts counters
@@ -93,6 +95,8 @@ struct IcgnParams {
     // persistent work queue (nullptr: one block per entry): 8 zeroed claim
     // counters, and the number of resident blocks to launch
     int* queue;
+    const unsigned* count;    // device entry count, at most n (nullptr: n); set, block b runs
+                              // entry b (one block per entry, no queue)
     int queue_blocks;
 };
 

@@ -279,13 +279,14 @@ __global__ __launch_bounds__(64) void k_fftcc(FftccParams p)
     __shared__ float2 spec_all[PPB][NH];
     __shared__ float rows_all[PPB][N];
     const int lane = threadIdx.x, g = lane / G, j = lane % G;
-    const int i = xcd_remap(blockIdx.x, gridDim.x) * PPB + g;
+    const int i = (p.count ? (int)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x)) * PPB + g;
+    const int n_pois = p.count ? min((int)*p.count, p.n_pois) : p.n_pois;
     float2* spec = spec_all[g];
     float* rows = rows_all[g];
     const float2* tw = reinterpret_cast<const float2*>(p.twiddles);
     const int w = p.w, h = p.h;
 
-    bool act = i < p.n_pois;
+    bool act = i < n_pois;
     float x = 0.f, y = 0.f, u0 = 0.f, v0 = 0.f;
     fi_poi* poi = nullptr;
     if (act) {

@@ -675,8 +675,13 @@ void k_icgn(IcgnParams p)
     __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
     __shared__ int flags[WAVES];
     extern __shared__ float tile[];
-    const int b = xcd_remap(blockIdx.x, gridDim.x);
-    if (b < p.n) icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+    if (p.count) {
+        const int b = (int)blockIdx.x;
+        if (b < min((int)*p.count, p.n)) icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+    } else {
+        const int b = xcd_remap(blockIdx.x, gridDim.x);
+        if (b < p.n) icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+    }
 }
 
 template <int WAVES, int SPL, bool EXACT>
@@ -743,7 +748,7 @@ template <int WAVES, int SPL, bool EXACT>
 void launch_icgn_t(const IcgnParams& p, hipStream_t s)
 {
     const size_t lds = p.tile_w > 0 ? (size_t)p.tile_stride * p.tile_h * sizeof(float) : 0;
-    if (p.queue)
+    if (p.queue && !p.count)
         hipLaunchKernelGGL((k_icgn_queue<WAVES, SPL, EXACT>), dim3(std::max(1, std::min(p.n, p.queue_blocks))),
                            dim3(64 * WAVES), lds, s, p);
     else

@@ -154,11 +154,18 @@ __global__ __launch_bounds__(kPolicyThreads) void k_seed_scan(const fi_poi* __re
     }
 }
 
+// Entries a launch covers: n, or the device count when one is given (at most
+// n, the capacity the grid was sized for).
+__device__ __forceinline__ int bounded(int n, const unsigned* count)
+{
+    return count ? min((int)*count, n) : n;
+}
+
 __global__ void k_gather(const fi_poi* __restrict__ src, const uint32_t* __restrict__ idx, int n,
-                         fi_poi* __restrict__ dst)
+                         fi_poi* __restrict__ dst, const unsigned* count)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) dst[j] = src[idx[j]];
+    if (j < bounded(n, count)) dst[j] = src[idx[j]];
 }
 
 __global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restrict__ scratch, int n)
@@ -169,10 +176,11 @@ __global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restric
 }
 
 __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restrict__ rep,
-                               const uint32_t* __restrict__ idx, int n, FrameCounters* counters)
+                               const uint32_t* __restrict__ idx, int n, FrameCounters* counters,
+                               const unsigned* count)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+    if (j >= bounded(n, count)) return;
     const uint32_t i = idx[j];
     if (should_replace(cur[i], rep[j])) {
         const bool recovered = !is_bad(rep[j]);
@@ -213,8 +221,9 @@ __global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict_
 
 // Sum of per-entry executed iteration counts into *total.
 __global__ __launch_bounds__(kPolicyThreads) void k_sum_iters(const uint32_t* __restrict__ iters, int n,
-                                                               unsigned long long* total)
+                                                               unsigned long long* total, const unsigned* count)
 {
+    n = bounded(n, count);
     unsigned long long acc = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += iters[i];
     unsigned v[1] = { (unsigned)acc }, mx[1] = { 0u };
@@ -234,10 +243,10 @@ __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n)
 }
 
 // flags[idx[j]] = 1 for j < n
-__global__ void k_mark(const uint32_t* __restrict__ idx, int n, uint8_t* __restrict__ flags)
+__global__ void k_mark(const uint32_t* __restrict__ idx, int n, uint8_t* __restrict__ flags, const unsigned* count)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < n) flags[idx[j]] = 1u;
+    if (j < bounded(n, count)) flags[idx[j]] = 1u;
 }
 
 __global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v)
@@ -265,9 +274,9 @@ void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_wor
                        dim3(256), 0, s, total, words, n_words, flags, n, v);
 }
 
-void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s)
+void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s, const unsigned* count)
 {
-    if (n > 0) hipLaunchKernelGGL(k_mark, grid_for(n), dim3(256), 0, s, idx, n, flags);
+    if (n > 0) hipLaunchKernelGGL(k_mark, grid_for(n), dim3(256), 0, s, idx, n, flags, count);
 }
 
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s)
@@ -306,9 +315,10 @@ void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float thresh
                            candidates);
 }
 
-void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s)
+void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s,
+                   const unsigned* count)
 {
-    if (n > 0) hipLaunchKernelGGL(k_gather, grid_for(n), dim3(256), 0, s, src, idx, n, dst);
+    if (n > 0) hipLaunchKernelGGL(k_gather, grid_for(n), dim3(256), 0, s, src, idx, n, dst, count);
 }
 
 void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s)
@@ -317,11 +327,11 @@ void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int 
 }
 
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
-                         FrameCounters* counters, hipStream_t s)
+                         FrameCounters* counters, hipStream_t s, const unsigned* count)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_merge_repair, grid_for(n), dim3(256), 0, s, cur, repaired, idx, n,
-                           counters);
+                           counters, count);
 }
 
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s)
@@ -331,11 +341,12 @@ void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s)
                            dim3(kPolicyThreads), 0, s, cur, n, counters);
 }
 
-void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s)
+void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s,
+                      const unsigned* count)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_sum_iters, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks)),
-                           dim3(kPolicyThreads), 0, s, iters, n, total);
+                           dim3(kPolicyThreads), 0, s, iters, n, total, count);
 }
 
 }  // namespace fi
