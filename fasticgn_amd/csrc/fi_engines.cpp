@@ -1002,13 +1002,19 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
                 launch_scan(cur.p + c.off, prev.p + c.off, (int)c.n, collect, counters.p + ci, cand.p + c.off,
                             stream);
         }
-        // learned seeds: the FFTCC fallback of the scan's candidates, sized for
-        // repair_cap and bounded by the scan's count on the device (no host
-        // round trip; candidates past the cap are repaired after the segment)
-        if (mode == 0)
+        // The FFTCC repair of the scan's candidates, sized for repair_cap and
+        // bounded by the scan's count on the device (no host round trip;
+        // candidates past the cap are repaired after the segment): always for
+        // learned seeds; speculatively for temporal seeds, whose whole-frame
+        // fallback (decided on the host) then starts again from the
+        // unrepaired results kept in `save`.
+        if (mode == 0 || (mode == 2 && collect)) {
+            if (mode == 2)
+                FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             for (int ci = 0; ci < C; ci++)
                 if (classes[ci]->n > 0)
                     repair_class(*classes[ci], repair_cap(*classes[ci]), 0, &counters.p[ci].candidates);
+        }
         launch_qa(cur.p, cnt, counters.p + C, stream);
         FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
                               stream));
@@ -1081,9 +1087,12 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         st.pre_repair_bad_ratio = ratio;
         const float threshold = baseline_bad + kBadRatioMargin;
         if (ratio > threshold) {
-            // whole-frame fallback with per-POI merge (:3115-3152)
+            // whole-frame fallback with per-POI merge (:3115-3152), against the
+            // results before any POI-level repair (the segment kept them in
+            // `save` when it repaired)
             st.fallback_used = 1;
-            FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+            if (!collect)
+                FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             fftcc_all(cur.p);
             icgn_all(cur.p);
@@ -1093,16 +1102,23 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             st.fftcc_pois = cnt;
             redo_qa = true;
         } else if (collect && m_all > 0) {
-            // POI-level repair on cached spectra and resident Hessians (:2955-3022)
-            for (int ci = 0; ci < C; ci++) repair_class(*classes[ci], (int)pc[ci].candidates);
-            std::vector<FrameCounters> rc(C + 1);
-            read_counters(rc.data());
+            // POI-level repair on cached spectra and resident Hessians
+            // (:2955-3022): the segment repaired up to repair_cap candidates
+            // per class, the rest here
+            std::vector<FrameCounters> rc(pc);
+            for (int ci = 0; ci < C; ci++) {
+                const int m = (int)pc[ci].candidates, cap = repair_cap(*classes[ci]);
+                if (m > cap) {
+                    repair_class(*classes[ci], m, cap);
+                    redo_qa = true;
+                }
+            }
+            if (redo_qa) read_counters(rc.data());
             unsigned long long rec = 0;
             for (int ci = 0; ci < C; ci++) rec += rc[ci].recovered;
             st.poi_fallback_count = (int32_t)m_all;
             st.poi_fallback_recovered = (int32_t)rec;
             st.fftcc_pois = (int64_t)m_all;
-            redo_qa = true;
         }
     }
     if (redo_qa) {
