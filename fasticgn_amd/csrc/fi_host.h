@@ -97,15 +97,12 @@ struct DeviceImage {
     void upload(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12,
                 hipStream_t s);
     // The two halves of upload: pack the rows into the pinned staging buffer
-    // (host), then one DMA on stream s that is not waited for.
+    // (host), then one DMA on stream s that is not waited for.  stage rejects a
+    // right-aligned 12-bit image with a value above 4095 (src/oc_image.cpp:172-178).
     void stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12);
     void dma(hipStream_t s);
     size_t staged_bytes = 0;
 };
-
-// Rejects a right-aligned 12-bit image with a value above 4095
-// (src/oc_image.cpp:172-178).
-void check_depth12(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12);
 
 void check_image_args(const void* px, int width, int height, size_t stride_bytes, int pixel,
                       int depth12);

@@ -36,18 +36,6 @@ void check_image_args(const void* px, int width, int height, size_t stride_bytes
                                         " is shorter than a row");
 }
 
-void check_depth12(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12)
-{
-    if (depth12 != FI_DEPTH12_RIGHT || pixel != FI_PIXEL_U16) return;
-    uint16_t mx = 0;
-    for (int y = 0; y < height; y++) {
-        const uint16_t* row = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes);
-        for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
-    }
-    // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
-    if (mx > 4095) fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
-}
-
 void DeviceImage::stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12)
 {
     check_image_args(px, width, height, stride_bytes, pixel, depth12);
@@ -56,9 +44,20 @@ void DeviceImage::stage(const void* px, int width, int height, size_t stride_byt
     // rows packed into pinned memory (the copy out of pageable memory is a
     // plain memcpy; the DMA then runs at the link rate)
     this->stage_buf.reserve(total);
-    for (int y = 0; y < height; y++)
-        std::memcpy(stage_buf.p + (size_t)y * row_bytes, static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes,
-                    row_bytes);
+    // a right-aligned 12-bit image is range checked on the staged rows (each
+    // still in cache after its copy): one pass over the caller's pixels
+    const bool check12 = depth12 == FI_DEPTH12_RIGHT && pixel == FI_PIXEL_U16;
+    uint16_t mx = 0;
+    for (int y = 0; y < height; y++) {
+        uint8_t* dst = stage_buf.p + (size_t)y * row_bytes;
+        std::memcpy(dst, static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes, row_bytes);
+        if (check12) {
+            const uint16_t* row = reinterpret_cast<const uint16_t*>(dst);
+            for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
+        }
+    }
+    // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
+    if (mx > 4095) fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
     raw.reserve(total);
     staged_bytes = total;
     view.data = raw.p;
@@ -78,8 +77,6 @@ void DeviceImage::dma(hipStream_t s)
 void DeviceImage::upload(const void* px, int width, int height, size_t stride_bytes, int pixel,
                          int depth12, hipStream_t s)
 {
-    check_image_args(px, width, height, stride_bytes, pixel, depth12);
-    check_depth12(px, width, height, stride_bytes, pixel, depth12);
     stage(px, width, height, stride_bytes, pixel, depth12);
     dma(s);
     FI_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next upload

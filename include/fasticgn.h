@@ -186,7 +186,10 @@ int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n);
  * copies it to pinned memory and the device while a running frame computes
  * (two slots; a third queued target is FI_E_LOGIC).  The pixels must stay
  * valid until the next fi_frame_run / fi_frame_autotune returns.  A run with
- * no new target processes the last one again. */
+ * no new target processes the last one again.  The 12-bit range of a
+ * FI_DEPTH12_RIGHT target is checked as the staging thread copies it: the
+ * fi_frame_run (or autotune) that takes a target above 4095 fails with
+ * FI_E_INVALID_ARGUMENT (src/oc_image.cpp:172-178) and the driver stays usable. */
 int fi_frame_upload_target(fi_frame* h, const void* px, int width, int height,
                            size_t stride_bytes, int pixel, int depth12);
 /* Process the uploaded target with the given policy; results stay on the

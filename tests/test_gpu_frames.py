@@ -217,7 +217,9 @@ def test_frame_option_errors(seq, targets):
 def test_streamed_uploads_match_serial_frames(seq, targets):
     """Target k+1 queued while frame k runs (two slots, background copy):
     byte-identical to uploading and running one frame at a time; a third
-    queued target is refused; a 12-bit image above 4095 is refused at upload."""
+    queued target is refused; a 12-bit image above 4095 is refused by the run
+    that takes it (the staging thread checks it as it copies), and the driver
+    runs the next target."""
     with mixed_driver(seq) as d:
         serial = run_sequence(d, targets)
     with mixed_driver(seq, graph=True) as d:
@@ -235,8 +237,12 @@ def test_streamed_uploads_match_serial_frames(seq, targets):
         bad = targets[STEPS[0]].copy()
         bad[3, 3] = 5000
         d.run()
+        d.run()
+        d.upload_target(img(bad))
         with pytest.raises(ValueError):
-            d.upload_target(img(bad))
+            d.run()
+        d.upload_target(img(targets[STEPS[0]]))
+        d.run()
     for f, ((a, _), (b, _)) in enumerate(zip(serial, streamed)):
         assert same_bytes(a, b), f
 
