@@ -239,6 +239,15 @@ int fi_icgn_compute(fi_icgn* h, fi_poi* pois, size_t n)
     return guarded([&] { need(h); h->compute(pois, n); });
 }
 
+int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* center_offsets)
+{
+    return guarded([&] {
+        need(h);
+        if (n > 0 && !center_offsets) fail(FI_E_INVALID_ARGUMENT, "center offsets are null");
+        h->compute(pois, n, center_offsets);
+    });
+}
+
 // ---- frame driver ----
 int fi_frame_create(int radius, float conv, float stop, int device, fi_frame** out)
 {

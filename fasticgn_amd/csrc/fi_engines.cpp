@@ -338,7 +338,10 @@ void fi_icgn::prepare_tar()
     coef_ready = true;
 }
 
-void fi_icgn::compute(fi_poi* pois, size_t n)
+// compute(vector<POI2D>&) (src/oc_icgn.cpp:389-397); with offsets,
+// compute(queue, center_offset_queue) (:399-625), whose Hessians depend on the
+// offsets and so are never taken from the prepared-layout cache.
+void fi_icgn::compute(fi_poi* pois, size_t n, const float* offsets)
 {
     if (!grads_ready || !coef_ready)
         fail(FI_E_LOGIC, "ICGN2D1 compute requires prepareRef and prepareTar after the images are set");
@@ -369,7 +372,7 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
         key[4 * i + 2] = (float)gx_r;
         key[4 * i + 3] = (float)gy_r;
     }
-    const bool prep_hit = prep_version == ref_version && prep_key.size() == key.size() &&
+    const bool prep_hit = !offsets && prep_version == ref_version && prep_key.size() == key.size() &&
                           std::memcmp(prep_key.data(), key.data(), key.size() * sizeof(float)) == 0;
 
     h_pois.reserve(n);
@@ -390,6 +393,11 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
     p.prep = prep.p;
     p.conv = conv;
     p.stop = stop;
+    if (offsets) {
+        d_offset.reserve(2 * n);
+        FI_HIP(hipMemcpyAsync(d_offset.p, offsets, 2 * n * sizeof(float), hipMemcpyHostToDevice, stream));
+        p.offset = d_offset.p;
+    }
     if (!self_adaptive) {
         p.index = nullptr;
         p.n = (int)n;
@@ -433,8 +441,12 @@ void fi_icgn::compute(fi_poi* pois, size_t n)
     FI_HIP(hipMemcpyAsync(h_pois.p, d_pois.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
     FI_HIP(hipStreamSynchronize(stream));
     std::memcpy(pois, h_pois.p, n * sizeof(fi_poi));
-    prep_key.swap(key);
-    prep_version = ref_version;
+    if (offsets) {
+        prep_key.clear();  // the records hold this call's offsets
+    } else {
+        prep_key.swap(key);
+        prep_version = ref_version;
+    }
 }
 
 // ===========================================================================

@@ -73,6 +73,7 @@ struct fi_icgn {
     fi::DevArray<fi::PoiPrep> prep;
     fi::DevArray<fi_poi> d_pois;
     fi::DevArray<uint32_t> d_index;
+    fi::DevArray<float> d_offset;  // (x, y) subset centre offsets of the last compute
     fi::PinnedArray<fi_poi> h_pois;
 
     fi_icgn(int rx, int ry, float conv, float stop, int dev);
@@ -81,7 +82,8 @@ struct fi_icgn {
     void set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12);
     void prepare_ref();
     void prepare_tar();
-    void compute(fi_poi* pois, size_t n);
+    // offsets: n (x, y) pairs (compute(POI2D*, Point2D&)), or nullptr
+    void compute(fi_poi* pois, size_t n, const float* offsets = nullptr);
 };
 
 // One radius class of the frame driver's template: POIs [off, off + n) of the

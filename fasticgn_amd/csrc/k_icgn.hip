@@ -124,6 +124,8 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
     }
     PoiPrep* out = p.prep + i;
     const int w = p.w, h = p.h;
+    // centre offset (src/oc_icgn.cpp:446-447): steepest-descent local coordinates
+    const float ox = p.offset ? p.offset[2 * (size_t)i] : 0.f, oy = p.offset ? p.offset[2 * (size_t)i + 1] : 0.f;
     if (!ref_gate_ok(x, y, rx, ry, w, h)) {
         if (lane == 0) { out->status = 0; out->rx = rx; out->ry = ry; }
         return;
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
         sample_coords(g, s, c, r);
         const size_t gi = (size_t)(py + r - ry) * w + (px + c - rx);
         const float gxv = p.gx[gi], gyv = p.gy[gi];
-        const float xl = (float)(c - rx), yl = (float)(r - ry);
+        const float xl = (float)(c - rx) - ox, yl = (float)(r - ry) - oy;
         const float sd[6] = { gxv, gxv * xl, gxv * yl, gyv, gyv * xl, gyv * yl };
         int q = 0;
 #pragma unroll

@@ -392,6 +392,14 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     const int i = p.index ? (int)p.index[b] : b;
     fi_poi* poi = p.pois + i;
     const float x = poi->x, y = poi->y;
+    // compute(POI2D*, Point2D& center_offset), src/oc_icgn.cpp:399-615: the
+    // warp acts about (x + ox, y + oy) on local coordinates minus the offset
+    float ox = 0.f, oy = 0.f;
+    if (p.offset) {
+        ox = p.offset[2 * (size_t)i];
+        oy = p.offset[2 * (size_t)i + 1];
+    }
+    const float xc = x + ox, yc = y + oy;
     const float u_in = poi->deformation[0], ux_in = poi->deformation[1], uy_in = poi->deformation[2];
     const float v_in = poi->deformation[6], vx_in = poi->deformation[7], vy_in = poi->deformation[8];
     const float z_in = poi->result[2];
@@ -461,7 +469,9 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     {
         const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
         const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
-        const float cx = x + W02, cy = y + W12;
+        // the warped image of the local box's centre (-ox, -oy)
+        const float cx = xc + __builtin_fmaf(-W00, ox, __builtin_fmaf(-W01, oy, W02));
+        const float cy = yc + __builtin_fmaf(-W10, ox, __builtin_fmaf(-W11, oy, W12));
         if (tile_w > 0 && cx - ex >= 1.f + kEdgeMargin && cx + ex < wm2 - kEdgeMargin &&
             cy - ey >= 1.f + kEdgeMargin && cy + ey < hm2 - kEdgeMargin) {
             const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
@@ -478,7 +488,7 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     }
     const float m_all = bd.active ? 1.f : 0.f;
     const float m_dup = bd.last ? 0.f : 1.f;
-    const float xl = (float)(bd.col - rx), yl0 = (float)(bd.row0 - ry);
+    const float xl = (float)(bd.col - rx) - ox, yl0 = (float)(bd.row0 - ry) - oy;
     const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)w * (uint32_t)h * 64u);
     const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
 
@@ -488,15 +498,16 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         it++;
         // sample k of the lane sits at (xb + k W01, yb + k W11): the POI
         // centre, the warp of the lane's column and first row, then its steps
-        const float xb = x + __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
-        const float yb = y + __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
+        const float xb = xc + __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
+        const float yb = yc + __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
         // The samples are an affine image of the grid: they lie within the
         // warped bounding box x + W02 +- (|W00| rx + |W01| ry), so all are in
         // the interpolatable range (:39-43) when the box is, by a margin.  A NaN
         // anywhere in the warp fails the test.
         const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
         const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
-        const float cx = x + W02, cy = y + W12;
+        const float cx = xc + __builtin_fmaf(-W00, ox, __builtin_fmaf(-W01, oy, W02));
+        const float cy = yc + __builtin_fmaf(-W10, ox, __builtin_fmaf(-W11, oy, W12));
         const bool box_in = (cx - ex >= 1.f + kEdgeMargin) & (cx + ex < wm2 - kEdgeMargin) &
                             (cy - ey >= 1.f + kEdgeMargin) & (cy + ey < hm2 - kEdgeMargin);
         if (!box_in) {

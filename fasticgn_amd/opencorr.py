@@ -224,9 +224,18 @@ class ICGN2D1(_Handle):
     def prepare(self) -> None:
         nat.call("fi_icgn_prepare", self._h)
 
-    def compute(self, queue: np.ndarray) -> None:
+    def compute(self, queue: np.ndarray, center_offsets: np.ndarray | None = None) -> None:
+        """compute(queue), or with center_offsets (n x 2) compute(queue,
+        center_offset_queue) (src/oc_icgn.cpp:399-625): POI i is matched with
+        its warp centred at (x + ox, y + oy)."""
         q = _check_queue(queue)
-        nat.call("fi_icgn_compute", self._h, nat.ptr(q), len(q))
+        if center_offsets is None:
+            nat.call("fi_icgn_compute", self._h, nat.ptr(q), len(q))
+            return
+        off = np.ascontiguousarray(center_offsets, dtype=np.float32)
+        if off.shape != (len(q), 2):
+            raise ValueError("center_offsets must be one (x, y) pair per POI")
+        nat.call("fi_icgn_compute_offset", self._h, nat.ptr(q), len(q), nat.ptr(off))
 
 
 def format_raw(queue: np.ndarray) -> bytes:

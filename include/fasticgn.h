@@ -167,6 +167,13 @@ int fi_icgn_prepare_tar(fi_icgn* h);   /* BicubicBspline table of the target (:1
 int fi_icgn_prepare(fi_icgn* h);       /* both (:155-159) */
 /* compute(vector<POI2D>&) (:389-397, per POI :168-387). */
 int fi_icgn_compute(fi_icgn* h, fi_poi* pois, size_t n);
+/* compute(POI2D*, Point2D& center_offset) / compute(vector<POI2D>&,
+ * vector<Point2D>& center_offset_queue) (src/oc_icgn.cpp:399-625; Zhou et al.,
+ * Opt. Lasers Eng. 55 (2014) 5-11): center_offsets holds n (x, y) pairs; POI i
+ * is matched with its warp centred at (x + ox, y + oy), so u, v are the
+ * displacement of that point.  A non-finite offset gives the reference's
+ * per-POI statuses (-5 for NaN, -3 out of range). */
+int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* center_offsets);
 
 /* ---- multi-frame driver --------------------------------------------------
  * The fixed-reference sequence of opencorr_cli_cuda_mvp.cpp:3834-4509 with the

@@ -182,6 +182,23 @@ public:
     void prepare() { detail::check(fi_icgn_prepare(h_)); }
     void compute(POI2D* poi) { detail::check(fi_icgn_compute(h_, reinterpret_cast<fi_poi*>(poi), 1)); }
     void compute(std::vector<POI2D>& queue) { detail::check(fi_icgn_compute(h_, detail::raw(queue), queue.size())); }
+    // center-offset subsets (src/oc_icgn.cpp:399-625)
+    void compute(POI2D* poi, Point2D& center_offset)
+    {
+        const float off[2] = { center_offset.x, center_offset.y };
+        detail::check(fi_icgn_compute_offset(h_, reinterpret_cast<fi_poi*>(poi), 1, off));
+    }
+    void compute(std::vector<POI2D>& queue, std::vector<Point2D>& center_offset_queue)
+    {
+        if (center_offset_queue.size() != queue.size())
+            throw std::invalid_argument("ICGN2D1::compute requires one center offset per POI");
+        std::vector<float> off(2 * queue.size());
+        for (size_t i = 0; i < queue.size(); i++) {
+            off[2 * i] = center_offset_queue[i].x;
+            off[2 * i + 1] = center_offset_queue[i].y;
+        }
+        detail::check(fi_icgn_compute_offset(h_, detail::raw(queue), queue.size(), off.data()));
+    }
 
     int subset_radius_x, subset_radius_y;
 
@@ -249,6 +266,9 @@ public:
     {
         opencorr::detail::check(fi_frame_set_seeder(h_, s ? s->handle() : nullptr, zncc_threshold));
     }
+    // seeder passes per frame (1..4): each re-crops the target at the
+    // previous pass's rounded u, v
+    void setSeedPasses(int passes) { opencorr::detail::check(fi_frame_set_seed_passes(h_, passes)); }
     // Queues a target; its pixels must stay valid until the next run() returns.
     void uploadTarget(const opencorr::Image2D& tar)
     {

@@ -79,6 +79,16 @@ void or_icgn_compute_f32(const float* ref, int w, int h, const float* gx, const 
 void or_icgn_compute_f64(const float* ref, int w, int h, const double* gx, const double* gy,
                          const double* coef, int rx, int ry, double conv, double stop,
                          int self_adaptive, or_poi* pois, size_t n, int threads);
+/* compute(POI2D*, Point2D& center_offset) / compute(queue, center_offset_queue)
+ * (src/oc_icgn.cpp:399-625): offsets holds n (x, y) pairs. */
+void or_icgn_compute_offset_f32(const float* ref, int w, int h, const float* gx, const float* gy,
+                                const float* coef, int rx, int ry, float conv, float stop,
+                                int self_adaptive, or_poi* pois, size_t n, const float* offsets,
+                                int threads);
+void or_icgn_compute_offset_f64(const float* ref, int w, int h, const double* gx, const double* gy,
+                                const double* coef, int rx, int ry, double conv, double stop,
+                                int self_adaptive, or_poi* pois, size_t n, const float* offsets,
+                                int threads);
 
 /* ---- linear algebra pieces, exported for the oracle's own tests ---------- */
 /* 6x6 inverse by partial-pivot LU (Eigen PartialPivLU semantics), row-major. */

@@ -57,6 +57,8 @@ def lib() -> ctypes.CDLL:
             "or_fftcc_compute_prepared": ([_P, _I, _I, _I, _P, _P, _P, _P, _S, _P, _I], None),
             "or_icgn_compute_f32": ([_P, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I, _P, _S, _I], None),
             "or_icgn_compute_f64": ([_P, _I, _I, _P, _P, _P, _I, _I, _D, _D, _I, _P, _S, _I], None),
+            "or_icgn_compute_offset_f32": ([_P, _I, _I, _P, _P, _P, _I, _I, _F, _F, _I, _P, _S, _P, _I], None),
+            "or_icgn_compute_offset_f64": ([_P, _I, _I, _P, _P, _P, _I, _I, _D, _D, _I, _P, _S, _P, _I], None),
             "or_inverse6_f32": ([_P, _P], None),
             "or_inverse6_f64": ([_P, _P], None),
         }
@@ -169,8 +171,9 @@ class PreparedRef:
 
 def icgn(ref: np.ndarray, tar: np.ndarray, radius, queue: np.ndarray, conv: float = 0.001,
          stop: float = 10.0, f64: bool = False, self_adaptive: bool = False,
-         prepared=None) -> None:
-    """ICGN2D1 prepare() + compute(queue), in place.  radius: int or (rx, ry)."""
+         prepared=None, center_offsets: np.ndarray | None = None) -> None:
+    """ICGN2D1 prepare() + compute(queue), in place.  radius: int or (rx, ry).
+    center_offsets (n x 2, optional): compute(queue, center_offset_queue)."""
     rx, ry = (radius, radius) if np.isscalar(radius) else radius
     ref = np.ascontiguousarray(ref, dtype=np.float32)
     if prepared is None:
@@ -180,9 +183,16 @@ def icgn(ref: np.ndarray, tar: np.ndarray, radius, queue: np.ndarray, conv: floa
         gx, gy, coef = prepared
     h, w = ref.shape
     assert queue.dtype == POI_DTYPE and queue.flags.c_contiguous
-    fn = lib().or_icgn_compute_f64 if f64 else lib().or_icgn_compute_f32
-    fn(_p(ref), w, h, _p(gx), _p(gy), _p(coef), int(rx), int(ry), conv, stop,
-       int(self_adaptive), _p(queue), len(queue), threads())
+    if center_offsets is None:
+        fn = lib().or_icgn_compute_f64 if f64 else lib().or_icgn_compute_f32
+        fn(_p(ref), w, h, _p(gx), _p(gy), _p(coef), int(rx), int(ry), conv, stop,
+           int(self_adaptive), _p(queue), len(queue), threads())
+    else:
+        off = np.ascontiguousarray(center_offsets, dtype=np.float32)
+        assert off.shape == (len(queue), 2)
+        fn = lib().or_icgn_compute_offset_f64 if f64 else lib().or_icgn_compute_offset_f32
+        fn(_p(ref), w, h, _p(gx), _p(gy), _p(coef), int(rx), int(ry), conv, stop,
+           int(self_adaptive), _p(queue), len(queue), _p(off), threads())
 
 
 def inverse6(a: np.ndarray, f64: bool = False) -> np.ndarray:

@@ -82,6 +82,19 @@ int main()
     close = 0;
     for (const auto& p : res) close += std::fabs(p.deformation.u - 2.f) < 0.01f && std::fabs(p.deformation.v + 1.f) < 0.01f;
     check("frame-driver-learned-seed", close == (int)layout.size() && s2.seed_pois == (int64_t)layout.size());
+    drv.setSeedPasses(2);
+    drv.resetSequence();
+    const fi_frame_stats s3 = drv.process(T, res, FI_POLICY_LEARNED_SEED);
+    close = 0;
+    for (const auto& p : res) close += std::fabs(p.deformation.u - 2.f) < 0.01f && std::fabs(p.deformation.v + 1.f) < 0.01f;
+    check("frame-driver-two-seed-passes", close == (int)layout.size() && s3.seed_pois == (int64_t)layout.size());
+    bool refused = false;
+    try {
+        drv.setSeedPasses(5);
+    } catch (const std::invalid_argument&) {
+        refused = true;
+    }
+    check("seed-passes-range", refused);
     drv.setSeeder(nullptr);
     return failures ? 1 : 0;
 }

@@ -126,6 +126,47 @@ def run_icgn_gpu(ref, tar, radius, q, conv=0.001, stop=10.0, self_adaptive=False
     return g
 
 
+def test_icgn_center_offset_parity(oracle):
+    """compute(queue, center_offset_queue) (src/oc_icgn.cpp:399-625) on the
+    device against the CPU oracle's, offsets U[-3, 3] px on an affine field with
+    gradients up to 0.05: the parity gate, u, v equal to the displacement at the
+    offset centre, and zero offsets byte-identical to compute(queue)."""
+    sc = synth.affine_scene("offset", 384, 384, 3400, 11, t=(1.3, -0.8), grad=(0.04, 0.02, -0.03, 0.05),
+                            radius=15, grid=24)
+    q = sc.pois.copy()
+    u, v, ux, uy, vx, vy = sc.truth(q["x"], q["y"])
+    d = q["deformation"]
+    d[:, 0], d[:, 6] = np.round(u), np.round(v)
+    d[:, 1], d[:, 2], d[:, 7], d[:, 8] = ux, uy, vx, vy
+    off = np.random.default_rng(5).uniform(-3, 3, (len(q), 2)).astype(np.float32)
+    with fa.ICGN2D1(15, 15, 0.001, 10) as ic:
+        ic.setImages(fa.Image2D(sc.ref), fa.Image2D(sc.tar))
+        ic.prepare()
+        plain, zero, g = q.copy(), q.copy(), q.copy()
+        ic.compute(plain)
+        ic.compute(zero, np.zeros((len(q), 2), np.float32))
+        ic.compute(g, off)
+        again = q.copy()
+        ic.compute(again)  # the prepared records are rebuilt after an offset call
+    assert plain.tobytes() == zero.tobytes()
+    assert plain.tobytes() == again.tobytes()
+    o = q.copy()
+    oracle.icgn(sc.ref.astype(np.float32), sc.tar.astype(np.float32), 15, o, center_offsets=off)
+    cmp = compare(o, g)
+    report("icgn center offset", cmp)
+    assert cmp["pass"], cmp
+    ok = finite_valid(g)
+    ut, vt, *_ = sc.truth(g["x"] + off[:, 0], g["y"] + off[:, 1])
+    err = np.hypot(g["deformation"][ok, 0] - ut[ok], g["deformation"][ok, 6] - vt[ok])
+    say("center offset vs truth at the offset centre: p99", float(np.percentile(err, 99)))
+    assert ok.mean() > 0.8 and np.percentile(err, 99) < 0.03
+    with fa.ICGN2D1(15, 15, 0.001, 10) as ic:
+        ic.setImages(fa.Image2D(sc.ref), fa.Image2D(sc.tar))
+        ic.prepare()
+        with pytest.raises(ValueError):
+            ic.compute(q.copy(), off[:-1])
+
+
 def report(tag, cmp):
     keys = ["n", "cpu_valid", "gpu_valid", "status_mismatch", "median_du", "p99_du", "max_du",
             "median_dv", "p99_dv", "max_dv"]

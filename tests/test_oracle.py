@@ -348,3 +348,47 @@ def test_icgn_self_adaptive_radius(oracle):
     ok = q["result"][:, 2] >= 0
     assert ok.all()
     np.testing.assert_array_equal(q["subset_radius"][:, 0], [10, 15, 20] * 20)
+
+
+def _offset_scene():
+    return synth.affine_scene("offset", 384, 384, 3400, 11, t=(1.3, -0.8), grad=(0.04, 0.02, -0.03, 0.05),
+                              radius=15, grid=24)
+
+
+def test_icgn_center_offset_closed_form(oracle):
+    """compute(queue, center_offset_queue) (src/oc_icgn.cpp:399-625): the warp
+    acts about the POI plus its offset, so on an affine field the converged u, v
+    are the displacement at (x + ox, y + oy) (a wrong sign lands 2 |grad . o|
+    away) and the gradients are the field's; zero offsets reproduce
+    compute(queue) bit for bit, in FP32 and in the FP64 twin."""
+    sc = _offset_scene()
+    ref, tar = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
+    q = sc.pois.copy()
+    u, v, ux, uy, vx, vy = sc.truth(q["x"], q["y"])
+    d = q["deformation"]
+    d[:, 0], d[:, 6] = np.round(u), np.round(v)
+    d[:, 1], d[:, 2], d[:, 7], d[:, 8] = ux, uy, vx, vy
+    for f64 in (False, True):
+        plain, zero = q.copy(), q.copy()
+        oracle.icgn(ref, tar, 15, plain, f64=f64)
+        oracle.icgn(ref, tar, 15, zero, f64=f64, center_offsets=np.zeros((len(q), 2), np.float32))
+        assert plain.tobytes() == zero.tobytes()
+    rng = np.random.default_rng(5)
+    off = rng.uniform(-3, 3, (len(q), 2)).astype(np.float32)
+    g = q.copy()
+    oracle.icgn(ref, tar, 15, g, center_offsets=off)
+    ok = g["result"][:, 2] >= 0
+    assert ok.mean() > 0.8
+    ut, vt, *_ = sc.truth(g["x"] + off[:, 0], g["y"] + off[:, 1])
+    uw, vw, *_ = sc.truth(g["x"] - off[:, 0], g["y"] - off[:, 1])
+    err = np.hypot(g["deformation"][ok, 0] - ut[ok], g["deformation"][ok, 6] - vt[ok])
+    wrong = np.hypot(g["deformation"][ok, 0] - uw[ok], g["deformation"][ok, 6] - vw[ok])
+    assert np.percentile(err, 99) < 0.03
+    assert np.median(wrong) > 0.1
+    assert np.abs(g["deformation"][ok, 1] - ux[ok]).mean() < 2e-3
+    assert np.abs(g["deformation"][ok, 8] - vy[ok]).mean() < 2e-3
+    # the plain run measures the displacement at the POI itself
+    p = q.copy()
+    oracle.icgn(ref, tar, 15, p)
+    okp = ok & (p["result"][:, 2] >= 0)
+    assert np.abs(p["deformation"][okp, 0] - u[okp]).max() < 0.05
