@@ -27,7 +27,8 @@ FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT, FI_DEPTH12_LEFT = 0, 1, 2
 FI_POLICY_SKIP_FFTCC = 1
 FI_POLICY_POI_FALLBACK = 2
 FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
-FI_SAMPLING_AUTO, FI_SAMPLING_TABLE = 0, 1
+FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP = 0, 1, 2
+SAMPLING_MODES = {"auto": FI_SAMPLING_AUTO, "table": FI_SAMPLING_TABLE, "group": FI_SAMPLING_GROUP}
 FI_POLICY_LEARNED_SEED = 4
 
 # opencorr::POI2D, src/oc_poi.h:102-108: 25 float32 = 100 bytes

@@ -387,6 +387,7 @@ void fi_icgn::compute(fi_poi* pois, size_t n, const float* offsets)
     p.gy = gy.p;
     p.coef = coef.p;
     p.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+    p.group = sampling == FI_SAMPLING_GROUP ? 1 : 0;
     p.w = ref_raw.view.width;
     p.h = ref_raw.view.height;
     p.pois = d_pois.p;
@@ -785,6 +786,7 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     // samples the f32 target when a warp outgrows the tile
     ip.coef = mode == FI_SAMPLING_TABLE ? coef.p : nullptr;
     ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+    ip.group = mode == FI_SAMPLING_GROUP ? 1 : 0;
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
     ip.pois = queue_base;
@@ -842,6 +844,7 @@ void fi_frame::icgn_fused(fi_poi* base)
     // samples the f32 target when a warp outgrows the tile
     ip.coef = mode == FI_SAMPLING_TABLE ? coef.p : nullptr;
     ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+    ip.group = mode == FI_SAMPLING_GROUP ? 1 : 0;
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
     ip.pois = base;
@@ -1238,10 +1241,13 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
         int bw = 0, bs = 0, bsamp = -1, bq = 0;
         size_t best_entry = SIZE_MAX;
         for (int si = 0; si < ns; si++) {
-            for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE }) {
-                // resident blocks of the queue launch: 12 waves per CU at the
-                // kernel's occupancy target of 3 waves per SIMD
-                for (int qb : queue_counts(c.r, ws[si], ss[si], samp)) {
+            for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP }) {
+                // shared coefficient tiles: one-wave shapes, one block per
+                // four entries (no queue)
+                if (samp == FI_SAMPLING_GROUP && ws[si] != 1) continue;
+                const std::vector<int> qbs =
+                    samp == FI_SAMPLING_GROUP ? std::vector<int>{ 0 } : queue_counts(c.r, ws[si], ss[si], samp);
+                for (int qb : qbs) {
                     c.waves = ws[si];
                     c.spl = ss[si];
                     c.sampling = samp;

@@ -97,6 +97,8 @@ struct IcgnParams {
     int* queue;
     const unsigned* count;    // device entry count, at most n (nullptr: n); set, block b runs
                               // entry b (one block per entry, no queue)
+    int group;                // 1: one-wave shapes run as k_icgn_group (four neighbouring
+                              // entries share a coefficient tile of tile_h rows)
     const float* offset;      // per POI (indexed like pois) an (x, y) pair: subset centre
                               // offset of compute(POI2D*, Point2D&), nullptr = none
     int queue_blocks;

@@ -306,7 +306,28 @@ static IcgnParams with_tile(const IcgnParams& p)
 
 bool launch_icgn_shape(const IcgnParams& p0, int waves, int spl, hipStream_t s)
 {
-    const IcgnParams p = with_tile(p0);
+    IcgnParams p = with_tile(p0);
+    if (p.offset) {
+        // centre offsets: the generic shape of this width covers the subset
+        const int R = band_rows(waves, p.rx, p.ry);
+        if (R == 0 || R > spl || spl > 16) return false;
+        if (p.n <= 0) return true;
+        p.group = 0;
+        p.queue = nullptr;
+        if (waves == 1) launch_icgn_offset_t<1>(p, s);
+        else if (waves == 2) launch_icgn_offset_t<2>(p, s);
+        else if (waves == 4) launch_icgn_offset_t<4>(p, s);
+        else return false;
+        return true;
+    }
+    if (p.group) {
+        // shared coefficient tiles serve one-wave shapes on neighbouring
+        // entries; anything else runs with a tile per entry
+        if (waves != 1 || p.tar == nullptr || p.count)
+            p.group = 0;
+        else
+            p.tile_h = 2 * p.ry + 4 + 2 * kTileMargin + kGroupRowSlack;
+    }
     const int R = band_rows(waves, p.rx, p.ry);
     if (R == 0 || R > spl) return false;
     if (p.n <= 0) return true;

@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "fi_device.h"
 #include "fi_launch.h"
@@ -193,6 +194,59 @@ __device__ __forceinline__ float target_sample(const float* tar, uint32_t w, flo
         return __builtin_fmaf(x3, q[3], __builtin_fmaf(x2, q[2], __builtin_fmaf(x1, q[1], x0 * q[0])));
     };
     return __builtin_fmaf(y3, row(d), __builtin_fmaf(y2, row(c), __builtin_fmaf(y1, row(b), y0 * row(a))));
+}
+
+// ---------------------------------------------------------------------------
+// Shared coefficient tile (sampling mode FI_SAMPLING_GROUP): kGroupPois
+// neighbouring POIs, one wave each, share one LDS tile holding, per pixel
+// (row n, column ix), the x-polynomial of the row segment [ix, ix+1]:
+//     r_n(dx) = sum_m w_m(dx) Q[n][ix-1+m] = c0 + c1 dx + c2 dx^2 + c3 dx^3,
+//     c3 = sum_m BC[0][m] Q[n][ix-1+m], c2 = sum_m BC[1][m] .., c1 = sum_m BC[2][m] ..,
+//     c0 = Q[n][ix] (BC's last row is (0, 1, 0, 0)),
+// so a sample is four Horner rows and the y weights: about 14 VALU fewer than
+// the raw tile's weights and 4x4 combination.  16 bytes per pixel is too much
+// LDS for one POI per block at 12 blocks per CU; a block of four neighbouring
+// POIs (5 px apart on a grid) shares one.
+// ---------------------------------------------------------------------------
+constexpr int kGroupPois = 4;
+constexpr int kGroupStride = 64;  // entries per LDS row: four rows are immediate offsets of one address
+constexpr int kGroupRowSlack = 8;  // rows of vertical spread the four neighbourhoods may add
+
+// Entry (row, col) of a tile whose pixel (0, 0) is image pixel (tx0, ty0)
+// holds the polynomial of pixel column tx0 + 1 + col; cb = -(ty0 + 1) *
+// kGroupStride - (tx0 + 1) places the sample's first row.  The caller
+// guarantees the neighbourhood lies in the tile.
+__device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx, float gy)
+{
+    const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
+    const float4* q = ct + (__mul24((int)gy, kGroupStride) + (int)gx + cb);
+    auto horner = [&](const float4 c) {
+        return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, dx, c.z), dx, c.y), dx, c.x);
+    };
+    const float r0 = horner(q[0]), r1 = horner(q[kGroupStride]);
+    const float r2 = horner(q[2 * kGroupStride]), r3 = horner(q[3 * kGroupStride]);
+    float y0, y1, y2, y3;
+    spline_weights(dy, y0, y1, y2, y3);
+    return __builtin_fmaf(y3, r3, __builtin_fmaf(y2, r2, __builtin_fmaf(y1, r1, y0 * r0)));
+}
+
+// Coefficients of pixels tx0 + 1 .. tx0 + tw - 3 of rows ty0 .. ty0 + th - 1
+// (the image pixels tx0 .. tx0 + tw - 1 they read lie in the image).
+__device__ __forceinline__ void stage_coef(float4* ct, const float* tar, int w, int tx0, int ty0, int tw, int th,
+                                           int tid, int nt)
+{
+    const int cols = tw - 3, n = cols * th;
+    for (int e = tid; e < n; e += nt) {
+        const int r = e / cols, c = e - r * cols;
+        const float* q = tar + (size_t)(ty0 + r) * w + (tx0 + c);
+        const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+        float4 v;
+        v.x = q1;
+        v.y = __builtin_fmaf(bc(2, 3), q3, __builtin_fmaf(bc(2, 2), q2, __builtin_fmaf(bc(2, 1), q1, bc(2, 0) * q0)));
+        v.z = __builtin_fmaf(bc(1, 3), q3, __builtin_fmaf(bc(1, 2), q2, __builtin_fmaf(bc(1, 1), q1, bc(1, 0) * q0)));
+        v.w = __builtin_fmaf(bc(0, 3), q3, __builtin_fmaf(bc(0, 2), q2, __builtin_fmaf(bc(0, 1), q1, bc(0, 0) * q0)));
+        ct[r * kGroupStride + c] = v;
+    }
 }
 
 // Copy image rows ty0.. and columns tx0.. (tw x th pixels) into the tile,
@@ -372,6 +426,12 @@ __device__ __forceinline__ const __attribute__((address_space(4))) T* reread(con
 // tile survives the first iterations' motion without restaging.
 constexpr int kTileMargin = 2;
 
+// Corners of a warped subset closer than this to the interpolatable range's
+// edge send the iteration through the exact per-sample test; a sample and the
+// box bound it is tested against differ by at most kTileSlack in rounding.
+constexpr float kEdgeMargin = 1.f / 64.f;
+constexpr float kTileSlack = 0.125f;
+
 // Largest band overlap a launched shape may have (checked on the host).
 constexpr int kMaxOverlap = 4;
 
@@ -384,22 +444,38 @@ constexpr int kMaxOverlap = 4;
 // One block of WAVES wavefronts per POI; each lane owns SPL rows of one subset
 // column.  EXACT: the host guarantees R == SPL, so no slot is skipped.  Every
 // early return is taken by the whole block.
-template <int WAVES, int SPL, bool EXACT>
-__device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, float* red, int* flags, float* tile)
+// GROUP (WAVES == 1, k_icgn_group): tile is the block's shared coefficient
+// tile, pixel (0, 0) = image (gx0, gy0), gw x gh pixels (gw = 0: none); it is
+// never restaged, and an iteration whose neighbourhoods leave it samples the
+// target in device memory.  No block barrier is reached on this path, so the
+// block's waves run their entries independently.
+// OFFSET: p.offset holds the entries' centre offsets (compiled only where a
+// launch needs them: the offsets are uniform values live across the whole
+// iteration, which the register budget of the plain kernels cannot afford).
+template <int WAVES, int SPL, bool EXACT, bool GROUP = false, bool OFFSET = false>
+__device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, float* red, int* flags, float* tile,
+                                           int gx0 = 0, int gy0 = 0, int gw = 0, int gh = 0)
 {
+    static_assert(!GROUP || WAVES == 1, "a shared coefficient tile serves one-wave entries");
     constexpr int NT = 64 * WAVES;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the thread's index within the entry: in a group block each wave is an
+    // entry of its own
+    const int tid = GROUP ? (int)(threadIdx.x & 63) : (int)threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int i = p.index ? (int)p.index[b] : b;
     fi_poi* poi = p.pois + i;
     const float x = poi->x, y = poi->y;
     // compute(POI2D*, Point2D& center_offset), src/oc_icgn.cpp:399-615: the
     // warp acts about (x + ox, y + oy) on local coordinates minus the offset
     float ox = 0.f, oy = 0.f;
-    if (p.offset) {
+    if constexpr (OFFSET) {
         ox = p.offset[2 * (size_t)i];
         oy = p.offset[2 * (size_t)i + 1];
     }
-    const float xc = x + ox, yc = y + oy;
+    const float xc = OFFSET ? x + ox : x, yc = OFFSET ? y + oy : y;
+    // the warped image of the local box's centre (-ox, -oy), relative to (xc, yc)
+    auto box_dx = [&](float W00_, float W01_, float W02_) {
+        return OFFSET ? __builtin_fmaf(-W00_, ox, __builtin_fmaf(-W01_, oy, W02_)) : W02_;
+    };
     const float u_in = poi->deformation[0], ux_in = poi->deformation[1], uy_in = poi->deformation[2];
     const float v_in = poi->deformation[6], vx_in = poi->deformation[7], vy_in = poi->deformation[8];
     const float z_in = poi->result[2];
@@ -438,16 +514,12 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     const float rx2 = (float)(rx * rx), ry2 = (float)(ry * ry);
     const float inv_ns = 1.f / (float)((2 * rx + 1) * (2 * ry + 1));
     const float conv_crit = p.conv, stop = p.stop;
-    // corners closer than this to the interpolatable range's edge send the
-    // iteration through the exact per-sample test
-    constexpr float kEdgeMargin = 1.f / 64.f;
 
     // The LDS tile holds the raw target around the warped subset; it is
     // (re)staged when an iteration's neighbourhoods leave it, and an iteration
     // whose neighbourhoods cannot fit in it samples the table instead.
-    const int tile_w = p.tile_w, tile_h = p.tile_h, tile_stride = p.tile_stride;
-    int tx0 = -(1 << 30), ty0 = -(1 << 30);
-    constexpr float kTileSlack = 0.125f;  // bound on sample vs box rounding
+    const int tile_w = GROUP ? gw : p.tile_w, tile_h = GROUP ? gh : p.tile_h, tile_stride = p.tile_stride;
+    int tx0 = GROUP ? gx0 : -(1 << 30), ty0 = GROUP ? gy0 : -(1 << 30);
 
     // Every lane loads (idle lanes and repeated rows address real subset
     // samples; a generic shape's slots past R repeat row R-1), then masks: no
@@ -466,12 +538,11 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     }
     // Stage the tile for the incoming warp now, so its loads are in flight
     // together with the reference loads above (one round trip, not two).
-    {
+    if constexpr (!GROUP) {
         const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
         const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
-        // the warped image of the local box's centre (-ox, -oy)
-        const float cx = xc + __builtin_fmaf(-W00, ox, __builtin_fmaf(-W01, oy, W02));
-        const float cy = yc + __builtin_fmaf(-W10, ox, __builtin_fmaf(-W11, oy, W12));
+        const float cx = xc + box_dx(W00, W01, W02);
+        const float cy = yc + box_dx(W10, W11, W12);
         if (tile_w > 0 && cx - ex >= 1.f + kEdgeMargin && cx + ex < wm2 - kEdgeMargin &&
             cy - ey >= 1.f + kEdgeMargin && cy + ey < hm2 - kEdgeMargin) {
             const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
@@ -488,7 +559,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     }
     const float m_all = bd.active ? 1.f : 0.f;
     const float m_dup = bd.last ? 0.f : 1.f;
-    const float xl = (float)(bd.col - rx) - ox, yl0 = (float)(bd.row0 - ry) - oy;
+    const float xl = OFFSET ? (float)(bd.col - rx) - ox : (float)(bd.col - rx);
+    const float yl0 = OFFSET ? (float)(bd.row0 - ry) - oy : (float)(bd.row0 - ry);
     const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)w * (uint32_t)h * 64u);
     const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
 
@@ -506,8 +578,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         // anywhere in the warp fails the test.
         const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
         const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
-        const float cx = xc + __builtin_fmaf(-W00, ox, __builtin_fmaf(-W01, oy, W02));
-        const float cy = yc + __builtin_fmaf(-W10, ox, __builtin_fmaf(-W11, oy, W12));
+        const float cx = xc + box_dx(W00, W01, W02);
+        const float cy = yc + box_dx(W10, W11, W12);
         const bool box_in = (cx - ex >= 1.f + kEdgeMargin) & (cx + ex < wm2 - kEdgeMargin) &
                             (cy - ey >= 1.f + kEdgeMargin) & (cy + ey < hm2 - kEdgeMargin);
         if (!box_in) {
@@ -540,16 +612,21 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         const int nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
         const int ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
         const int ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
-        const bool use_tile = tile_w > 0 && nx1 - nx0 < tile_w && ny1 - ny0 < tile_h;
-        if (use_tile && !(nx0 >= tx0 && nx1 < tx0 + tile_w && ny0 >= ty0 && ny1 < ty0 + tile_h)) {
-            // centre the tile on the neighbourhood, inside the image
-            tx0 = min(max(nx0 - (tile_w - (nx1 - nx0 + 1)) / 2, 0), w - tile_w);
-            ty0 = min(max(ny0 - (tile_h - (ny1 - ny0 + 1)) / 2, 0), h - tile_h);
-            if (WAVES > 1) __syncthreads();  // every wave is done with the old tile
-            stage_tile(tile, p.tar, w, tx0, ty0, tile_w, tile_h, tile_stride, tid, NT);
-            __syncthreads();
+        bool use_tile;
+        if constexpr (GROUP) {
+            use_tile = tile_w > 0 && nx0 >= tx0 && nx1 < tx0 + tile_w && ny0 >= ty0 && ny1 < ty0 + tile_h;
+        } else {
+            use_tile = tile_w > 0 && nx1 - nx0 < tile_w && ny1 - ny0 < tile_h;
+            if (use_tile && !(nx0 >= tx0 && nx1 < tx0 + tile_w && ny0 >= ty0 && ny1 < ty0 + tile_h)) {
+                // centre the tile on the neighbourhood, inside the image
+                tx0 = min(max(nx0 - (tile_w - (nx1 - nx0 + 1)) / 2, 0), w - tile_w);
+                ty0 = min(max(ny0 - (tile_h - (ny1 - ny0 + 1)) / 2, 0), h - tile_h);
+                if (WAVES > 1) __syncthreads();  // every wave is done with the old tile
+                stage_tile(tile, p.tar, w, tx0, ty0, tile_w, tile_h, tile_stride, tid, NT);
+                __syncthreads();
+            }
         }
-        const int cbase = (ty0 + 1) * tile_stride + tx0 + 1;
+        const int cbase = GROUP ? -(ty0 + 1) * kGroupStride - (tx0 + 1) : (ty0 + 1) * tile_stride + tx0 + 1;
 
         // pass 1: sample; t' per slot, and the sums of t', t'^2, t' r (a
         // repeated row counts once: its t' is masked, its r is zero)
@@ -571,9 +648,14 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
                 }
             }
         };
-        if (use_tile)
-            pass1([&](float gx, float gy, int) { return tile_sample(tile, cbase, gx, gy); });
-        else if (p.coef)
+        if (use_tile) {
+            if constexpr (GROUP)
+                pass1([&](float gx, float gy, int) {
+                    return coef_sample(reinterpret_cast<const float4*>(tile), cbase, gx, gy);
+                });
+            else
+                pass1([&](float gx, float gy, int) { return tile_sample(tile, cbase, gx, gy); });
+        } else if (p.coef)
             pass1([&](float gx, float gy, int) { return bspline_sample(table, row_bytes, plane_bytes, gx, gy); });
         else
             pass1([&](float gx, float gy, int) { return target_sample(p.tar, (uint32_t)w, gx, gy); });
@@ -679,20 +761,17 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
 // counters, kQueueStride ints apart, zeroed.
 constexpr int kQueueChunk = 4;
 
-template <int WAVES, int SPL, bool EXACT>
+template <int WAVES, int SPL, bool EXACT, bool OFFSET = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
 void k_icgn(IcgnParams p)
 {
     __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
     __shared__ int flags[WAVES];
     extern __shared__ float tile[];
-    if (p.count) {
-        const int b = (int)blockIdx.x;
-        if (b < min((int)*p.count, p.n)) icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
-    } else {
-        const int b = xcd_remap(blockIdx.x, gridDim.x);
-        if (b < p.n) icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
-    }
+    // one call site: the entry is inlined once
+    const int b = p.count ? (int)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
+    const int n = p.count ? min((int)*p.count, p.n) : p.n;
+    if (b < n) icgn_entry<WAVES, SPL, EXACT, false, OFFSET>(p, b, red, flags, tile);
 }
 
 template <int WAVES, int SPL, bool EXACT>
@@ -746,6 +825,95 @@ void k_icgn_queue(IcgnParams p)
     }
 }
 
+// Pixel neighbourhood (columns nx0..nx1, rows ny0..ny1) of entry b's incoming
+// warp, as icgn_entry computes it; false when the entry fails the gate or its
+// warped box is not inside the interpolatable range by kEdgeMargin.
+__device__ __forceinline__ bool entry_box(const IcgnParams& p, int b, int& nx0, int& nx1, int& ny0, int& ny1)
+{
+    const int i = p.index ? (int)p.index[b] : b;
+    const fi_poi* poi = p.pois + i;
+    const float x = poi->x, y = poi->y;
+    float ox = 0.f, oy = 0.f;
+    if (p.offset) {
+        ox = p.offset[2 * (size_t)i];
+        oy = p.offset[2 * (size_t)i + 1];
+    }
+    const float u = poi->deformation[0], ux = poi->deformation[1], uy = poi->deformation[2];
+    const float v = poi->deformation[6], vx = poi->deformation[7], vy = poi->deformation[8];
+    int rx = p.rx, ry = p.ry;
+    if (p.self_adaptive) {
+        rx = (int)poi->subset_radius[0];
+        ry = (int)poi->subset_radius[1];
+    }
+    const int w = p.w, h = p.h;
+    if (!ref_gate_ok(x, y, rx, ry, w, h) || fabsf(u) >= (float)w || fabsf(v) >= (float)h || poi->result[2] < 0.f ||
+        isnan(u) || isnan(v))
+        return false;
+    const float W00 = 1.f + ux, W01 = uy, W10 = vx, W11 = 1.f + vy;
+    const float ex = __builtin_fmaf(fabsf(W00), (float)rx, fabsf(W01) * (float)ry);
+    const float ey = __builtin_fmaf(fabsf(W10), (float)rx, fabsf(W11) * (float)ry);
+    const float cx = (x + ox) + __builtin_fmaf(-W00, ox, __builtin_fmaf(-W01, oy, u));
+    const float cy = (y + oy) + __builtin_fmaf(-W10, ox, __builtin_fmaf(-W11, oy, v));
+    if (!(cx - ex >= 1.f + kEdgeMargin && cx + ex < (float)(w - 2) - kEdgeMargin && cy - ey >= 1.f + kEdgeMargin &&
+          cy + ey < (float)(h - 2) - kEdgeMargin))
+        return false;
+    nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
+    nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
+    ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
+    ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
+    return true;
+}
+
+// kGroupPois consecutive entries per block, one wave each, sharing one
+// coefficient tile that covers their incoming neighbourhoods plus
+// kTileMargin (p.tile_h = rows allocated; a union that does not fit leaves the
+// block without a tile).  The two barriers come before any wave starts its
+// entry.
+template <int SPL, bool EXACT>
+__global__ __launch_bounds__(64 * kGroupPois) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
+void k_icgn_group(IcgnParams p)
+{
+    __shared__ int gbox[kGroupPois][4];
+    __shared__ float red[1];
+    __shared__ int flags[kGroupPois];
+    extern __shared__ float4 ctile[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int b = xcd_remap(blockIdx.x, gridDim.x) * kGroupPois + wave;
+    int bx0 = 0, bx1 = -1, by0 = 0, by1 = -1;
+    const bool has = b < p.n && entry_box(p, b, bx0, bx1, by0, by1);
+    if (lane == 0) {
+        gbox[wave][0] = has ? bx0 : INT_MAX;
+        gbox[wave][1] = bx1;
+        gbox[wave][2] = by0;
+        gbox[wave][3] = by1;
+    }
+    __syncthreads();
+    int X0 = INT_MAX, X1 = INT_MIN, Y0 = INT_MAX, Y1 = INT_MIN;
+#pragma unroll
+    for (int v = 0; v < kGroupPois; v++)
+        if (gbox[v][0] != INT_MAX) {
+            X0 = min(X0, gbox[v][0]);
+            X1 = max(X1, gbox[v][1]);
+            Y0 = min(Y0, gbox[v][2]);
+            Y1 = max(Y1, gbox[v][3]);
+        }
+    int gw = 0, gh = 0;
+    if (X0 <= X1) {
+        X0 = max(X0 - kTileMargin, 0);
+        X1 = min(X1 + kTileMargin, p.w - 1);
+        Y0 = max(Y0 - kTileMargin, 0);
+        Y1 = min(Y1 + kTileMargin, p.h - 1);
+        if (X1 - X0 - 2 <= kGroupStride && Y1 - Y0 + 1 <= p.tile_h) {
+            gw = X1 - X0 + 1;
+            gh = Y1 - Y0 + 1;
+        }
+    }
+    if (gw > 0) stage_coef(ctile, p.tar, p.w, X0, Y0, gw, gh, threadIdx.x, 64 * kGroupPois);
+    __syncthreads();
+    if (b < p.n)
+        icgn_entry<1, SPL, EXACT, true>(p, b, red, flags + wave, reinterpret_cast<float*>(ctile), X0, Y0, gw, gh);
+}
+
 // Instantiated lane shapes (waves, rows per band).  EXACT_A/B are compiled in
 // k_icgn_xa.hip / k_icgn_xb.hip, GENERIC in k_icgn.hip.
 #define FI_ICGN_EXACT_A(X) \
@@ -758,12 +926,30 @@ void k_icgn_queue(IcgnParams p)
 template <int WAVES, int SPL, bool EXACT>
 void launch_icgn_t(const IcgnParams& p, hipStream_t s)
 {
+    if constexpr (WAVES == 1) {
+        if (p.group) {
+            // p.tile_h: coefficient rows per block
+            const size_t glds = (size_t)p.tile_h * kGroupStride * sizeof(float4);
+            hipLaunchKernelGGL((k_icgn_group<SPL, EXACT>), dim3((p.n + kGroupPois - 1) / kGroupPois),
+                               dim3(64 * kGroupPois), glds, s, p);
+            return;
+        }
+    }
     const size_t lds = p.tile_w > 0 ? (size_t)p.tile_stride * p.tile_h * sizeof(float) : 0;
     if (p.queue && !p.count)
         hipLaunchKernelGGL((k_icgn_queue<WAVES, SPL, EXACT>), dim3(std::max(1, std::min(p.n, p.queue_blocks))),
                            dim3(64 * WAVES), lds, s, p);
     else
         hipLaunchKernelGGL((k_icgn<WAVES, SPL, EXACT>), dim3(p.n), dim3(64 * WAVES), lds, s, p);
+}
+
+// Centre offsets (p.offset): the generic shape of the same width, one block
+// per entry.
+template <int WAVES>
+void launch_icgn_offset_t(const IcgnParams& p, hipStream_t s)
+{
+    const size_t lds = p.tile_w > 0 ? (size_t)p.tile_stride * p.tile_h * sizeof(float) : 0;
+    hipLaunchKernelGGL((k_icgn<WAVES, 16, false, true>), dim3(p.n), dim3(64 * WAVES), lds, s, p);
 }
 
 // Blocks of k_icgn_queue<WAVES, SPL, EXACT> resident on one CU at p's tile

@@ -70,8 +70,9 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_set_shape", self._h, int(waves), int(rows_per_band))
 
     def set_sampling(self, mode: str) -> None:
-        """IC-GN target sampling: 'auto' or 'table' (see ICGN2D1.setSampling)."""
-        modes = {"auto": nat.FI_SAMPLING_AUTO, "table": nat.FI_SAMPLING_TABLE}
+        """IC-GN target sampling: 'auto', 'table' or 'group' (see
+        ICGN2D1.setSampling)."""
+        modes = nat.SAMPLING_MODES
         if mode not in modes:
             raise ValueError(f"sampling must be one of {sorted(modes)}")
         nat.call("fi_frame_set_sampling", self._h, modes[mode])

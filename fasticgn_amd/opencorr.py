@@ -198,9 +198,11 @@ class ICGN2D1(_Handle):
         nat.call("fi_icgn_set_shape", self._h, int(waves), int(rows_per_band))
 
     def setSampling(self, mode: str) -> None:
-        """'auto' (raw neighbourhood staged in LDS, table fallback) or 'table'
-        (every sample from the B-spline table)."""
-        modes = {"auto": nat.FI_SAMPLING_AUTO, "table": nat.FI_SAMPLING_TABLE}
+        """'auto' (raw neighbourhood staged in LDS, table fallback), 'table'
+        (every sample from the B-spline table) or 'group' (four neighbouring
+        POIs share an LDS tile of x-polynomial coefficients; one-wave lane
+        shapes, others run as 'auto')."""
+        modes = nat.SAMPLING_MODES
         if mode not in modes:
             raise ValueError(f"sampling must be one of {sorted(modes)}")
         nat.call("fi_icgn_set_sampling", self._h, modes[mode])

@@ -157,6 +157,11 @@ int fi_icgn_set_shape(fi_icgn* h, int waves, int rows_per_band);
  * differ only in FP32 rounding.  No reference counterpart. */
 #define FI_SAMPLING_AUTO 0
 #define FI_SAMPLING_TABLE 1
+/* FI_SAMPLING_GROUP: four neighbouring POIs (consecutive in the queue), one
+ * wave each, share one LDS tile of per-pixel x-polynomial coefficients (Horner
+ * rows instead of weights x pixels); an iteration that leaves it reads the
+ * target in device memory.  One-wave lane shapes only; others run as AUTO. */
+#define FI_SAMPLING_GROUP 2
 int fi_icgn_set_sampling(fi_icgn* h, int mode);
 int fi_icgn_set_ref(fi_icgn* h, const void* px, int width, int height,
                     size_t stride_bytes, int pixel, int depth12);

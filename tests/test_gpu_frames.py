@@ -172,7 +172,10 @@ def test_autotune_keeps_one_configuration(seq, targets):
     for rad in synth.RADII_MIXED:
         rows = [e for e in rep if e["radius"] == rad]
         assert len(rows) >= 4
-        assert {e["sampling"] for e in rows} == {0, 1} and {e["persistent"] > 0 for e in rows} == {False, True}
+        # group sampling (2) exists for the one-wave shapes, which radii 10 and 15 have
+        assert {e["sampling"] for e in rows} >= {0, 1} and {e["persistent"] > 0 for e in rows} == {False, True}
+        assert (2 in {e["sampling"] for e in rows}) == any(e["waves"] == 1 for e in rows)
+        assert all(e["persistent"] == 0 for e in rows if e["sampling"] == 2)
         assert sum(e["chosen"] for e in rows) == (0 if fused_chosen else 1)
         say("radius", rad, "best", min(rows, key=lambda e: e["ms"]), "of", len(rows))
     say("fused best", min(fused_rows, key=lambda e: e["ms"]), "chosen" if fused_chosen else "not chosen")
@@ -302,3 +305,25 @@ def test_frame_large_strain_samples_target_without_table():
         say("large strain", mode, {k: cmp[k] for k in ("n", "gpu_valid", "status_mismatch", "p99_du", "max_du")})
         assert cmp["pass"], cmp
         assert cmp["gpu_valid"] > 0.5 * cmp["n"]
+
+
+def test_frame_group_sampling_meets_the_gate(seq, targets):
+    """Group sampling (four neighbouring POIs per block sharing a coefficient
+    tile) through the mixed-radius driver: radii 10 and 15 run one-wave shapes
+    in group mode, radius 20 runs as 'auto'; the frame meets the parity gate
+    against the default sampling, and graph replay is byte-identical."""
+    k = STEPS[2]
+    with mixed_driver(seq) as d:
+        base, _ = d.process(img(targets[k]), policy=0)
+    with mixed_driver(seq) as d:
+        d.set_sampling("group")
+        grp, st = d.process(img(targets[k]), policy=0)
+    cmp = compare(base, grp)
+    say("group vs auto", {x: cmp[x] for x in ("n", "gpu_valid", "status_mismatch", "p99_du", "max_du")})
+    assert cmp["pass"], cmp
+    with mixed_driver(seq, graph=True) as d:
+        d.set_sampling("group")
+        a, _ = d.process(img(targets[k]), policy=0)
+        d.reset_sequence()
+        b, _ = d.process(img(targets[k]), policy=0)
+    assert same_bytes(a, b) and same_bytes(a, grp)

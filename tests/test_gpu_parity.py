@@ -126,6 +126,28 @@ def run_icgn_gpu(ref, tar, radius, q, conv=0.001, stop=10.0, self_adaptive=False
     return g
 
 
+@pytest.mark.parametrize("grid", [5, 10])
+def test_icgn_group_sampling_parity(oracle, grid):
+    """Group sampling: four consecutive POIs per block share one LDS tile of
+    x-polynomial coefficients.  On a 5 px grid a block's neighbourhoods fit one
+    tile; on a 10 px grid they do not and every sample reads the target in
+    device memory (the table here).  Both meet the parity gate against the CPU
+    oracle, validity flags identical."""
+    sc = synth.affine_scene("group", 384, 384, 3400, 13, t=(1.6, -0.9), grad=(2e-3, 0.0, 0.0, 2e-3),
+                            radius=15, grid=grid)
+    q = sc.pois.copy()
+    u, v, *_ = sc.truth(q["x"], q["y"])
+    q["deformation"][:, 0] = np.round(u)
+    q["deformation"][:, 6] = np.round(v)
+    g = run_icgn_gpu(sc.ref, sc.tar, 15, q, sampling="group")
+    o = q.copy()
+    oracle.icgn(sc.ref.astype(np.float32), sc.tar.astype(np.float32), 15, o)
+    cmp = compare(o, g)
+    report(f"icgn group sampling grid {grid}", cmp)
+    assert cmp["pass"], cmp
+    assert finite_valid(g).mean() > 0.9
+
+
 def test_icgn_center_offset_parity(oracle):
     """compute(queue, center_offset_queue) (src/oc_icgn.cpp:399-625) on the
     device against the CPU oracle's, offsets U[-3, 3] px on an affine field with
