@@ -977,8 +977,11 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 
     auto segment = [&](bool timed) {
         // a fill kernel, not memsets: this runs inside captured frame graphs
+        // FFTCC and learned seeds start from the template: copied here, in the
+        // reset launch
         launch_frame_reset(iter_total.p, reinterpret_cast<uint32_t*>(counters.p),
-                           (C + 1) * sizeof(FrameCounters) / 4, redo.p, n, mode == 1 ? 1 : 0, stream);
+                           (C + 1) * sizeof(FrameCounters) / 4, redo.p, n, mode == 1 ? 1 : 0, stream,
+                           mode == 2 ? nullptr : tmpl.p, cur.p, n * sizeof(fi_poi));
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
         // the B-spline table, when a launch samples it (HBM-write-bound), on a
         // second stream beside the seeding (FFTCC or the seeder read only the
@@ -992,7 +995,6 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         launch_to_float(tview, tar_f.p, stream);
         if (timed) FI_HIP(hipEventRecord(ev[1], stream));
         if (mode == 0) {
-            FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             // each pass crops the target at the previous pass's rounded u, v; a
             // POI a later pass cannot crop keeps the earlier seed, and only the
             // first pass's flag decides the FFTCC fallback for an unseeded POI
@@ -1000,7 +1002,6 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             for (int pass = 1; pass < seed_passes; pass++)
                 seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, nullptr, stream);
         } else if (mode == 1) {
-            FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             fftcc_all(cur.p);
         } else {
             launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);

@@ -142,6 +142,6 @@ void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s, cons
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 // *total = 0, words[0..n_words) = 0, flags[0..n) = v: one launch
 void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
-                        uint8_t v, hipStream_t s);
+                        uint8_t v, hipStream_t s, const void* src = nullptr, void* dst = nullptr, size_t bytes = 0);
 
 }  // namespace fi

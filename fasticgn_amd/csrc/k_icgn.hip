@@ -326,7 +326,7 @@ bool launch_icgn_shape(const IcgnParams& p0, int waves, int spl, hipStream_t s)
         if (waves != 1 || p.tar == nullptr || p.count)
             p.group = 0;
         else
-            p.tile_h = 2 * p.ry + 4 + 2 * kTileMargin + kGroupRowSlack;
+            p.tile_h = 2 * p.ry + 4 + 2 * kGroupMargin + kGroupRowSlack;
     }
     const int R = band_rows(waves, p.rx, p.ry);
     if (R == 0 || R > spl) return false;

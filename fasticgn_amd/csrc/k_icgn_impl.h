@@ -211,6 +211,12 @@ __device__ __forceinline__ float target_sample(const float* tar, uint32_t w, flo
 constexpr int kGroupPois = 4;
 constexpr int kGroupStride = 64;  // entries per LDS row: four rows are immediate offsets of one address
 constexpr int kGroupRowSlack = 8;  // rows of vertical spread the four neighbourhoods may add
+// pixels of slack around the union of the four incoming neighbourhoods: motion
+// the entries' iterations may make before they leave the tile
+#ifndef FI_GROUP_MARGIN
+#define FI_GROUP_MARGIN 2
+#endif
+constexpr int kGroupMargin = FI_GROUP_MARGIN;
 
 // Entry (row, col) of a tile whose pixel (0, 0) is image pixel (tx0, ty0)
 // holds the polynomial of pixel column tx0 + 1 + col; cb = -(ty0 + 1) *
@@ -899,10 +905,10 @@ void k_icgn_group(IcgnParams p)
         }
     int gw = 0, gh = 0;
     if (X0 <= X1) {
-        X0 = max(X0 - kTileMargin, 0);
-        X1 = min(X1 + kTileMargin, p.w - 1);
-        Y0 = max(Y0 - kTileMargin, 0);
-        Y1 = min(Y1 + kTileMargin, p.h - 1);
+        X0 = max(X0 - kGroupMargin, 0);
+        X1 = min(X1 + kGroupMargin, p.w - 1);
+        Y0 = max(Y0 - kGroupMargin, 0);
+        Y1 = min(Y1 + kGroupMargin, p.h - 1);
         if (X1 - X0 - 2 <= kGroupStride && Y1 - Y0 + 1 <= p.tile_h) {
             gw = X1 - X0 + 1;
             gh = Y1 - Y0 + 1;

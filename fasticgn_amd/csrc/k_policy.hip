@@ -256,22 +256,34 @@ __global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v)
 }
 
 // Start of a frame in one launch: zero the iteration total and the counters,
-// set every per-POI FFTCC-redo flag to v.
+// set every per-POI FFTCC-redo flag to v and, when src is given, copy bytes
+// (a multiple of 4; both buffers 16-byte aligned) from src to dst: the POI
+// template into the frame's queue, without a copy node of its own.
 __global__ void k_frame_reset(unsigned long long* __restrict__ total, uint32_t* __restrict__ words, size_t n_words,
-                              uint8_t* __restrict__ flags, size_t n, uint8_t v)
+                              uint8_t* __restrict__ flags, size_t n, uint8_t v, const void* __restrict__ src,
+                              void* __restrict__ dst, size_t bytes)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
     if (t == 0) *total = 0ull;
     for (size_t i = t; i < n_words; i += stride) words[i] = 0u;
     for (size_t i = t; i < n; i += stride) flags[i] = v;
+    if (src) {
+        const size_t n16 = bytes / 16;
+        const uint4* s4 = static_cast<const uint4*>(src);
+        uint4* d4 = static_cast<uint4*>(dst);
+        for (size_t i = t; i < n16; i += stride) d4[i] = s4[i];
+        const size_t tail = (bytes - 16 * n16) / 4;
+        if (t < tail)
+            static_cast<uint32_t*>(dst)[4 * n16 + t] = static_cast<const uint32_t*>(src)[4 * n16 + t];
+    }
 }
 
 void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
-                        uint8_t v, hipStream_t s)
+                        uint8_t v, hipStream_t s, const void* src, void* dst, size_t bytes)
 {
-    const size_t m = std::max(n_words, n);
+    const size_t m = std::max(std::max(n_words, n), src ? bytes / 16 : 0);
     hipLaunchKernelGGL(k_frame_reset, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + 255) / 256, 1024))),
-                       dim3(256), 0, s, total, words, n_words, flags, n, v);
+                       dim3(256), 0, s, total, words, n_words, flags, n, v, src, dst, bytes);
 }
 
 void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s, const unsigned* count)
