@@ -475,7 +475,6 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
     for (auto& t : tslot) FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
     iter_total.reserve(1);
-    h_iter_total.reserve(1);
     stager = std::thread([this] { stage_loop(); });
 }
 
@@ -897,8 +896,7 @@ void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* 
     while (classes[ci].get() != &c) ci++;
     const uint32_t* idx = cand.p + c.off + lo;
     fi_poi* rep = repair.p + c.off + lo;
-    launch_gather(tmpl.p + c.off, idx, k, rep, stream, count_dev);
-    launch_mark(idx, k, redo.p + c.off, stream, count_dev);
+    launch_gather(tmpl.p + c.off, idx, k, rep, stream, count_dev, redo.p + c.off);
     fftcc_pass(c, rep, k, idx, count_dev);
     icgn_pass(c, rep, k, idx, count_dev);
     launch_merge_repair(cur.p + c.off, rep, idx, k, counters.p + ci, stream, count_dev);
@@ -1032,10 +1030,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
                 if (classes[ci]->n > 0)
                     repair_class(*classes[ci], repair_cap(*classes[ci]), 0, &counters.p[ci].candidates);
         }
-        launch_qa(cur.p, cnt, counters.p + C, stream);
+        launch_qa(cur.p, cnt, counters.p + C, stream, iter_total.p);
         FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
-                              stream));
-        FI_HIP(hipMemcpyAsync(h_iter_total.p, iter_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                               stream));
         FI_HIP(hipGetLastError());
     };
@@ -1140,10 +1136,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     }
     if (redo_qa) {
         launch_zero_words(reinterpret_cast<uint32_t*>(counters.p + C), sizeof(FrameCounters) / 4, stream);
-        launch_qa(cur.p, cnt, counters.p + C, stream);
+        launch_qa(cur.p, cnt, counters.p + C, stream, iter_total.p);
         FI_HIP(hipGetLastError());
-        FI_HIP(hipMemcpyAsync(h_iter_total.p, iter_total.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                              stream));
         qa = read_counters(nullptr);
     }
     FI_HIP(hipEventRecord(ev[4], stream));
@@ -1163,7 +1157,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     st.bad_ratio = bad_ratio;
     st.valid_count = (int64_t)qa.valid;
     st.valid_ratio = cnt ? (float)qa.valid / (float)cnt : 0.f;
-    st.iteration_sum = (int64_t)*h_iter_total.p;
+    st.iteration_sum = (int64_t)qa.iterations;
     st.iteration_mean = qa.iter_finite ? (float)((double)qa.iter_sum / (double)qa.iter_finite) : 0.f;
     st.iteration_max = (int32_t)qa.iter_max;
     float ms = 0.f;

@@ -157,7 +157,6 @@ struct fi_frame {
     fi::PinnedArray<fi::FrameCounters> h_counters;
     fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry
     fi::DevArray<unsigned long long> iter_total;
-    fi::PinnedArray<unsigned long long> h_iter_total;
     fi::DevArray<int> queue;       // persistent-queue claim counters, kQueueWords per class
     fi::PinnedArray<fi_poi> h_out;
     fi::DevArray<uint8_t> redo;    // per POI: re-run from FFTCC in the last frame

@@ -123,12 +123,15 @@ void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
                  FrameCounters* counters, uint32_t* candidates, hipStream_t s);
 // count (optional, device): the launch covers min(*count, n) entries; n is
 // the capacity the grid is sized for
+// flags (optional): flags[idx[j]] = 1 for every gathered entry
 void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s,
-                   const unsigned* count = nullptr);
+                   const unsigned* count = nullptr, uint8_t* flags = nullptr);
 void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s);
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
                          FrameCounters* counters, hipStream_t s, const unsigned* count = nullptr);
-void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s);
+// iter_total (optional): copied into counters->iterations
+void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s,
+               const unsigned long long* iter_total = nullptr);
 // learned-seed fallback candidates (see k_policy.hip)
 void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float threshold, FrameCounters* counters,
                       uint32_t* candidates, hipStream_t s);
@@ -137,8 +140,7 @@ void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, h
                       const unsigned* count = nullptr);
 // p[0..n) = 0 (32-bit words), by a kernel
 void launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
-// flags[idx[j]] = 1, j < n; p[0..n) = v (bytes)
-void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s, const unsigned* count = nullptr);
+// p[0..n) = v (bytes)
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
 // *total = 0, words[0..n_words) = 0, flags[0..n) = v: one launch
 void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,

@@ -41,6 +41,7 @@ struct FrameCounters {
     unsigned long long recovered;    // repaired POIs that are no longer bad
     unsigned int iter_max;
     unsigned int candidates;         // POI-level repair candidates written
+    unsigned long long iterations;   // the frame's executed iterations, copied by k_qa
 };
 
 }  // namespace fi

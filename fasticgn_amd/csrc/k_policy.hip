@@ -162,10 +162,14 @@ __device__ __forceinline__ int bounded(int n, const unsigned* count)
 }
 
 __global__ void k_gather(const fi_poi* __restrict__ src, const uint32_t* __restrict__ idx, int n,
-                         fi_poi* __restrict__ dst, const unsigned* count)
+                         fi_poi* __restrict__ dst, const unsigned* count, uint8_t* __restrict__ flags)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < bounded(n, count)) dst[j] = src[idx[j]];
+    if (j < bounded(n, count)) {
+        const uint32_t i = idx[j];
+        dst[j] = src[i];
+        if (flags) flags[i] = 1u;
+    }
 }
 
 __global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restrict__ scratch, int n)
@@ -192,8 +196,11 @@ __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restric
 // computeFrameQa (:629-700): valid = finite zncc >= 0; iteration statistics
 // over finite iteration counts; bad per isBadPoiForFallback.
 __global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict__ cur, int n,
-                                                        FrameCounters* counters)
+                                                        FrameCounters* counters,
+                                                        const unsigned long long* __restrict__ iter_total)
 {
+    // the iteration total rides along with the counters (one read-back)
+    if (iter_total && blockIdx.x == 0 && threadIdx.x == 0) counters->iterations = *iter_total;
     // grid-stride: a few hundred blocks, so the five counters see one atomic
     // per block (same-address atomics serialise)
     unsigned v[4] = { 0u, 0u, 0u, 0u }, mx[1] = { 0u };
@@ -242,13 +249,6 @@ __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n)
         p[i] = 0u;
 }
 
-// flags[idx[j]] = 1 for j < n
-__global__ void k_mark(const uint32_t* __restrict__ idx, int n, uint8_t* __restrict__ flags, const unsigned* count)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < bounded(n, count)) flags[idx[j]] = 1u;
-}
-
 __global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v)
 {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -284,11 +284,6 @@ void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_wor
     const size_t m = std::max(std::max(n_words, n), src ? bytes / 16 : 0);
     hipLaunchKernelGGL(k_frame_reset, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + 255) / 256, 1024))),
                        dim3(256), 0, s, total, words, n_words, flags, n, v, src, dst, bytes);
-}
-
-void launch_mark(const uint32_t* idx, int n, uint8_t* flags, hipStream_t s, const unsigned* count)
-{
-    if (n > 0) hipLaunchKernelGGL(k_mark, grid_for(n), dim3(256), 0, s, idx, n, flags, count);
 }
 
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s)
@@ -328,9 +323,9 @@ void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float thresh
 }
 
 void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s,
-                   const unsigned* count)
+                   const unsigned* count, uint8_t* flags)
 {
-    if (n > 0) hipLaunchKernelGGL(k_gather, grid_for(n), dim3(256), 0, s, src, idx, n, dst, count);
+    if (n > 0) hipLaunchKernelGGL(k_gather, grid_for(n), dim3(256), 0, s, src, idx, n, dst, count, flags);
 }
 
 void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s)
@@ -346,11 +341,11 @@ void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* id
                            counters, count);
 }
 
-void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s)
+void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s,
+               const unsigned long long* iter_total)
 {
-    if (n > 0)
-        hipLaunchKernelGGL(k_qa, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks)),
-                           dim3(kPolicyThreads), 0, s, cur, n, counters);
+    hipLaunchKernelGGL(k_qa, dim3(std::max(1, std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks))),
+                       dim3(kPolicyThreads), 0, s, cur, n, counters, iter_total);
 }
 
 void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s,
