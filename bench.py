@@ -336,7 +336,8 @@ def main() -> int:
         return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported by this build "
-              "(the path shards over POIs; see DESIGN.md)", file=sys.stderr)
+              "(the path would shard over POIs: every GPU holding its shard's reference-side state and "
+              "receiving each frame's target)", file=sys.stderr)
         return 2
     if args.workload == "config2":
         return bench_config2(args)

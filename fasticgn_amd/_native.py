@@ -182,8 +182,7 @@ def load() -> ctypes.CDLL:
         return _lib
     path = Path(os.environ.get("FASTICGN_LIB", LIB_PATH))
     if not path.exists():
-        raise ImportError(f"{path} is not built; run __graft_entry__.build() "
-                          "(make -C fasticgn_amd/csrc)")
+        raise ImportError(f"{path} is not built; build it with make -C fasticgn_amd/csrc")
     lib = ctypes.CDLL(str(path))
     for name, args in SIGNATURES.items():
         fn = getattr(lib, name)
