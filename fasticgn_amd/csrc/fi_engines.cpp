@@ -24,9 +24,18 @@ int safe_int(float v)
 
 std::string size_str(int w, int h) { return std::to_string(w) + "x" + std::to_string(h); }
 
+// The IC-GN kernels read the reference, gradient and target images through
+// buffer resources with 32-bit byte offsets.
+void check_image_offsets(int w, int h)
+{
+    if ((size_t)w * (size_t)h * sizeof(float) > (size_t)INT_MAX)
+        fail(FI_E_UNSUPPORTED, "image " + size_str(w, h) + " exceeds the 2 GiB f32 image limit of IC-GN");
+}
+
 // IC-GN launch with an optional explicit lane-to-POI shape
 void icgn_launch(const IcgnParams& p, int waves, int spl, hipStream_t s)
 {
+    check_image_offsets(p.w, p.h);
     const std::string sub = std::to_string(2 * p.rx + 1) + "x" + std::to_string(2 * p.ry + 1);
     if (waves == 0) {
         if (!launch_icgn(p, s)) fail(FI_E_UNSUPPORTED, "no IC-GN lane shape fits a " + sub + " subset");
@@ -293,6 +302,7 @@ fi_icgn::~fi_icgn()
 
 void fi_icgn::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
 {
+    check_image_offsets(w, h);
     DeviceGuard g(device);
     ref_raw.upload(px, w, h, stride, pixel, depth12, stream);
     grads_ready = false;
@@ -574,6 +584,7 @@ void fi_frame::invalidate_graphs()
 
 void fi_frame::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
 {
+    check_image_offsets(w, h);
     DeviceGuard g(device);
     ref_raw.upload(px, w, h, stride, pixel, depth12, stream);
     ref_set = true;

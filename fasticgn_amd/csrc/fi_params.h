@@ -102,6 +102,8 @@ struct IcgnParams {
     const float* offset;      // per POI (indexed like pois) an (x, y) pair: subset centre
                               // offset of compute(POI2D*, Point2D&), nullptr = none
     int queue_blocks;
+    double conv2;             // set by the launcher: the iteration continues while the FP64
+                              // squared norm is >= conv2, i.e. (float)sqrt(norm^2) >= conv
 };
 
 }  // namespace fi

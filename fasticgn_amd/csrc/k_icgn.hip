@@ -291,9 +291,25 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 // kTileMargin on each side, rows kTileStride words apart, when the image is
 // that large, the tile at most kTileStride wide (radius <= 28) and the f32
 // target is given; otherwise the kernel samples the B-spline table.
+// (float)sqrt(s2) >= c, the reference's loop test on the FP64 squared norm
+// s2 (src/oc_icgn.cpp:343-352), as one FP64 compare s2 >= conv_threshold2(c):
+// for c > 0, (float)sqrt(s2) >= c exactly when sqrt(s2) reaches the midpoint
+// m of c and the float below it (an exact tie aside), and m, with at most 25
+// significant bits, squares exactly in FP64.  c = 0 passes every norm, c < 0
+// every norm but NaN, c = NaN none.
+static double conv_threshold2(float c)
+{
+    if (c != c) return (double)c;
+    if (c < 0.f) return -HUGE_VAL;
+    if (c == 0.f) return 0.0;
+    const double m = 0.5 * ((double)nextafterf(c, 0.f) + (double)c);
+    return m * m;
+}
+
 static IcgnParams with_tile(const IcgnParams& p)
 {
     IcgnParams q = p;
+    q.conv2 = conv_threshold2(p.conv);
     q.tile_w = q.tile_h = q.tile_stride = 0;
     if (p.tar == nullptr) return q;
     const int tw = 2 * p.rx + 4 + 2 * kTileMargin, th = 2 * p.ry + 4 + 2 * kTileMargin;

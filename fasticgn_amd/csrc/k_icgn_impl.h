@@ -100,6 +100,14 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t table_rsrc(const float* coef, 
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(coef), (short)0, (int)bytes, (int)kBufferWord3);
 }
 
+// One float of an f32 image read through a raw buffer resource: the lane's
+// byte offset in a VGPR, a wave-uniform row offset as the scalar offset, so a
+// run of rows costs no per-load address arithmetic.
+__device__ __forceinline__ float ld1(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+}
+
 __device__ __forceinline__ float4 ld4(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff)
 {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0);
@@ -237,21 +245,44 @@ __device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx,
 }
 
 // Coefficients of pixels tx0 + 1 .. tx0 + tw - 3 of rows ty0 .. ty0 + th - 1
-// (the image pixels tx0 .. tx0 + tw - 1 they read lie in the image).
-__device__ __forceinline__ void stage_coef(float4* ct, const float* tar, int w, int tx0, int ty0, int tw, int th,
-                                           int tid, int nt)
+// (the image pixels tx0 .. tx0 + tw - 1 they read lie in the image).  Lane =
+// column (tw - 3 <= 64), wave = row modulo the block's waves; three rows per
+// pass, so twelve loads are in flight before the first store.
+__device__ __forceinline__ float4 row_coef(float q0, float q1, float q2, float q3)
 {
-    const int cols = tw - 3, n = cols * th;
-    for (int e = tid; e < n; e += nt) {
-        const int r = e / cols, c = e - r * cols;
-        const float* q = tar + (size_t)(ty0 + r) * w + (tx0 + c);
-        const float q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
-        float4 v;
-        v.x = q1;
-        v.y = __builtin_fmaf(bc(2, 3), q3, __builtin_fmaf(bc(2, 2), q2, __builtin_fmaf(bc(2, 1), q1, bc(2, 0) * q0)));
-        v.z = __builtin_fmaf(bc(1, 3), q3, __builtin_fmaf(bc(1, 2), q2, __builtin_fmaf(bc(1, 1), q1, bc(1, 0) * q0)));
-        v.w = __builtin_fmaf(bc(0, 3), q3, __builtin_fmaf(bc(0, 2), q2, __builtin_fmaf(bc(0, 1), q1, bc(0, 0) * q0)));
-        ct[r * kGroupStride + c] = v;
+    float4 v;
+    v.x = q1;
+    v.y = __builtin_fmaf(bc(2, 3), q3, __builtin_fmaf(bc(2, 2), q2, __builtin_fmaf(bc(2, 1), q1, bc(2, 0) * q0)));
+    v.z = __builtin_fmaf(bc(1, 3), q3, __builtin_fmaf(bc(1, 2), q2, __builtin_fmaf(bc(1, 1), q1, bc(1, 0) * q0)));
+    v.w = __builtin_fmaf(bc(0, 3), q3, __builtin_fmaf(bc(0, 2), q2, __builtin_fmaf(bc(0, 1), q1, bc(0, 0) * q0)));
+    return v;
+}
+
+template <int WAVES>
+__device__ __forceinline__ void stage_coef(float4* ct, const float* tar, int w, int tx0, int ty0, int tw, int th,
+                                           int lane, int wave)
+{
+    if (lane >= tw - 3) return;
+    constexpr int kRows = 3;
+    const float* q = tar + (size_t)(ty0 + wave) * w + (tx0 + lane);
+    float4* d = ct + wave * kGroupStride + lane;
+    const size_t step = (size_t)WAVES * w;
+    int r = wave;
+    for (; r + (kRows - 1) * WAVES < th; r += kRows * WAVES) {
+        float v[kRows][4];
+#pragma unroll
+        for (int j = 0; j < kRows; j++)
+#pragma unroll
+            for (int m = 0; m < 4; m++) v[j][m] = q[j * step + m];
+#pragma unroll
+        for (int j = 0; j < kRows; j++) d[j * WAVES * kGroupStride] = row_coef(v[j][0], v[j][1], v[j][2], v[j][3]);
+        q += kRows * step;
+        d += kRows * WAVES * kGroupStride;
+    }
+    for (; r < th; r += WAVES) {
+        *d = row_coef(q[0], q[1], q[2], q[3]);
+        q += step;
+        d += WAVES * kGroupStride;
     }
 }
 
@@ -530,13 +561,19 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     // Every lane loads (idle lanes and repeated rows address real subset
     // samples; a generic shape's slots past R repeat row R-1), then masks: no
     // divergent branch, so all loads are in flight together.
+    // The images are read as buffers: the host caps w * h * 4 below 2^31, and
+    // every slot addresses a subset sample inside the image (entry gate).
     float rz[SPL], gxv[SPL], gyv[SPL];
+    const uint32_t pitch = (uint32_t)w * 4u, img_bytes = pitch * (uint32_t)h;
+    const __amdgpu_buffer_rsrc_t rs_ref = table_rsrc(p.ref, img_bytes);
+    const __amdgpu_buffer_rsrc_t rs_gx = table_rsrc(p.gx, img_bytes), rs_gy = table_rsrc(p.gy, img_bytes);
+    const uint32_t ro = (uint32_t)((uly + bd.row0) * w + (ulx + bd.col)) * 4u;
+    const uint32_t go = (uint32_t)((py - ry + bd.row0) * w + (px - rx + bd.col)) * 4u;
 #pragma unroll
     for (int k = 0; k < SPL; k++) {
-        const int row = bd.row0 + (EXACT ? k : min(k, R - 1));
-        const float rv = p.ref[(size_t)(uly + row) * w + (ulx + bd.col)];
-        const size_t gi = (size_t)(py + row - ry) * w + (px + bd.col - rx);
-        const float gxl = p.gx[gi], gyl = p.gy[gi];
+        const uint32_t so = (uint32_t)(EXACT ? k : min(k, R - 1)) * pitch;
+        const float rv = ld1(rs_ref, ro, so);
+        const float gxl = ld1(rs_gx, go, so), gyl = ld1(rs_gy, go, so);
         const bool valid = bd.active && !(bd.last && k < ovl) && (EXACT || k < R);
         rz[k] = valid ? rv - ref_mean : 0.f;
         gxv[k] = valid ? gxl : 0.f;
@@ -571,7 +608,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
 
     int it = 0;
-    float conv = 0.f, zncc = 0.f;
+    float zncc = 0.f;
+    double cn;
     do {
         it++;
         // sample k of the lane sits at (xb + k W01, yb + k W11): the POI
@@ -724,15 +762,16 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         const float N22 = W20 * I02 + (W21 * I12 + W22 * I22);
         W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12; W20 = N20; W21 = N21; W22 = N22;
 
-        // convergence norm (:343-352), accumulated in FP64
-        double cn = (double)dp[0] * dp[0];
+        // convergence norm (:343-352), accumulated in FP64; the loop compares
+        // its square with the launcher's threshold, the root is taken once
+        cn = (double)dp[0] * dp[0];
         cn += (double)dp[1] * dp[1] * rx2;
         cn += (double)dp[2] * dp[2] * ry2;
         cn += (double)dp[3] * dp[3];
         cn += (double)dp[4] * dp[4] * rx2;
         cn += (double)dp[5] * dp[5] * ry2;
-        conv = (float)sqrt(cn);
-    } while ((float)it < stop && conv >= conv_crit);
+    } while ((float)it < stop && cn >= p.conv2);
+    const float conv = (float)sqrt(cn);
 
     if (tid == 0) {
         if (p.iters) p.iters[b] = (uint32_t)it;
@@ -914,7 +953,7 @@ void k_icgn_group(IcgnParams p)
             gh = Y1 - Y0 + 1;
         }
     }
-    if (gw > 0) stage_coef(ctile, p.tar, p.w, X0, Y0, gw, gh, threadIdx.x, 64 * kGroupPois);
+    if (gw > 0) stage_coef<kGroupPois>(ctile, p.tar, p.w, X0, Y0, gw, gh, lane, wave);
     __syncthreads();
     if (b < p.n)
         icgn_entry<1, SPL, EXACT, true>(p, b, red, flags + wave, reinterpret_cast<float*>(ctile), X0, Y0, gw, gh);
