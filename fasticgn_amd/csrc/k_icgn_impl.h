@@ -154,16 +154,34 @@ __device__ __forceinline__ float row4(const float* q, float w0, float w1, float 
     return __builtin_fmaf(w3, q[3], __builtin_fmaf(w2, q[2], __builtin_fmaf(w1, q[1], w0 * q[0])));
 }
 
+// bias + sum_n w_n(d) r_n for the four neighbourhood rows, from the factored
+// weights above: with P = d (1 - d), d21 = r2 - r1 and w_n = [lerp part] +
+// P (a_n d + b_n),
+//     sum = r1 + d d21 + P (d A + B),   A = (24/56)(r0 - r3) + (384/336) d21,
+//     B = -(33/56) r0 + (318/336) r1 - (66/336) r2 - (9/56) r3
+// (a_1 = -a_2 and a_3 = -a_0 give A; the b_n are the constant terms): thirteen
+// operations with the bias, where the four weights and their combination take
+// fourteen and the bias one more.
+__device__ __forceinline__ float ycombine(float d, float r0, float r1, float r2, float r3, float bias)
+{
+    const float P = __builtin_fmaf(-d, d, d);
+    const float d21 = r2 - r1;
+    const float A = __builtin_fmaf(24.f / 56.f, r0 - r3, (384.f / 336.f) * d21);
+    const float B = __builtin_fmaf(-9.f / 56.f, r3,
+                                   __builtin_fmaf(-66.f / 336.f, r2, __builtin_fmaf(318.f / 336.f, r1, (-33.f / 56.f) * r0)));
+    return __builtin_fmaf(P, __builtin_fmaf(d, A, B), __builtin_fmaf(d, d21, r1 + bias));
+}
+
 // LDS row stride of the per-POI tile (tiles up to 64 pixels wide, radius <=
 // 28): a neighbourhood's four rows lie 0, 64, 128 and 192 words from one
 // address, so its 16 pixels are eight ds_read2_b32 with immediate offsets.
 constexpr int kTileStride = 64;
 
-// Sample at (gx, gy) from a tile whose element (0, 0) is image pixel
+// bias + sample at (gx, gy) from a tile whose element (0, 0) is image pixel
 // (tx0, ty0); cbase = (ty0 + 1) * kTileStride + tx0 + 1 places the
 // neighbourhood's first row and column.  The caller guarantees the
 // neighbourhood lies in the tile.
-__device__ __forceinline__ float tile_sample(const float* tile, int cbase, float gx, float gy)
+__device__ __forceinline__ float tile_sample(const float* tile, int cbase, float gx, float gy, float bias)
 {
     const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
     const int base = ((int)gy << 6) + (int)gx - cbase;
@@ -172,22 +190,21 @@ __device__ __forceinline__ float tile_sample(const float* tile, int cbase, float
     const float* q1 = q0 + kTileStride;
     const float* q2 = q1 + kTileStride;
     const float* q3 = q2 + kTileStride;
-    float x0, x1, x2, x3, y0, y1, y2, y3;
+    float x0, x1, x2, x3;
     spline_weights(dx, x0, x1, x2, x3);
-    spline_weights(dy, y0, y1, y2, y3);
     const float r0 = row4(q0, x0, x1, x2, x3), r1 = row4(q1, x0, x1, x2, x3);
     const float r2 = row4(q2, x0, x1, x2, x3), r3 = row4(q3, x0, x1, x2, x3);
-    return __builtin_fmaf(y3, r3, __builtin_fmaf(y2, r2, __builtin_fmaf(y1, r1, y0 * r0)));
+    return ycombine(dy, r0, r1, r2, r3, bias);
 }
 
-// Sample at (gx, gy) from the f32 target in global memory (rows w floats
+// bias + sample at (gx, gy) from the f32 target in global memory (rows w floats
 // apart) with the tile path's weights: an iteration whose neighbourhoods
 // outgrow the LDS tile, when no coefficient table was built.  The caller's
 // range test (1 <= gx < w-2, 1 <= gy < h-2) keeps the 4x4 neighbourhood in the
 // image; a row's four pixels are one unaligned 16-byte load.
 typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 
-__device__ __forceinline__ float target_sample(const float* tar, uint32_t w, float gx, float gy)
+__device__ __forceinline__ float target_sample(const float* tar, uint32_t w, float gx, float gy, float bias)
 {
     const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
     const float* q0 = tar + ((size_t)((uint32_t)(int)gy - 1u) * w + ((uint32_t)(int)gx - 1u));
@@ -195,13 +212,12 @@ __device__ __forceinline__ float target_sample(const float* tar, uint32_t w, flo
     const f32x4u b = *reinterpret_cast<const f32x4u*>(q0 + w);
     const f32x4u c = *reinterpret_cast<const f32x4u*>(q0 + 2 * (size_t)w);
     const f32x4u d = *reinterpret_cast<const f32x4u*>(q0 + 3 * (size_t)w);
-    float x0, x1, x2, x3, y0, y1, y2, y3;
+    float x0, x1, x2, x3;
     spline_weights(dx, x0, x1, x2, x3);
-    spline_weights(dy, y0, y1, y2, y3);
     auto row = [&](const f32x4u q) {
         return __builtin_fmaf(x3, q[3], __builtin_fmaf(x2, q[2], __builtin_fmaf(x1, q[1], x0 * q[0])));
     };
-    return __builtin_fmaf(y3, row(d), __builtin_fmaf(y2, row(c), __builtin_fmaf(y1, row(b), y0 * row(a))));
+    return ycombine(dy, row(a), row(b), row(c), row(d), bias);
 }
 
 // ---------------------------------------------------------------------------
@@ -226,11 +242,12 @@ constexpr int kGroupRowSlack = 8;  // rows of vertical spread the four neighbour
 #endif
 constexpr int kGroupMargin = FI_GROUP_MARGIN;
 
-// Entry (row, col) of a tile whose pixel (0, 0) is image pixel (tx0, ty0)
+// bias + sample at (gx, gy).  Entry (row, col) of a tile whose pixel (0, 0)
+// is image pixel (tx0, ty0)
 // holds the polynomial of pixel column tx0 + 1 + col; cb = -(ty0 + 1) *
 // kGroupStride - (tx0 + 1) places the sample's first row.  The caller
 // guarantees the neighbourhood lies in the tile.
-__device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx, float gy)
+__device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx, float gy, float bias)
 {
     const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
     const float4* q = ct + (__mul24((int)gy, kGroupStride) + (int)gx + cb);
@@ -239,9 +256,7 @@ __device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx,
     };
     const float r0 = horner(q[0]), r1 = horner(q[kGroupStride]);
     const float r2 = horner(q[2 * kGroupStride]), r3 = horner(q[3 * kGroupStride]);
-    float y0, y1, y2, y3;
-    spline_weights(dy, y0, y1, y2, y3);
-    return __builtin_fmaf(y3, r3, __builtin_fmaf(y2, r2, __builtin_fmaf(y1, r1, y0 * r0)));
+    return ycombine(dy, r0, r1, r2, r3, bias);
 }
 
 // Coefficients of pixels tx0 + 1 .. tx0 + tw - 3 of rows ty0 .. ty0 + th - 1
@@ -672,8 +687,9 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         }
         const int cbase = GROUP ? -(ty0 + 1) * kGroupStride - (tx0 + 1) : (ty0 + 1) * tile_stride + tx0 + 1;
 
-        // pass 1: sample; t' per slot, and the sums of t', t'^2, t' r (a
-        // repeated row counts once: its t' is masked, its r is zero)
+        // pass 1: sample; t' = t - ref_mean per slot (the samplers subtract
+        // it), and the sums of t', t'^2, t' r (a repeated row counts once: its
+        // t' is masked, its r is zero)
         float tpv[SPL];
         float st = 0.f, stt = 0.f, str = 0.f;
         auto pass1 = [&](auto sample) {
@@ -683,7 +699,7 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
                 if (EXACT || k < R) {
                     const float gxp = __builtin_fmaf((float)k, W01, xb);
                     const float gyp = __builtin_fmaf((float)k, W11, yb);
-                    const float tp = sample(gxp, gyp, k) - ref_mean;
+                    const float tp = sample(gxp, gyp, k);
                     const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
                     tpv[k] = tp;
                     st += tpm;
@@ -695,14 +711,16 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         if (use_tile) {
             if constexpr (GROUP)
                 pass1([&](float gx, float gy, int) {
-                    return coef_sample(reinterpret_cast<const float4*>(tile), cbase, gx, gy);
+                    return coef_sample(reinterpret_cast<const float4*>(tile), cbase, gx, gy, -ref_mean);
                 });
             else
-                pass1([&](float gx, float gy, int) { return tile_sample(tile, cbase, gx, gy); });
+                pass1([&](float gx, float gy, int) { return tile_sample(tile, cbase, gx, gy, -ref_mean); });
         } else if (p.coef)
-            pass1([&](float gx, float gy, int) { return bspline_sample(table, row_bytes, plane_bytes, gx, gy); });
+            pass1([&](float gx, float gy, int) {
+                return bspline_sample(table, row_bytes, plane_bytes, gx, gy) - ref_mean;
+            });
         else
-            pass1([&](float gx, float gy, int) { return target_sample(p.tar, (uint32_t)w, gx, gy); });
+            pass1([&](float gx, float gy, int) { return target_sample(p.tar, (uint32_t)w, gx, gy, -ref_mean); });
         // lanes mirroring band 0 drop their t' sums (their r is zero)
         float s1[3] = { st * m_all, stt * m_all, str };
         block_sum<WAVES>(s1, red, wave, lane);
