@@ -506,7 +506,7 @@ constexpr int kMaxOverlap = 4;
 // iteration, which the register budget of the plain kernels cannot afford).
 template <int WAVES, int SPL, bool EXACT, bool GROUP = false, bool OFFSET = false>
 __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, float* red, int* flags, float* tile,
-                                           int gx0 = 0, int gy0 = 0, int gw = 0, int gh = 0)
+                                           float* hl, int gx0 = 0, int gy0 = 0, int gw = 0, int gh = 0)
 {
     static_assert(!GROUP || WAVES == 1, "a shared coefficient tile serves one-wave entries");
     constexpr int NT = 64 * WAVES;
@@ -549,6 +549,11 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     const PoiPrep* pr = p.prep + (p.prep_index ? p.prep_index[i] : (uint32_t)i);
     const float ref_mean = pr->mean, ref_norm = pr->norm;
     const float inv_ref_norm = 1.f / ref_norm;
+    // the wave's copy of the inverse Hessian (hl: 36 floats of LDS per wave),
+    // from which lane a < 6 forms dp[a] each iteration; written and read by
+    // this wave only
+    hl += 36 * wave;
+    if (lane < 36) hl[lane] = pr->hinv[lane];
 
     const Bands bd = make_bands(rx, ry, NT, tid);
     const int R = bd.R, ovl = bd.ovl;
@@ -752,15 +757,17 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         float bnum[6] = { A, xl * A, __builtin_fmaf(yl0, A, AK), B, xl * B, __builtin_fmaf(yl0, B, BK) };
         block_sum<WAVES>(bnum, red, wave, lane);
 
-        // dp = H^-1 b (src/oc_icgn.cpp:324-331)
+        // dp = H^-1 b (src/oc_icgn.cpp:324-331): lane a < 6 forms dp[a] from
+        // its row (lanes above repeat row 5), the wave reads the six back
+        float dpl = 0.f;
+        {
+            const float* hr = hl + 6 * min(lane, 5);
+#pragma unroll
+            for (int c = 0; c < 6; c++) dpl = __builtin_fmaf(hr[c], bnum[c], dpl);
+        }
         float dp[6];
 #pragma unroll
-        for (int a = 0; a < 6; a++) {
-            float d = 0.f;
-#pragma unroll
-            for (int c = 0; c < 6; c++) d = __builtin_fmaf(pc->hinv[a * 6 + c], bnum[c], d);
-            dp[a] = d;
-        }
+        for (int a = 0; a < 6; a++) dp[a] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dpl), a));
         // W <- W * W(dp)^-1 (:333-340); the 3x3 inverse by cofactors down column 0
         const float D00 = 1.f + dp[1], D01 = dp[2], D02 = dp[0];
         const float D10 = dp[4], D11 = 1.f + dp[5], D12 = dp[3];
@@ -830,11 +837,12 @@ void k_icgn(IcgnParams p)
 {
     __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
     __shared__ int flags[WAVES];
+    __shared__ float hl[WAVES * 36];
     extern __shared__ float tile[];
     // one call site: the entry is inlined once
     const int b = p.count ? (int)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
     const int n = p.count ? min((int)*p.count, p.n) : p.n;
-    if (b < n) icgn_entry<WAVES, SPL, EXACT, false, OFFSET>(p, b, red, flags, tile);
+    if (b < n) icgn_entry<WAVES, SPL, EXACT, false, OFFSET>(p, b, red, flags, tile, hl);
 }
 
 template <int WAVES, int SPL, bool EXACT>
@@ -843,6 +851,7 @@ void k_icgn_queue(IcgnParams p)
 {
     __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
     __shared__ int flags[WAVES];
+    __shared__ float hl[WAVES * 36];
     // claim[0]: the entry the block runs next, -1 once every range is
     // drained; claim[1]: end of the chunk it came from; claim[2]: ranges
     // thread 0 has passed since its own.  The walk state lives here rather
@@ -883,7 +892,7 @@ void k_icgn_queue(IcgnParams p)
         const int b = __builtin_amdgcn_readfirstlane(claim[0]);
         __syncthreads();  // every thread has read the entry before it is rewritten
         if (b < 0) break;
-        icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile);
+        icgn_entry<WAVES, SPL, EXACT>(p, b, red, flags, tile, hl);
         __syncthreads();  // shared state is reused by the next entry
     }
 }
@@ -939,6 +948,7 @@ void k_icgn_group(IcgnParams p)
     __shared__ int gbox[kGroupPois][4];
     __shared__ float red[1];
     __shared__ int flags[kGroupPois];
+    __shared__ float hl[kGroupPois * 36];
     extern __shared__ float4 ctile[];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int b = xcd_remap(blockIdx.x, gridDim.x) * kGroupPois + wave;
@@ -974,7 +984,8 @@ void k_icgn_group(IcgnParams p)
     if (gw > 0) stage_coef<kGroupPois>(ctile, p.tar, p.w, X0, Y0, gw, gh, lane, wave);
     __syncthreads();
     if (b < p.n)
-        icgn_entry<1, SPL, EXACT, true>(p, b, red, flags + wave, reinterpret_cast<float*>(ctile), X0, Y0, gw, gh);
+        icgn_entry<1, SPL, EXACT, true>(p, b, red, flags + wave, reinterpret_cast<float*>(ctile), hl + 36 * wave, X0,
+                                        Y0, gw, gh);
 }
 
 // Instantiated lane shapes (waves, rows per band).  EXACT_A/B are compiled in
