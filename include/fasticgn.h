@@ -163,6 +163,9 @@ int fi_icgn_set_shape(fi_icgn* h, int waves, int rows_per_band);
  * target in device memory.  One-wave lane shapes only; others run as AUTO. */
 #define FI_SAMPLING_GROUP 2
 int fi_icgn_set_sampling(fi_icgn* h, int mode);
+/* setRefImage / setTarImage.  The IC-GN kernels address the f32 images with
+ * 32-bit offsets: a reference whose width * height * 4 bytes exceeds 2 GiB
+ * returns FI_E_UNSUPPORTED (the reference itself has no such limit). */
 int fi_icgn_set_ref(fi_icgn* h, const void* px, int width, int height,
                     size_t stride_bytes, int pixel, int depth12);
 int fi_icgn_set_tar(fi_icgn* h, const void* px, int width, int height,
@@ -190,6 +193,8 @@ int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* cent
 #define FI_POLICY_DEFAULT (FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK)
 int fi_frame_create(int radius, float conv, float stop, int device, fi_frame** out);
 int fi_frame_destroy(fi_frame* h);
+/* Reference image of the sequence; as fi_icgn_set_ref, a reference above
+ * 2 GiB as f32 returns FI_E_UNSUPPORTED. */
 int fi_frame_set_ref(fi_frame* h, const void* px, int width, int height,
                      size_t stride_bytes, int pixel, int depth12);
 /* POI template (buildPoiQueue, :2439-2472, or any explicit list). */

@@ -236,6 +236,20 @@ def test_icgn_other_radii_parity(oracle, c0, radius):
     assert cmp["pass"], cmp
 
 
+def test_icgn_rejects_images_beyond_the_offset_range():
+    """The IC-GN kernels address the f32 images with 32-bit offsets: a
+    reference above 2 GiB as f32 (23171^2 pixels) is refused when it is set,
+    before any pixel is read or uploaded; 23170^2 is the largest square."""
+    assert 23170 * 23170 * 4 <= 2**31 - 1 < 23171 * 23171 * 4
+    big = fa.Image2D(np.zeros((23171, 23171), np.uint8))  # calloc: never touched
+    with fa.ICGN2D1(15, 15, 0.001, 10) as ic:
+        with pytest.raises(NotImplementedError, match="2 GiB"):
+            ic.setRefImage(big)
+    with fa.FrameDriver(15) as d:
+        with pytest.raises(NotImplementedError, match="2 GiB"):
+            d.set_reference(big)
+
+
 def test_icgn_edge_cases(oracle, c0):
     """OOB band, zero texture (-5), entry gate: statuses identical, and the
     fields the reference leaves alone are left alone."""
