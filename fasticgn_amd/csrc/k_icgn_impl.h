@@ -938,7 +938,7 @@ __device__ __forceinline__ bool entry_box(const IcgnParams& p, int b, int& nx0, 
 
 // kGroupPois consecutive entries per block, one wave each, sharing one
 // coefficient tile that covers their incoming neighbourhoods plus
-// kTileMargin (p.tile_h = rows allocated; a union that does not fit leaves the
+// kGroupMargin (p.tile_h = rows allocated; a union that does not fit leaves the
 // block without a tile).  The two barriers come before any wave starts its
 // entry.
 template <int SPL, bool EXACT>
