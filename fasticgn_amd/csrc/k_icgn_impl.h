@@ -58,15 +58,24 @@ struct Bands {
     bool last;    // the lane is in the last band and that band overlaps
 };
 
+// a / b for 0 <= a < 2^21 and b > 0: (a + 0.5) / b lies at least 0.5 / b from
+// an integer, and the float product is within (a / b) 2^-22 of it, so the
+// truncation is the quotient (a reciprocal and a multiply where the integer
+// division takes four quarter-rate multiplies).
+__device__ __forceinline__ int div_small(int a, int b)
+{
+    return (int)(((float)a + 0.5f) * __builtin_amdgcn_rcpf((float)b));
+}
+
 __device__ __forceinline__ Bands make_bands(int rx, int ry, int nt, int tid)
 {
     Bands b;
     const int sw = 2 * rx + 1, sh = 2 * ry + 1;
-    const int G = nt / sw;
-    b.R = (sh + G - 1) / G;
-    const int gu = (sh + b.R - 1) / b.R;
+    const int G = div_small(nt, sw);
+    b.R = div_small(sh + G - 1, G);
+    const int gu = div_small(sh + b.R - 1, b.R);
     b.ovl = gu * b.R - sh;
-    int g = tid / sw;
+    int g = div_small(tid, sw);
     b.col = tid - g * sw;
     b.active = g < gu;
     if (!b.active) g = 0;
@@ -568,8 +577,9 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     float W20 = 0.f, W21 = 0.f, W22 = 1.f;
     const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
     const float rxf = (float)rx, ryf = (float)ry;
-    const float rx2 = (float)(rx * rx), ry2 = (float)(ry * ry);
-    const float inv_ns = 1.f / (float)((2 * rx + 1) * (2 * ry + 1));
+    // small integers: the float products are exact
+    const float rx2 = (float)rx * (float)rx, ry2 = (float)ry * (float)ry;
+    const float inv_ns = 1.f / ((float)(2 * rx + 1) * (float)(2 * ry + 1));
     const float conv_crit = p.conv, stop = p.stop;
 
     // The LDS tile holds the raw target around the warped subset; it is
@@ -587,8 +597,9 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     const uint32_t pitch = (uint32_t)w * 4u, img_bytes = pitch * (uint32_t)h;
     const __amdgpu_buffer_rsrc_t rs_ref = table_rsrc(p.ref, img_bytes);
     const __amdgpu_buffer_rsrc_t rs_gx = table_rsrc(p.gx, img_bytes), rs_gy = table_rsrc(p.gy, img_bytes);
-    const uint32_t ro = (uint32_t)((uly + bd.row0) * w + (ulx + bd.col)) * 4u;
-    const uint32_t go = (uint32_t)((py - ry + bd.row0) * w + (px - rx + bd.col)) * 4u;
+    // rows and widths below 2^24: 24-bit multiplies
+    const uint32_t ro = (__umul24((uint32_t)(uly + bd.row0), (uint32_t)w) + (uint32_t)(ulx + bd.col)) * 4u;
+    const uint32_t go = (__umul24((uint32_t)(py - ry + bd.row0), (uint32_t)w) + (uint32_t)(px - rx + bd.col)) * 4u;
 #pragma unroll
     for (int k = 0; k < SPL; k++) {
         const uint32_t so = (uint32_t)(EXACT ? k : min(k, R - 1)) * pitch;
