@@ -119,6 +119,7 @@ SIGNATURES = {
     "fi_device_synchronize": [_I],
     "fi_last_error": [ctypes.c_char_p, _S],
     "fi_dft_twiddles": [_I, _P],
+    "fi_conv_threshold2": [_F, ctypes.POINTER(ctypes.c_double)],
     "fi_fftcc_create": [_I, _I, _I, ctypes.POINTER(_P)],
     "fi_fftcc_destroy": [_P],
     "fi_fftcc_set_ref": [_P, _P, _I, _I, _S, _I, _I],

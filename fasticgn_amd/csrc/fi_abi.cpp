@@ -94,6 +94,14 @@ int fi_dft_twiddles(int n, float* out)
     });
 }
 
+int fi_conv_threshold2(float conv, double* out)
+{
+    return guarded([&] {
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        *out = conv_threshold2(conv);
+    });
+}
+
 // ---- FFTCC2D ----
 int fi_fftcc_create(int rx, int ry, int device, fi_fftcc** out)
 {

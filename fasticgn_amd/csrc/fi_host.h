@@ -116,6 +116,10 @@ struct DeviceGuard {
 // each rounded from double.
 void dft_twiddles(int n, float* out);
 
+// The IC-GN loop continues while its FP64 squared convergence norm s2 is >=
+// conv_threshold2(conv), which is (float)sqrt(s2) >= conv (k_icgn.hip).
+double conv_threshold2(float conv);
+
 // Frame-policy kernels (k_policy.hip).
 void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, fi_poi* out, int n,
                                hipStream_t s);

@@ -297,7 +297,7 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 // m of c and the float below it (an exact tie aside), and m, with at most 25
 // significant bits, squares exactly in FP64.  c = 0 passes every norm, c < 0
 // every norm but NaN, c = NaN none.
-static double conv_threshold2(float c)
+double conv_threshold2(float c)
 {
     if (c != c) return (double)c;
     if (c < 0.f) return -HUGE_VAL;

@@ -106,6 +106,12 @@ int fi_last_error(char* buf, size_t len);
 /* The FFTCC DFT twiddle table for length n (2n floats: cos, -sin), as the
  * device kernels use it.  For tests of the DFT specification. */
 int fi_dft_twiddles(int n, float* out);
+/* The FP64 threshold the IC-GN kernels compare their squared convergence norm
+ * with (the loop test of src/oc_icgn.cpp:343-352 without a root per
+ * iteration): (float)sqrt(s2) >= conv exactly when s2 >= *out, an exact tie
+ * of sqrt(s2) with the rounding midpoint below conv aside.  Host only; for
+ * tests of the loop test. */
+int fi_conv_threshold2(float conv, double* out);
 
 /* ---- FFTCC2D -------------------------------------------------------------
  * FFTCC2D(int rx, int ry, int threads)    src/oc_fftcc.cpp:154-170
