@@ -67,6 +67,8 @@ class FrameStats(ctypes.Structure):
         ("seed_pois", ctypes.c_int64),
         ("seed_fallback_count", ctypes.c_int64),
         ("seed_ms", ctypes.c_double),
+        ("temporal_seed_count", ctypes.c_int32),
+        ("temporal_seed_used", ctypes.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -160,6 +162,9 @@ SIGNATURES = {
     "fi_frame_download_redo": [_P, _P, _S],
     "fi_frame_set_fused": [_P, _I],
     "fi_frame_autotune": [_P, _P, _S, ctypes.POINTER(_S)],
+    "fi_frame_records": [_P, ctypes.POINTER(_P), ctypes.POINTER(_S)],
+    "fi_frame_set_records": [_P, _I],
+    "fi_frame_set_temporal_extrapolation": [_P, _I, _F],
     "fi_seeder_create": [_I, ctypes.POINTER(_P)],
     "fi_seeder_destroy": [_P],
     "fi_seeder_param_count": [ctypes.POINTER(_S)],

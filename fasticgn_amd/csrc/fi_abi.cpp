@@ -301,6 +301,27 @@ int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n)
     return guarded([&] { need(h); h->download_redo(out, n); });
 }
 
+int fi_frame_records(fi_frame* h, const void** records, size_t* n_records)
+{
+    return guarded([&] { need(h); h->records(records, n_records); });
+}
+
+int fi_frame_set_records(fi_frame* h, int on)
+{
+    return guarded([&] { need(h); h->set_records(on != 0); });
+}
+
+int fi_frame_set_temporal_extrapolation(fi_frame* h, int on, float max_delta)
+{
+    return guarded([&] {
+        need(h);
+        if (!(max_delta >= 0.f)) fail(FI_E_INVALID_ARGUMENT, "temporal max_delta must be >= 0 (0 = the radius)");
+        h->temporal = on != 0;
+        h->temporal_max_delta = max_delta;
+        h->invalidate_graphs();
+    });
+}
+
 int fi_frame_reset_sequence(fi_frame* h)
 {
     return guarded([&] { need(h); h->reset_sequence(); });
@@ -478,9 +499,15 @@ int fi_format_raw(const fi_poi* pois, size_t n, void* out, size_t out_len, size_
         if (n > 0 && (!pois || !out)) fail(FI_E_INVALID_ARGUMENT, "null buffer");
         if (out_len < n * 36) fail(FI_E_INVALID_ARGUMENT, "output buffer shorter than n * 36 bytes");
         unsigned char* c = static_cast<unsigned char*>(out);
+        // (int32_t) of a coordinate, defined for every float: truncation, and
+        // INT32_MIN for NaN and values outside the int32 range (what the x86
+        // conversion instruction returns; the device packer does the same)
+        auto coord = [](float v) -> int32_t {
+            return (v >= -2147483648.f && v < 2147483648.f) ? (int32_t)v : INT32_MIN;
+        };
         for (size_t i = 0; i < n; i++) {
             const fi_poi& p = pois[i];
-            const int32_t x = (int32_t)p.x, y = (int32_t)p.y;
+            const int32_t x = coord(p.x), y = coord(p.y);
             const float f[7] = { p.deformation[0], p.deformation[6], p.deformation[1], p.deformation[2],
                                  p.deformation[7], p.deformation[8], p.result[2] };
             std::memcpy(c, &x, 4);

@@ -484,7 +484,11 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
     FI_HIP(hipEventCreate(&join_ev));
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
     for (auto& t : tslot) FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+    rec_stream = make_stream();
+    for (auto& e : rec_ev) FI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    FI_HIP(hipEventCreateWithFlags(&pack_ev, hipEventDisableTiming));
     iter_total.reserve(1);
+    threshold.reserve(1);
     stager = std::thread([this] { stage_loop(); });
 }
 
@@ -501,6 +505,13 @@ fi_frame::~fi_frame()
         if (e) (void)hipEventDestroy(e);
     for (auto& t : tslot)
         if (t.ready) (void)hipEventDestroy(t.ready);
+    for (auto& e : rec_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (pack_ev) (void)hipEventDestroy(pack_ev);
+    if (rec_stream) {
+        (void)hipStreamSynchronize(rec_stream);
+        (void)hipStreamDestroy(rec_stream);
+    }
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
     if (aux_stream) (void)hipStreamDestroy(aux_stream);
@@ -642,6 +653,8 @@ void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pi
 void fi_frame::reset_sequence()
 {
     have_prev = false;
+    have_prev2 = false;
+    rec_frame = -1;
     baseline_ready = false;
     baseline_bad = NAN;
     frame_index = 0;
@@ -712,6 +725,7 @@ void fi_frame::prepare()
     seed_ok.reserve(cap);
     cur.reserve(cap);
     prev.reserve(cap);
+    prev2.reserve(cap);
     save.reserve(cap);
     repair.reserve(cap);
     cand.reserve(cap);
@@ -721,6 +735,16 @@ void fi_frame::prepare()
     h_counters.reserve(nclass() + 1);
     queue.reserve((size_t)kQueueWords * std::max<size_t>(nclass(), 1));
     if (n > 0) FI_HIP(hipMemcpyAsync(tmpl.p, lay, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    std::vector<unsigned> sizes(nclass());
+    for (size_t c = 0; c < nclass(); c++) sizes[c] = (unsigned)classes[c]->n;
+    class_n.reserve(std::max<size_t>(nclass(), 1));
+    if (!sizes.empty())
+        FI_HIP(hipMemcpyAsync(class_n.p, sizes.data(), sizes.size() * sizeof(unsigned), hipMemcpyHostToDevice,
+                              stream));
+    if (!order.empty()) {
+        d_order.reserve(n);
+        FI_HIP(hipMemcpyAsync(d_order.p, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    }
     for (auto& cp : classes) {
         FrameClass& c = *cp;
         std::vector<float> t(4 * c.r);
@@ -956,11 +980,12 @@ bool fi_frame::table_needed() const
     return false;
 }
 
-// processFrame, opencorr_cli_cuda_mvp.cpp:3024-3210.  The device work up to
-// the frame's decisions (target prefilter, seeding, IC-GN of every radius
-// class, fallback scan, QA, counter read-back) is one segment, run eagerly or
-// replayed as a captured hipGraph; the rare fallback and repair steps after it
-// run eagerly.
+// processFrame, opencorr_cli_cuda_mvp.cpp:3024-3210.  The device work of a
+// frame (target prefilter, seeding, IC-GN of every radius class, fallback scan,
+// the whole-frame fallback decision and, when taken, the from-scratch pass and
+// per-POI merge, POI-level repair, QA, counter read-back) is one segment, run
+// eagerly or replayed as a captured hipGraph.  Only repair candidates past a
+// class's repair_cap run after it, eagerly.
 void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 {
     DeviceGuard g(device);
@@ -983,6 +1008,18 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     const bool learned = !skip && (policy & FI_POLICY_LEARNED_SEED) && seeder != nullptr;
     const int collect = (policy & FI_POLICY_POI_FALLBACK) ? 1 : 0;
     const int mode = learned ? 0 : (skip ? 2 : 1);
+    // the frame before the previous one, for the two-frame extrapolation
+    const fi_poi* hist2 = (mode == 2 && temporal && have_prev2) ? prev2.p : nullptr;
+    const float max_delta = temporal_max_delta > 0.f ? temporal_max_delta : (float)r;
+    if (mode == 2) {
+        // the device decides the whole-frame fallback against baseline + margin
+        const float thr = baseline_bad + kBadRatioMargin;
+        if (!(thr == threshold_host)) {
+            threshold_host = thr;
+            FI_HIP(hipMemcpyAsync(threshold.p, &threshold_host, sizeof(float), hipMemcpyHostToDevice, stream));
+            FI_HIP(hipStreamSynchronize(stream));
+        }
+    }
 
     auto segment = [&](bool timed) {
         // a fill kernel, not memsets: this runs inside captured frame graphs
@@ -1013,7 +1050,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         } else if (mode == 1) {
             fftcc_all(cur.p);
         } else {
-            launch_seed_from_previous(tmpl.p, prev.p, cur.p, cnt, stream);
+            launch_seed_from_previous(tmpl.p, prev.p, hist2, max_delta, cur.p, cnt, counters.p + C, stream);
         }
         if (table) FI_HIP(hipStreamWaitEvent(stream, join_ev, 0));
         if (timed) FI_HIP(hipEventRecord(ev[2], stream));
@@ -1028,18 +1065,40 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
                 launch_scan(cur.p + c.off, prev.p + c.off, (int)c.n, collect, counters.p + ci, cand.p + c.off,
                             stream);
         }
-        // The FFTCC repair of the scan's candidates, sized for repair_cap and
-        // bounded by the scan's count on the device (no host round trip;
-        // candidates past the cap are repaired after the segment): always for
-        // learned seeds; speculatively for temporal seeds, whose whole-frame
-        // fallback (decided on the host) then starts again from the
-        // unrepaired results kept in `save`.
+        if (mode == 2) {
+            // bad ratio against the threshold: sets the fallback flag and the
+            // per-class repair and fallback counts the launches below read
+            launch_decide(counters.p, C, class_n.p, cnt, threshold.p, collect, stream);
+            // the skip attempt, kept for the per-POI merge
+            FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
+        }
+        // POI-level repair (:2955-3022) of the scan's candidates, sized for
+        // repair_cap and bounded by a count on the device (candidates past the
+        // cap are repaired after the segment): always for learned seeds; for
+        // temporal seeds unless the frame falls back
         if (mode == 0 || (mode == 2 && collect)) {
-            if (mode == 2)
-                FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
             for (int ci = 0; ci < C; ci++)
                 if (classes[ci]->n > 0)
-                    repair_class(*classes[ci], repair_cap(*classes[ci]), 0, &counters.p[ci].candidates);
+                    repair_class(*classes[ci], repair_cap(*classes[ci]), 0,
+                                 mode == 0 ? &counters.p[ci].candidates : &counters.p[ci].repair_n);
+        }
+        if (mode == 2) {
+            // whole-frame fallback with per-POI merge (:3115-3152), gated on the
+            // device: the template again, FFTCC on the cached spectra, IC-GN,
+            // then shouldTakeFrameFallbackResult against the skip attempt.  A
+            // frame that does not fall back launches these with zero entries.
+            const unsigned* flag = &counters.p[C].fallback;
+            launch_copy_if(tmpl.p, cur.p, n * sizeof(fi_poi), flag, stream);
+            for (int ci = 0; ci < C; ci++)
+                if (classes[ci]->n > 0)
+                    fftcc_pass(*classes[ci], cur.p + classes[ci]->off, (int)classes[ci]->n, nullptr,
+                               &counters.p[ci].fallback_n);
+            for (int ci = 0; ci < C; ci++)
+                if (classes[ci]->n > 0)
+                    icgn_pass(*classes[ci], cur.p + classes[ci]->off, (int)classes[ci]->n, nullptr,
+                              &counters.p[ci].fallback_n);
+            launch_merge_frame_fallback(save.p, cur.p, cnt, stream, flag);
+            launch_fill_bytes(redo.p, n, 1, stream, flag);
         }
         launch_qa(cur.p, cnt, counters.p + C, stream, iter_total.p);
         FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
@@ -1048,10 +1107,11 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     };
 
     if (use_graph) {
-        const uint64_t key[8] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
-                                  (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tview.data,
-                                  (uint64_t)(uintptr_t)(table ? coef.p : nullptr), (uint64_t)(uintptr_t)tar_f.p,
-                                  (uint64_t)(uintptr_t)seeder };
+        const uint64_t key[10] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
+                                   (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tview.data,
+                                   (uint64_t)(uintptr_t)(table ? coef.p : nullptr), (uint64_t)(uintptr_t)tar_f.p,
+                                   (uint64_t)(uintptr_t)seeder, (uint64_t)(uintptr_t)hist2,
+                                   (uint64_t)(uint32_t)__builtin_bit_cast(uint32_t, max_delta) };
         hipGraphExec_t exec = nullptr;
         for (auto& gr : graphs)
             if (std::memcmp(gr.key, key, sizeof(key)) == 0) exec = gr.exec;
@@ -1109,26 +1169,17 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         const float ratio = cnt ? (float)bad / (float)cnt : 0.f;
         st.skip_fftcc_used = 1;
         st.pre_repair_bad_ratio = ratio;
-        const float threshold = baseline_bad + kBadRatioMargin;
-        if (ratio > threshold) {
-            // whole-frame fallback with per-POI merge (:3115-3152), against the
-            // results before any POI-level repair (the segment kept them in
-            // `save` when it repaired)
+        st.temporal_seed_count = (int32_t)pc[C].temporal;
+        st.temporal_seed_used = pc[C].temporal > 0 ? 1 : 0;
+        const bool fell_back = pc[C].fallback != 0;
+        if (fell_back != (ratio > threshold_host))
+            fail(FI_E_DEVICE, "whole-frame fallback decision differs between device and host");
+        if (fell_back) {
             st.fallback_used = 1;
-            if (!collect)
-                FI_HIP(hipMemcpyAsync(save.p, cur.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
-            FI_HIP(hipMemcpyAsync(cur.p, tmpl.p, n * sizeof(fi_poi), hipMemcpyDeviceToDevice, stream));
-            fftcc_all(cur.p);
-            icgn_all(cur.p);
-            launch_merge_frame_fallback(save.p, cur.p, cnt, stream);
-            launch_fill_bytes(redo.p, n, 1, stream);
-            FI_HIP(hipGetLastError());
             st.fftcc_pois = cnt;
-            redo_qa = true;
         } else if (collect && m_all > 0) {
-            // POI-level repair on cached spectra and resident Hessians
-            // (:2955-3022): the segment repaired up to repair_cap candidates
-            // per class, the rest here
+            // the segment repaired up to repair_cap candidates per class, the
+            // rest here
             std::vector<FrameCounters> rc(pc);
             for (int ci = 0; ci < C; ci++) {
                 const int m = (int)pc[ci].candidates, cap = repair_cap(*classes[ci]);
@@ -1151,6 +1202,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         FI_HIP(hipGetLastError());
         qa = read_counters(nullptr);
     }
+    if (records_on) queue_records(cur.p);
     FI_HIP(hipEventRecord(ev[4], stream));
     last_slot = ts;
     FI_HIP(hipEventSynchronize(ev[4]));
@@ -1191,11 +1243,64 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         st.icgn_ms = ms;
     }
 
+    // history: prev2 <- prev, prev <- this frame, cur <- the oldest buffer
+    std::swap(prev2.p, prev.p);
+    std::swap(prev2.cap, prev.cap);
     std::swap(cur.p, prev.p);
     std::swap(cur.cap, prev.cap);
+    have_prev2 = have_prev;
     have_prev = true;
     frame_index++;
     if (stats) *stats = st;
+}
+
+// Packs the raw records of src (the frame just run, layout order) on the
+// compute stream and copies them to the next pinned slot on rec_stream, which
+// overlaps the next frame.  A slot is packed again only after its previous
+// copy finished (rec_ev).
+void fi_frame::queue_records(const fi_poi* src)
+{
+    const int s = rec_next;
+    rec_next ^= 1;
+    const size_t bytes = n * (size_t)36;
+    d_rec[s].reserve(std::max<size_t>(bytes, 36));
+    h_rec[s].reserve(std::max<size_t>(bytes, 36));
+    FI_HIP(hipStreamWaitEvent(stream, rec_ev[s], 0));
+    launch_pack_raw(src, order.empty() ? nullptr : d_order.p, (int)n, d_rec[s].p, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipEventRecord(pack_ev, stream));
+    FI_HIP(hipStreamWaitEvent(rec_stream, pack_ev, 0));
+    if (bytes) FI_HIP(hipMemcpyAsync(h_rec[s].p, d_rec[s].p, bytes, hipMemcpyDeviceToHost, rec_stream));
+    FI_HIP(hipEventRecord(rec_ev[s], rec_stream));
+    rec_last = s;
+    rec_frame = frame_index;
+}
+
+void fi_frame::set_records(bool on)
+{
+    records_on = on;
+}
+
+void fi_frame::records(const void** out, size_t* count)
+{
+    if (!have_prev) fail(FI_E_LOGIC, "no frame has been run");
+    if (out == nullptr) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");
+    DeviceGuard g(device);
+    if (rec_frame != frame_index - 1) {
+        // not packed by its run: pack the latest results (prev after the rotation)
+        const int keep = frame_index;
+        frame_index--;
+        try {
+            queue_records(prev.p);
+        } catch (...) {
+            frame_index = keep;
+            throw;
+        }
+        frame_index = keep;
+    }
+    FI_HIP(hipEventSynchronize(rec_ev[rec_last]));
+    *out = h_rec[rec_last].p;
+    if (count) *count = n;
 }
 
 // Autotuner: per radius class, every covering lane shape x sampling mode x

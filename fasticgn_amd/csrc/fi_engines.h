@@ -150,7 +150,9 @@ struct fi_frame {
     std::vector<fi_poi> tmpl_host;        // template in the caller's order
     std::vector<uint32_t> order;          // layout position -> template index (empty: identity)
     std::vector<std::unique_ptr<FrameClass>> classes;
-    fi::DevArray<fi_poi> tmpl, cur, prev, save, repair;
+    // cur: the frame being run; prev, prev2: the two frames before it
+    // (rotated after each frame); save: the skip attempt before repairs
+    fi::DevArray<fi_poi> tmpl, cur, prev, prev2, save, repair;
     fi::DevArray<uint32_t> cand, seed_ok;
     // counters[c] for the scans and repairs of class c, counters[classes] for QA
     fi::DevArray<fi::FrameCounters> counters;
@@ -160,14 +162,30 @@ struct fi_frame {
     fi::DevArray<int> queue;       // persistent-queue claim counters, kQueueWords per class
     fi::PinnedArray<fi_poi> h_out;
     fi::DevArray<uint8_t> redo;    // per POI: re-run from FFTCC in the last frame
+    fi::DevArray<unsigned> class_n;  // class sizes (k_decide)
+    fi::DevArray<float> threshold;   // baseline bad ratio + margin (k_decide)
+    fi::DevArray<uint32_t> d_order;  // layout position -> template index, on the device
+    // Two-frame temporal extrapolation of the seeds (seedPoiQueueFromPrevious
+    // with --temporal-extrapolation), steps up to temporal_max_delta px.
+    bool temporal = false;
+    float temporal_max_delta = 0.f;
+    // Raw records of each frame (36 B, template order): packed on the device
+    // after the frame and copied to pinned memory on rec_stream, two slots.
+    bool records_on = false;
+    fi::DevArray<uint8_t> d_rec[2];
+    fi::PinnedArray<uint8_t> h_rec[2];
+    hipEvent_t rec_ev[2] = {}, pack_ev = nullptr;
+    hipStream_t rec_stream = nullptr;
+    int rec_next = 0, rec_last = -1, rec_frame = -1;
     size_t n = 0;
     bool ref_set = false, pois_set = false, prepared = false, tar_ready = false;
-    bool have_prev = false, baseline_ready = false;
+    bool have_prev = false, have_prev2 = false, baseline_ready = false;
     float baseline_bad = NAN;
+    float threshold_host = NAN;      // the value last written to `threshold`
     int frame_index = 0;
     hipEvent_t ev[8] = {};
     struct Graph {
-        uint64_t key[8];
+        uint64_t key[10];
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;     // instantiated frame graphs (cleared on any change)
@@ -180,6 +198,10 @@ struct fi_frame {
     void upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12);
     void run(unsigned policy, fi_frame_stats* stats);
     void download(fi_poi* out, size_t n);
+    // The last frame's raw records in pinned host memory (valid until the
+    // second run after it); packs and copies them now if the frame did not.
+    void records(const void** out, size_t* count);
+    void set_records(bool on);
     void download_redo(uint8_t* out, size_t n);
     void reset_sequence();
     void invalidate_graphs();
@@ -190,6 +212,7 @@ struct fi_frame {
 
 private:
     void prepare();
+    void queue_records(const fi_poi* src);
     void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev,
                     const unsigned* count_dev = nullptr);
     void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index,

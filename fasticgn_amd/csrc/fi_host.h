@@ -121,8 +121,19 @@ void dft_twiddles(int n, float* out);
 double conv_threshold2(float conv);
 
 // Frame-policy kernels (k_policy.hip).
-void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, fi_poi* out, int n,
-                               hipStream_t s);
+// prev2 (optional): the frame before prev, for the two-frame extrapolation
+// (steps up to max_delta px); frame->temporal counts the extrapolated seeds.
+void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, const fi_poi* prev2, float max_delta,
+                               fi_poi* out, int n, FrameCounters* frame, hipStream_t s);
+// Whole-frame fallback decision (see k_policy.hip): counters[0..nclass) hold
+// the classes' scans, counters[nclass] the frame entry; class_n the class
+// sizes; threshold (device) the baseline bad ratio + margin.
+void launch_decide(FrameCounters* counters, int nclass, const unsigned* class_n, int n, const float* threshold,
+                   int collect, hipStream_t s);
+// dst = src (bytes, a multiple of 4) when *gate != 0
+void launch_copy_if(const void* src, void* dst, size_t bytes, const unsigned* gate, hipStream_t s);
+// 36-byte raw records of src[0..n) at out + 36 * order[i] (order null: i)
+void launch_pack_raw(const fi_poi* src, const uint32_t* order, int n, void* out, hipStream_t s);
 void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
                  FrameCounters* counters, uint32_t* candidates, hipStream_t s);
 // count (optional, device): the launch covers min(*count, n) entries; n is
@@ -130,7 +141,9 @@ void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
 // flags (optional): flags[idx[j]] = 1 for every gathered entry
 void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s,
                    const unsigned* count = nullptr, uint8_t* flags = nullptr);
-void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s);
+// gate (optional, device): merge only when *gate != 0
+void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s,
+                                 const unsigned* gate = nullptr);
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
                          FrameCounters* counters, hipStream_t s, const unsigned* count = nullptr);
 // iter_total (optional): copied into counters->iterations
@@ -144,8 +157,8 @@ void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, h
                       const unsigned* count = nullptr);
 // p[0..n) = 0 (32-bit words), by a kernel
 void launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
-// p[0..n) = v (bytes)
-void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s);
+// p[0..n) = v (bytes); with gate (device), only when *gate != 0
+void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s, const unsigned* gate = nullptr);
 // *total = 0, words[0..n_words) = 0, flags[0..n) = v: one launch
 void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
                         uint8_t v, hipStream_t s, const void* src = nullptr, void* dst = nullptr, size_t bytes = 0);

@@ -42,6 +42,11 @@ struct FrameCounters {
     unsigned int iter_max;
     unsigned int candidates;         // POI-level repair candidates written
     unsigned long long iterations;   // the frame's executed iterations, copied by k_qa
+    // set by k_decide on the device (temporal frames):
+    unsigned int repair_n;           // per class: candidates to repair (0 under a whole-frame fallback)
+    unsigned int fallback_n;         // per class: POIs the whole-frame fallback re-runs (0 or the class size)
+    unsigned int fallback;           // frame entry: 1 when the whole-frame fallback ran
+    unsigned int temporal;           // frame entry: seeds set by temporal extrapolation
 };
 
 }  // namespace fi

@@ -1,8 +1,9 @@
-// Device-side frame policy of the multi-frame driver: temporal seeding, the
-// bad-POI scan with repair-candidate collection, the per-POI merges and the
-// frame QA counters.  The predicates restate opencorr_cli_cuda_mvp.cpp:483-558
-// and seedPoiQueueFromPrevious (:560-624, temporal extrapolation off); they
-// run where the POIs already live so a frame needs no host scan.
+// Device-side frame policy of the multi-frame driver: temporal seeding (with
+// the optional two-frame extrapolation), the bad-POI scan with repair-candidate
+// collection, the whole-frame fallback decision, the per-POI merges, the frame
+// QA counters and the raw output records.  The predicates restate
+// opencorr_cli_cuda_mvp.cpp:483-558 and seedPoiQueueFromPrevious (:560-624);
+// they run where the POIs already live so a frame needs no host scan.
 #include <hip/hip_runtime.h>
 
 #include "fi_host.h"
@@ -49,24 +50,6 @@ __device__ __forceinline__ bool is_candidate(const fi_poi& cur, const fi_poi* pr
     return prev->result[2] >= kGoodZncc;
 }
 
-__global__ void k_seed(const fi_poi* __restrict__ tmpl, const fi_poi* __restrict__ prev,
-                       fi_poi* __restrict__ out, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    fi_poi q = tmpl[i];
-    const fi_poi& pv = prev[i];
-    if (finite_f(pv.result[2]) && pv.result[2] >= 0.f) {
-#pragma unroll
-        for (int k = 0; k < 12; k++) q.deformation[k] = pv.deformation[k];
-#pragma unroll
-        for (int k = 0; k < 6; k++) q.result[k] = 0.f;
-#pragma unroll
-        for (int k = 0; k < 3; k++) q.strain[k] = 0.f;
-    }
-    out[i] = q;
-}
-
 // Block sums of per-thread counts: one atomic per block and counter instead
 // of one per POI (same-address atomics serialise in L2).
 constexpr int kPolicyThreads = 256;
@@ -102,6 +85,85 @@ __device__ __forceinline__ void block_reduce(unsigned (&v)[N], unsigned (&mx)[1]
         unsigned t = 0;
         for (int u = 0; u < kPolicyThreads / 64; u++) t = max(t, red[N][u]);
         mx[0] = t;
+    }
+}
+
+constexpr float kTemporalMinStep = 0.05f;  // kTemporalSeedMinStepPx, :481
+
+// isEligibleTemporalSeedHistory (:499-504)
+__device__ __forceinline__ bool eligible_history(const fi_poi& p)
+{
+    return finite_f(p.deformation[0]) && finite_f(p.deformation[6]) && finite_f(p.result[2]) &&
+           p.result[2] >= kGoodZncc;
+}
+
+// seedPoiQueueFromPrevious (:560-624): the template, with the twelve shape
+// parameters of the previous result where it is valid (finite zncc >= 0) and
+// its result and strain cleared; with prev2 (the frame before), a POI whose two
+// previous results are good at the template location moves on by their
+// difference in u, v when that step is at least kTemporalMinStep px and each
+// component at most max_delta.  Extrapolated seeds are counted into
+// frame->temporal.
+__global__ __launch_bounds__(kPolicyThreads) void k_seed(const fi_poi* __restrict__ tmpl,
+                                                          const fi_poi* __restrict__ prev,
+                                                          const fi_poi* __restrict__ prev2, float max_delta,
+                                                          fi_poi* __restrict__ out, int n, FrameCounters* frame)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool extrapolated = false;
+    if (i < n) {
+        fi_poi q = tmpl[i];
+        const fi_poi& pv = prev[i];
+        if (finite_f(pv.result[2]) && pv.result[2] >= 0.f) {
+#pragma unroll
+            for (int k = 0; k < 12; k++) q.deformation[k] = pv.deformation[k];
+            if (prev2) {
+                const fi_poi& p2 = prev2[i];
+                if (pv.x == p2.x && pv.y == p2.y && pv.x == q.x && pv.y == q.y && eligible_history(pv) &&
+                    eligible_history(p2)) {
+                    const float du = pv.deformation[0] - p2.deformation[0];
+                    const float dv = pv.deformation[6] - p2.deformation[6];
+                    const float pu = pv.deformation[0] + du, pw = pv.deformation[6] + dv;
+                    const float step2 = du * du + dv * dv;  // two roundings and a sum (no contraction)
+                    if (finite_f(pu) && finite_f(pw) && step2 >= kTemporalMinStep * kTemporalMinStep &&
+                        fabsf(du) <= max_delta && fabsf(dv) <= max_delta) {
+                        q.deformation[0] = pu;
+                        q.deformation[6] = pw;
+                        extrapolated = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 6; k++) q.result[k] = 0.f;
+#pragma unroll
+            for (int k = 0; k < 3; k++) q.strain[k] = 0.f;
+        }
+        out[i] = q;
+    }
+    if (prev2) {
+        unsigned v[1] = { extrapolated ? 1u : 0u }, mx[1] = { 0u };
+        block_reduce<1>(v, mx);
+        if (threadIdx.x == 0 && v[0]) atomicAdd(&frame->temporal, v[0]);
+    }
+}
+
+// processFrame's whole-frame fallback decision (:3086-3115) on the device:
+// bad ratio = (float)bad / (float)n over the classes' scans, compared with the
+// threshold (baseline + margin) kept in device memory.  Sets the frame flag and,
+// per class, how many repair candidates the POI-level repair takes (none under
+// a fallback) and how many POIs the fallback re-runs (the class, or none).
+__global__ void k_decide(FrameCounters* __restrict__ counters, int nclass, const unsigned* __restrict__ class_n,
+                         int n, const float* __restrict__ threshold, int collect)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    unsigned long long bad = 0;
+    for (int c = 0; c < nclass; c++) bad += counters[c].bad;
+    const float ratio = n ? (float)bad / (float)n : 0.f;
+    const bool fb = ratio > *threshold;
+    counters[nclass].fallback = fb ? 1u : 0u;
+    for (int c = 0; c < nclass; c++) {
+        counters[c].repair_n = (fb || !collect) ? 0u : counters[c].candidates;
+        counters[c].fallback_n = fb ? class_n[c] : 0u;
     }
 }
 
@@ -172,10 +234,11 @@ __global__ void k_gather(const fi_poi* __restrict__ src, const uint32_t* __restr
     }
 }
 
-__global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restrict__ scratch, int n)
+__global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restrict__ scratch, int n,
+                              const unsigned* gate)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (gate && *gate == 0u)) return;
     if (!take_frame_fallback(skip[i], scratch[i])) scratch[i] = skip[i];
 }
 
@@ -249,8 +312,9 @@ __global__ void k_zero_words(uint32_t* __restrict__ p, size_t n)
         p[i] = 0u;
 }
 
-__global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v)
+__global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v, const unsigned* gate)
 {
+    if (gate && *gate == 0u) return;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
@@ -286,11 +350,11 @@ void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_wor
                        dim3(256), 0, s, total, words, n_words, flags, n, v, src, dst, bytes);
 }
 
-void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s)
+void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s, const unsigned* gate)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_fill_bytes, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256), 0, s,
-                           p, n, v);
+                           p, n, v, gate);
 }
 
 void launch_zero_words(uint32_t* p, size_t n, hipStream_t s)
@@ -300,10 +364,74 @@ void launch_zero_words(uint32_t* p, size_t n, hipStream_t s)
                            p, n);
 }
 
-void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, fi_poi* out, int n,
-                               hipStream_t s)
+void launch_seed_from_previous(const fi_poi* tmpl, const fi_poi* prev, const fi_poi* prev2, float max_delta,
+                               fi_poi* out, int n, FrameCounters* frame, hipStream_t s)
 {
-    if (n > 0) hipLaunchKernelGGL(k_seed, grid_for(n), dim3(256), 0, s, tmpl, prev, out, n);
+    if (n > 0)
+        hipLaunchKernelGGL(k_seed, grid_for(n), dim3(kPolicyThreads), 0, s, tmpl, prev, prev2, max_delta, out, n,
+                           frame);
+}
+
+void launch_decide(FrameCounters* counters, int nclass, const unsigned* class_n, int n, const float* threshold,
+                   int collect, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_decide, dim3(1), dim3(64), 0, s, counters, nclass, class_n, n, threshold, collect);
+}
+
+// dst = src (bytes, a multiple of 4; both 16-byte aligned) when *gate != 0.
+__global__ void k_copy_if(const void* __restrict__ src, void* __restrict__ dst, size_t bytes,
+                          const unsigned* __restrict__ gate)
+{
+    if (*gate == 0u) return;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    const size_t n16 = bytes / 16;
+    const uint4* s4 = static_cast<const uint4*>(src);
+    uint4* d4 = static_cast<uint4*>(dst);
+    for (size_t i = t; i < n16; i += stride) d4[i] = s4[i];
+    const size_t tail = (bytes - 16 * n16) / 4;
+    if (t < tail) static_cast<uint32_t*>(dst)[4 * n16 + t] = static_cast<const uint32_t*>(src)[4 * n16 + t];
+}
+
+void launch_copy_if(const void* src, void* dst, size_t bytes, const unsigned* gate, hipStream_t s)
+{
+    if (bytes > 0)
+        hipLaunchKernelGGL(k_copy_if, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((bytes / 16 + 255) / 256, 1024))),
+                           dim3(256), 0, s, src, dst, bytes, gate);
+}
+
+// (int32_t) of a float coordinate as the x86 host converts it (truncation;
+// NaN and values outside the int32 range give INT32_MIN), so the device records
+// equal fi_format_raw's byte for byte.
+__device__ __forceinline__ int32_t coord_i32(float v)
+{
+    return (v >= -2147483648.f && v < 2147483648.f) ? (int32_t)v : INT32_MIN;
+}
+
+// formatRawBinary (:2498-2536, no output filter): the 36-byte record x, y
+// (int32), u, v, ux, uy, vx, vy, zncc (float32) of layout entry i at record
+// order[i] (the caller's template order; identity when order is null).
+__global__ void k_pack_raw(const fi_poi* __restrict__ src, const uint32_t* __restrict__ order, int n,
+                           uint32_t* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const fi_poi& p = src[i];
+    uint32_t* r = out + 9 * (size_t)(order ? order[i] : (uint32_t)i);
+    r[0] = (uint32_t)coord_i32(p.x);
+    r[1] = (uint32_t)coord_i32(p.y);
+    r[2] = __float_as_uint(p.deformation[0]);
+    r[3] = __float_as_uint(p.deformation[6]);
+    r[4] = __float_as_uint(p.deformation[1]);
+    r[5] = __float_as_uint(p.deformation[2]);
+    r[6] = __float_as_uint(p.deformation[7]);
+    r[7] = __float_as_uint(p.deformation[8]);
+    r[8] = __float_as_uint(p.result[2]);
+}
+
+void launch_pack_raw(const fi_poi* src, const uint32_t* order, int n, void* out, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_pack_raw, grid_for(n), dim3(256), 0, s, src, order, n, static_cast<uint32_t*>(out));
 }
 
 void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
@@ -328,9 +456,10 @@ void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, h
     if (n > 0) hipLaunchKernelGGL(k_gather, grid_for(n), dim3(256), 0, s, src, idx, n, dst, count, flags);
 }
 
-void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s)
+void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s,
+                                 const unsigned* gate)
 {
-    if (n > 0) hipLaunchKernelGGL(k_merge_frame, grid_for(n), dim3(256), 0, s, skip, scratch_inout, n);
+    if (n > 0) hipLaunchKernelGGL(k_merge_frame, grid_for(n), dim3(256), 0, s, skip, scratch_inout, n, gate);
 }
 
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,

@@ -58,6 +58,28 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))
         return out
 
+    def records(self) -> np.ndarray:
+        """The last frame's 36-byte output records (x, y, u, v, ux, uy, vx,
+        vy, zncc; rawio.RAW_DTYPE), packed on the device, as a copy."""
+        from .rawio import RAW_DTYPE
+        p = ctypes.c_void_p()
+        m = ctypes.c_size_t(0)
+        nat.call("fi_frame_records", self._h, ctypes.byref(p), ctypes.byref(m))
+        if m.value == 0:
+            return np.zeros(0, dtype=RAW_DTYPE)
+        buf = (ctypes.c_char * (36 * m.value)).from_address(p.value)
+        return np.frombuffer(buf, dtype=RAW_DTYPE).copy()
+
+    def set_records(self, on: bool = True) -> None:
+        """Pack every frame's records on the device and copy them to host
+        memory beside the next frame (records() then only waits)."""
+        nat.call("fi_frame_set_records", self._h, int(bool(on)))
+
+    def set_temporal_extrapolation(self, on: bool = True, max_delta: float = 0.0) -> None:
+        """Two-frame extrapolation of the temporal seeds (--temporal-extrapolation,
+        opencorr_cli_cuda_mvp.cpp:560-624); max_delta 0 = the driver's radius."""
+        nat.call("fi_frame_set_temporal_extrapolation", self._h, int(bool(on)), float(max_delta))
+
     def download_redo(self) -> np.ndarray:
         """Per POI of the last frame: True where it went through FFTCC."""
         out = np.zeros(self.n, dtype=np.uint8)

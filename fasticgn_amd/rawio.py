@@ -47,6 +47,16 @@ def write_raw(queue: np.ndarray, path) -> int:
     return n.value
 
 
+def format_raw(queue: np.ndarray) -> np.ndarray:
+    """The 36-byte records of a POI queue through fi_format_raw (RAW_DTYPE)."""
+    q = _check_queue(queue)
+    out = np.zeros(len(q), dtype=RAW_DTYPE)
+    n = ctypes.c_size_t(0)
+    if len(q):
+        nat.call("fi_format_raw", nat.ptr(q), len(q), nat.ptr(out), out.nbytes, ctypes.byref(n))
+    return out
+
+
 def _nullable(v, kind=float):
     if v is None:
         return None
@@ -61,7 +71,8 @@ class RawWriter:
 
     def __init__(self, output_dir, ref_path: str = "", tar_list_path: str = "", subset_radius: int = 0,
                  grid_space: int = 0, conv_threshold: float = 0.001, max_iteration: int = 10,
-                 policy: int | None = None, depth: str = "native", learned_seed: bool = False):
+                 policy: int | None = None, depth: str = "native", learned_seed: bool = False,
+                 temporal_extrapolation: bool = False):
         from .driver import DEFAULT_POLICY, POI_FALLBACK, SKIP_FFTCC
         self.dir = Path(output_dir)
         self.dir.mkdir(parents=True, exist_ok=True)
@@ -86,7 +97,8 @@ class RawWriter:
                 "fftcc_backend": "hip-dft",
                 "skip_fftcc": bool(policy & SKIP_FFTCC),
                 "poi_fallback": bool(policy & POI_FALLBACK),
-                "temporal_extrapolation": False,
+                "temporal_extrapolation": bool(temporal_extrapolation),
+                "frame_fallback_policy": "per-poi-merge-v1",
                 "learned_seed": bool(learned_seed),
             },
             "input_image": {"depth": depth, "align": "none", "normalize": "none"},
@@ -102,7 +114,15 @@ class RawWriter:
         k = len(self.frames)
         name = output_name or f"frame_{k:06d}.bin"
         t0 = time.perf_counter()
-        nbytes = write_raw(results, self.dir / name)
+        if results.dtype == RAW_DTYPE:
+            # records already packed (FrameDriver.records())
+            path = self.dir / name
+            tmp = path.with_suffix(path.suffix + ".tmp")
+            tmp.write_bytes(np.ascontiguousarray(results).tobytes())
+            os.replace(tmp, path)
+            nbytes = results.nbytes
+        else:
+            nbytes = write_raw(results, self.dir / name)
         t1 = time.perf_counter()
         qa = qa or {}
         entry = {
@@ -118,8 +138,9 @@ class RawWriter:
             "iteration_max": _nullable(qa.get("iteration_max"), int),
             "poi_fallback_count": _nullable(qa.get("poi_fallback_count"), int),
             "poi_fallback_recovered": _nullable(qa.get("poi_fallback_recovered"), int),
-            "temporal_seed_used": _nullable(qa.get("skip_fftcc_used"), bool),
-            "temporal_seed_count": (int(len(results)) if qa.get("skip_fftcc_used") else 0) if qa else None,
+            # seeds set by the two-frame extrapolation (cli:3063-3067, 675-678)
+            "temporal_seed_used": _nullable(qa.get("temporal_seed_used"), bool),
+            "temporal_seed_count": _nullable(qa.get("temporal_seed_count"), int),
         }
         self.frames.append(entry)
         self.flush()

@@ -88,6 +88,8 @@ typedef struct fi_frame_stats {
     int64_t seed_pois;            /* POIs seeded by the learned seeder */
     int64_t seed_fallback_count;  /* of those, re-run from FFTCC (low ZNCC or failure) */
     double seed_ms;               /* learned-seeder inference */
+    int32_t temporal_seed_count;  /* seeds set by the two-frame extrapolation (cli:3063-3067) */
+    int32_t temporal_seed_used;   /* temporal_seed_count > 0 */
 } fi_frame_stats;
 
 typedef struct fi_fftcc fi_fftcc;
@@ -216,13 +218,36 @@ int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n);
 int fi_frame_upload_target(fi_frame* h, const void* px, int width, int height,
                            size_t stride_bytes, int pixel, int depth12);
 /* Process the uploaded target with the given policy; results stay on the
- * device (fi_frame_download copies them out).  stats may be NULL. */
+ * device (fi_frame_download copies them out).  stats may be NULL.  On a frame
+ * seeded from the previous result, the whole-frame fallback (bad ratio above
+ * baseline + 0.03: FFTCC + IC-GN from the template, merged per POI by
+ * shouldTakeFrameFallbackResult, cli:3086-3152) is decided and run on the
+ * device, inside the frame's graph when graphs are on. */
 int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats);
 int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
 /* Per POI of the last frame (template order): 1 if it was run through FFTCC
  * (an FFTCC-seeded frame, the whole-frame fallback, POI-level repair or the
  * learned seed's fallback), else 0. */
 int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n);
+/* The last frame's output records (formatRawBinary, cli:2498-2536, no output
+ * filter): n 36-byte records x, y (int32), u, v, ux, uy, vx, vy, zncc
+ * (float32) in template order, packed on the device.  *records points into
+ * pinned memory the driver owns; it stays valid until the second fi_frame_run
+ * after the frame it holds.  With fi_frame_set_records on, every run packs its
+ * records and copies them to host memory on a copy stream that overlaps the
+ * next frame, and this call only waits for that copy; otherwise it packs and
+ * copies the last frame now.  *n_records (may be NULL) = n. */
+int fi_frame_records(fi_frame* h, const void** records, size_t* n_records);
+int fi_frame_set_records(fi_frame* h, int on);
+/* Two-frame temporal extrapolation of the seeds (seedPoiQueueFromPrevious
+ * with --temporal-extrapolation, cli:560-624, 3364-3371): on a frame seeded
+ * from the previous result, a POI whose last two results are good (finite,
+ * zncc >= 0.8) at its template location starts from u + (u - u_before),
+ * v + (v - v_before) when that step is at least 0.05 px and each component at
+ * most max_delta px (max_delta 0 = the driver's radius, as the CLI passes its
+ * subset radius).  Off by default, as in the reference.  fi_frame_stats
+ * counts the extrapolated seeds. */
+int fi_frame_set_temporal_extrapolation(fi_frame* h, int on, float max_delta);
 /* Forget the previous frames (next run seeds by FFTCC again). */
 int fi_frame_reset_sequence(fi_frame* h);
 /* IC-GN lane-to-POI shape for this driver (see fi_icgn_set_shape). */
@@ -236,8 +261,10 @@ int fi_frame_set_sampling(fi_frame* h, int mode);
  * template order).  Off: every POI uses the driver's radius. */
 int fi_frame_set_self_adaptive(fi_frame* h, int on);
 /* Capture each frame's device work (target prefilter, seeding, IC-GN of every
- * radius class, fallback scan, QA) as a hipGraph and replay it; the rare
- * fallback and repair steps still run eagerly.  Results are byte-identical to
+ * radius class, fallback scan and decision, whole-frame fallback, POI-level
+ * repair, QA) as a hipGraph and replay it; only repair candidates past the
+ * in-graph capacity (n / 32 per radius class, at least 1,024) run eagerly
+ * after it.  Results are byte-identical to
  * the eager frame; per-stage times in fi_frame_stats are NaN. */
 int fi_frame_set_graph(fi_frame* h, int on);
 /* IC-GN launch of every radius class: blocks > 0 runs the persistent

@@ -3,9 +3,11 @@
 Bars:
   FFTCC  integer peaks, zncc and every other POI field bit-identical to the
          oracle (both follow one DFT specification).
-  ICGN   validity flags (valid / -3 / -4 / -5) identical; u, v within the
-         reference's parity gate, per frame median <= 1e-3 px, nearest-rank
-         p99 <= 1e-2 px, max <= 0.5 px (smoke/check_cpu_gpu_parity.py:46-98).
+  ICGN   validity flags (valid / -3 / -4 / -5) identical; u, v and the
+         displacement gradients ux, uy, vx, vy within the reference's parity
+         gate, per frame median <= 1e-3, nearest-rank p99 <= 1e-2, max <= 0.5
+         (smoke/check_cpu_gpu_parity.py:46-98, tests/parity.py); iteration
+         counts and convergence agreement reported, and bounded on config 1.
 """
 import numpy as np
 import pytest
@@ -191,7 +193,8 @@ def test_icgn_center_offset_parity(oracle):
 
 def report(tag, cmp):
     keys = ["n", "cpu_valid", "gpu_valid", "status_mismatch", "median_du", "p99_du", "max_du",
-            "median_dv", "p99_dv", "max_dv"]
+            "median_dv", "p99_dv", "max_dv", "p99_dux", "p99_duy", "p99_dvx", "p99_dvy", "max_dux", "max_dvy",
+            "iteration_mismatch", "iteration_mismatch_frac", "max_diteration", "max_dconvergence"]
     say(tag, {k: cmp.get(k) for k in keys})
 
 
