@@ -18,6 +18,7 @@ x = X + u(X) is the forward displacement map, then rounded and clipped.
 from __future__ import annotations
 
 import math
+import os
 from dataclasses import dataclass, field
 from typing import Callable
 
@@ -264,11 +265,34 @@ def config2(size: int = 1024, frames: int = 500, grid: int = 5, seed: int = 2026
     r = np.array([radii[i % len(radii)] for i in range(len(pois))], dtype=np.float32)
     pois["subset_radius"][:, 0] = r
     pois["subset_radius"][:, 1] = r
-    meta = {"seed": seed, "n_dots": n_dots, "frames": frames, "grid": grid, "margin": max(radii) + 2,
+    meta = {"seed": seed, "size": size, "n_dots": n_dots, "frames": frames, "grid": grid, "margin": max(radii) + 2,
             "radii": list(radii), "radius_rule": "radii[i % 3]", "direction_deg": 30.0,
             "translation": "15 k / 499 px", "strain": "5e-3 k / 499 (ux = vy)", "noise_sigma": 24.0,
             "background": 640, "peak": 2880, "bit_depth": 12, "resample": "keys-cubic a=-0.5"}
     return Sequence(f"config2_{size}", ref, pois, frames, frame, truth, meta)
+
+
+_POOL_SEQ = None
+
+
+def _config2_frame(job):
+    """Frame k of config2(size, frames) (process-pool worker)."""
+    global _POOL_SEQ
+    size, frames, k = job
+    if _POOL_SEQ is None or _POOL_SEQ.meta.get("size") != size or _POOL_SEQ.frames != frames:
+        _POOL_SEQ = config2(size=size, frames=frames)
+    return _POOL_SEQ.frame(k)
+
+
+def config2_frames(size: int = 1024, frames: int = 500, ks=None, workers: int = 16) -> list:
+    """Frames ks (default all) of config2(size, frames), rendered by a pool of
+    spawned worker processes (they use no GPU)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    ks = list(range(frames)) if ks is None else list(ks)
+    workers = max(1, min(workers, os.cpu_count() or 1, len(ks)))
+    with ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(_config2_frame, [(size, frames, k) for k in ks], chunksize=8))
 
 
 def config4(size: int = 4096, grid: int = 4, radius: int = 20, seed: int = 20264, jump: float = 1.5,

@@ -101,32 +101,86 @@ class PolicyOracle:
     repair); on a fallback frame stats["scratch"] holds the from-scratch pass.
     """
 
-    def __init__(self, oracle, ref, template, radius, conv=0.001, stop=10.0, temporal=False):
+    def __init__(self, oracle, ref, template, radius, conv=0.001, stop=10.0, temporal=False,
+                 self_adaptive=False, depth12="native"):
+        """self_adaptive: every template POI's (square) subset_radius is its
+        FFTCC and IC-GN radius (one cached FFTCC layout per radius), as the
+        device driver's radius classes; radius is then the driver's radius
+        (the temporal max_delta)."""
         self.o = oracle
-        self.ref = ref.astype(np.float32)
+        self.ref = oracle.as_f32(ref, depth12)
+        self.depth12 = depth12
         self.template = template
         self.r = radius
         self.conv, self.stop = conv, stop
         self.temporal = temporal
-        self.prepared = oracle.PreparedRef(self.ref, radius, template)
+        self.self_adaptive = self_adaptive
+        radii = (template["subset_radius"][:, 0].astype(int) if self_adaptive
+                 else np.full(len(template), radius))
+        # per radius: template indices, position of each template index in
+        # its class layout, and the class's cached reference spectra
+        self.classes = []
+        self.pos = np.zeros(len(template), dtype=np.int64)
+        for rad in sorted(set(radii.tolist())):
+            idx = np.nonzero(radii == rad)[0]
+            self.pos[idx] = np.arange(len(idx))
+            self.classes.append((rad, idx, oracle.PreparedRef(self.ref, rad, template[idx])))
         self.prev = None
         self.prev2 = None
         self.baseline = None
+        self._grad = oracle.gradient(self.ref)
+        self._coef_key = None
+
+    def _icgn(self, tar, q):
+        """ICGN2D1 on q with the reference gradients computed once and the
+        target's B-spline table kept while the target stays the same."""
+        key = (tar.shape, hash(tar.tobytes()))
+        if key != self._coef_key:
+            self._coef, self._coef_key = self.o.bspline(tar), key
+        self.o.icgn(self.ref, tar, self.r, q, self.conv, self.stop, self_adaptive=self.self_adaptive,
+                    prepared=(*self._grad, self._coef))
 
     def _fftcc_icgn(self, tar, q, idx=None):
-        self.prepared.compute(tar, q, idx)
-        self.o.icgn(self.ref, tar, self.r, q, self.conv, self.stop)
+        """FFTCC on the cached spectra, then IC-GN, of q = the template POIs
+        idx (None: all of them)."""
+        tidx = np.arange(len(self.template)) if idx is None else np.asarray(idx)
+        for rad, cidx, prep in self.classes:
+            m = np.isin(tidx, cidx)
+            if not m.any():
+                continue
+            sub = q[m].copy()
+            prep.compute(tar, sub, self.pos[tidx[m]])
+            q[m] = sub
+        self._icgn(tar, q)
+
+    def learned_frame(self, tar_img, seeds, ok, threshold=0.8):
+        """The learned-seed frame (FI_POLICY_LEARNED_SEED) from given seeds:
+        IC-GN from the seeds; POIs that could not be seeded (ok false) or
+        whose result is not a finite zncc >= threshold are redone from the
+        template by FFTCC + IC-GN and replaced under
+        shouldReplacePoiWithFallback (better zncc wins, cli:505-517)."""
+        tar = self.o.as_f32(tar_img, self.depth12)
+        q = seeds.copy()
+        self._icgn(tar, q)
+        z = q["result"][:, 2]
+        cand = np.nonzero(~ok | ~(np.isfinite(z) & (z >= np.float32(threshold))))[0]
+        if len(cand):
+            fb = self.template[cand].copy()
+            self._fftcc_icgn(tar, fb, cand)
+            rep = should_replace(q[cand], fb)
+            q[cand[rep]] = fb[rep]
+        return q, {"seed_fallback_count": len(cand)}
 
     def scratch(self, tar_img):
         """The frame from the template, FFTCC then IC-GN (--no-skip-fftcc)."""
         q = self.template.copy()
-        self._fftcc_icgn(tar_img.astype(np.float32), q)
+        self._fftcc_icgn(self.o.as_f32(tar_img, self.depth12), q)
         return q
 
     def frame(self, tar_img, skip_policy=True, poi_fallback=True, seed=None):
         """seed (optional): the previous frame's results to seed from instead
         of this oracle's own (one-step parity against a device sequence)."""
-        tar = tar_img.astype(np.float32)
+        tar = self.o.as_f32(tar_img, self.depth12)
         n = len(self.template)
         prev = self.prev if seed is None else seed
         st = {"skip_fftcc_used": 0, "fallback_used": 0, "poi_fallback_count": 0,
@@ -142,7 +196,7 @@ class PolicyOracle:
             st["skip_fftcc_used"] = 1
             q, st["temporal_seed_count"] = seed_from_previous(self.template, prev, self.prev2, self.temporal,
                                                               np.float32(self.r))
-            self.o.icgn(self.ref, tar, self.r, q, self.conv, self.stop)
+            self._icgn(tar, q)
             bad = is_bad(q)
             n_bad = int(bad.sum())
             st["pre_repair_bad_ratio"] = st["bad_ratio"] = bad_ratio(q)

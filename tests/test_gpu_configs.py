@@ -1,0 +1,142 @@
+"""The two large workloads at their real sizes, against the oracles.
+
+config 2  1024^2 u16 (12-bit) speckle, the full 500-frame motion (15 px at 30
+          degrees, strain ramp to 5e-3), radii 10 / 15 / 20 per POI, frames
+          replayed as hipGraphs with the autotuner's launches and the
+          reference's default policy.  Frames 1, 250 and 500 are checked by
+          one-step parity: the policy oracle (mixed radii) runs frame k seeded
+          from the device's frame k-1 and with the baseline of its own first
+          frame, and must take the same decisions (skip, fallback, repair
+          counts) and meet the parity gate with identical validity flags.
+config 4  4096^2 u16 speckle with a straight crack at 30 degrees (1.5 px jump
+          of u), r = 20, grid 4, 1,026,169 POIs: the FFTCC-seeded frame and the
+          learned-seed frame with FFTCC fallback (two seeder passes).  Every
+          POI within 2r of the crack plus 3,000 POIs drawn from the rest are
+          checked: FFTCC-seeded against the FP32 and FP64 oracles; the learned
+          frame against the oracle's learned frame from the same seeds (the
+          seeder's output for those POIs), with the same fallback set.
+Iteration counts may differ from the oracle's (FP32 summation order) on at
+most 1 % of the shared valid POIs (measured: 0.3-0.4 %).
+"""
+import numpy as np
+import pytest
+
+import fasticgn_amd as fa
+from fasticgn_amd import seeder as sd
+from fasticgn_amd import synth
+from tests.parity import compare, finite_valid
+from tests.policy import PolicyOracle
+
+pytestmark = pytest.mark.gpu
+
+KEYS = ("skip_fftcc_used", "fallback_used", "poi_fallback_count", "poi_fallback_recovered")
+GATE_KEYS = ("n", "cpu_valid", "gpu_valid", "status_mismatch", "p99_du", "p99_dv", "max_du", "max_dv", "p99_dux",
+             "p99_dvy", "iteration_mismatch", "iteration_mismatch_frac", "pass")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def test_config2_full_sequence_one_step_parity(oracle):
+    F = 500
+    checked = (0, 249, 499)
+    seq = synth.config2(size=1024, frames=F)
+    say("config2: rendering", F, "frames of 1024^2 u16,", len(seq.pois), "POIs")
+    frames = synth.config2_frames(1024, F)
+    img = lambda a: fa.Image2D(a, depth12="right")  # noqa: E731
+    keep = {k: None for k in checked}
+    keep.update({k - 1: None for k in checked if k})
+    stats = []
+    with fa.FrameDriver(15) as d:
+        d.set_self_adaptive(True)
+        d.set_reference(img(seq.ref))
+        d.set_pois(seq.pois)
+        d.set_graph(True)
+        d.upload_target(img(frames[F // 2]))
+        chosen = [e for e in d.autotune() if e["chosen"]]
+        say("config2 autotune", chosen)
+        d.reset_sequence()
+        d.upload_target(img(frames[0]))
+        for k in range(F):
+            if k + 1 < F:
+                d.upload_target(img(frames[k + 1]))
+            stats.append(d.run())
+            if k in keep:
+                keep[k] = d.download()
+            if k % 100 == 0:
+                say("config2 device frame", k, {x: stats[-1][x] for x in KEYS + ("valid_ratio", "iteration_mean")})
+    po = PolicyOracle(oracle, seq.ref, seq.pois, 15, self_adaptive=True, depth12="right")
+    for k in checked:
+        o, ost = po.frame(frames[k], seed=keep[k - 1] if k else None)
+        st = stats[k]
+        say("config2 frame", k + 1, {x: st[x] for x in KEYS}, {x: ost[x] for x in KEYS})
+        for key in KEYS:
+            assert st[key] == ost[key], (k, key)
+        cmp = compare(o, keep[k], max_iteration_mismatch=0.01)
+        say("config2 frame", k + 1, {x: cmp.get(x) for x in GATE_KEYS})
+        assert cmp["pass"], (k, cmp)
+        if k:
+            assert st["baseline_bad_ratio"] == pytest.approx(float(po.baseline), abs=0)
+    res = keep[F - 1]
+    u, v, *_ = seq.truth(F - 1, res["x"], res["y"])
+    ok = finite_valid(res)
+    r = seq.pois["subset_radius"][:, 0].astype(int)
+    for rad in (10, 15, 20):
+        m = ok & (r == rad)
+        e = np.maximum(np.abs(res["deformation"][m, 0] - u[m]), np.abs(res["deformation"][m, 6] - v[m]))
+        say("config2 last frame radius", rad, "valid", int(m.sum()), "of", int((r == rad).sum()),
+            "p99 err", float(np.percentile(e, 99)))
+        assert m.sum() > 0.97 * (r == rad).sum() and np.percentile(e, 99) < 0.05
+
+
+def test_config4_crack_field_parity(oracle):
+    sc = synth.config4()
+    R = sc.radius
+    say("config4", sc.ref.shape, len(sc.pois), "POIs, r", R)
+    seeder = sd.LearnedSeeder()
+    with fa.FrameDriver(R) as d:
+        d.set_reference(fa.Image2D(sc.ref))
+        d.set_pois(sc.pois)
+        d.upload_target(fa.Image2D(sc.tar))
+        fft = d.run(policy=0)
+        res_f = d.download()
+        chosen = [e for e in d.autotune() if e["chosen"]]
+        say("config4 autotune", chosen)
+        d.set_seeder(seeder, 0.8)
+        d.set_seed_passes(2)
+        lst = d.run(policy=fa.driver.LEARNED_SEED)
+        res_l = d.download()
+        redo = d.download_redo()
+    say("config4 FFTCC frame valid", fft["valid_count"], "learned frame valid", lst["valid_count"],
+        "seed fallback", lst["seed_fallback_count"])
+    near = sc.crack_distance(sc.pois["x"], sc.pois["y"]) <= 2 * R
+    rng = np.random.default_rng(44)
+    far = np.sort(rng.choice(np.nonzero(~near)[0], 3000, replace=False))
+    pick = np.sort(np.concatenate([np.nonzero(near)[0], far]))
+    say("config4 checked POIs", len(pick), "of which near the crack", int(near.sum()))
+    ref32, tar32 = oracle.as_f32(sc.ref), oracle.as_f32(sc.tar)
+    # FFTCC-seeded frame: FP32 and FP64 oracles
+    for f64 in (False, True):
+        o = sc.pois[pick].copy()
+        oracle.fftcc(ref32, tar32, R, o)
+        oracle.icgn(ref32, tar32, R, o, f64=f64)
+        cmp = compare(o, res_f[pick], max_iteration_mismatch=0.01)
+        cn = compare(o[near[pick]], res_f[pick][near[pick]], max_iteration_mismatch=0.01)
+        say("config4 FFTCC frame vs", "FP64" if f64 else "FP32", "oracle", {x: cmp.get(x) for x in GATE_KEYS})
+        say("  near the crack only", {x: cn.get(x) for x in GATE_KEYS})
+        assert cmp["pass"] and cn["pass"], (f64, cmp, cn)
+    # learned frame: the oracle's learned frame from the seeder's seeds
+    seeds = sc.pois[pick].copy()
+    ok = seeder.seed(ref32, tar32, seeds)
+    seeder.seed(ref32, tar32, seeds)  # second pass: crops at the first pass's rounded u, v
+    po = PolicyOracle(oracle, sc.ref, sc.pois[pick], R)
+    o, ost = po.learned_frame(sc.tar, seeds, ok, 0.8)
+    cmp = compare(o, res_l[pick], max_iteration_mismatch=0.01)
+    say("config4 learned frame vs FP32 oracle", {x: cmp.get(x) for x in GATE_KEYS},
+        "fallback on the sample: oracle", ost["seed_fallback_count"], "device", int(redo[pick].sum()))
+    assert cmp["pass"], cmp
+    assert ost["seed_fallback_count"] == int(redo[pick].sum())
+    seeder.close()
+    say("config4 fallback fraction within 2r of the crack", float(redo[near].mean()),
+        "elsewhere", float(redo[~near].mean()))

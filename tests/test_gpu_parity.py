@@ -451,9 +451,15 @@ def test_config1_full_size_properties(oracle):
         f.compute(g)
     assert fields_equal(g, o)
     oracle.icgn(ref32, tar32, 15, o)
-    cmp = compare(o, res[pick])
+    cmp = compare(o, res[pick], max_iteration_mismatch=0.01)
     report("config1 sample", cmp)
     assert cmp["pass"], cmp
+    o64 = sc.pois[pick].copy()
+    oracle.fftcc(ref32, tar32, 15, o64)
+    oracle.icgn(ref32, tar32, 15, o64, f64=True)
+    c64 = compare(o64, res[pick], max_iteration_mismatch=0.01)
+    report("config1 sample vs FP64 oracle", c64)
+    assert c64["pass"], c64
 
 
 # ------------------------------------------------------------ learned seed --
