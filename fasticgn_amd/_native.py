@@ -30,6 +30,11 @@ FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
 FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP = 0, 1, 2
 SAMPLING_MODES = {"auto": FI_SAMPLING_AUTO, "table": FI_SAMPLING_TABLE, "group": FI_SAMPLING_GROUP}
 FI_POLICY_LEARNED_SEED = 4
+FI_EXPORT_REF_F32, FI_EXPORT_GRAD_X, FI_EXPORT_GRAD_Y, FI_EXPORT_TAR_F32, FI_EXPORT_BSPLINE, FI_EXPORT_PREP = range(6)
+# PoiPrep (fi_params.h), the records FI_EXPORT_PREP copies out: 176 bytes
+PREP_DTYPE = np.dtype([("mean", "<f4"), ("norm", "<f4"), ("status", "<i4"), ("rx", "<i4"),
+                       ("hinv", "<f4", (36,)), ("ref_sum", "<f4"), ("ry", "<i4"), ("pad", "<f4", (2,))])
+assert PREP_DTYPE.itemsize == 176
 
 # opencorr::POI2D, src/oc_poi.h:102-108: 25 float32 = 100 bytes
 POI_DTYPE = np.dtype([
@@ -70,6 +75,15 @@ class FrameStats(ctypes.Structure):
         ("temporal_seed_count", ctypes.c_int32),
         ("temporal_seed_used", ctypes.c_int32),
     ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class PrepareStats(ctypes.Structure):
+    """fi_prepare_stats (include/fasticgn.h)."""
+    _fields_ = [(name, ctypes.c_double) for name in
+                ("total_ms", "alloc_ms", "reference_ms", "fftcc_ms", "hessian_ms")]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -144,6 +158,7 @@ SIGNATURES = {
     "fi_icgn_prepare": [_P],
     "fi_icgn_compute": [_P, _P, _S],
     "fi_icgn_compute_offset": [_P, _P, _S, _P],
+    "fi_icgn_export": [_P, _I, _P, _S],
     "fi_frame_create": [_I, _F, _F, _I, ctypes.POINTER(_P)],
     "fi_frame_destroy": [_P],
     "fi_frame_set_ref": [_P, _P, _I, _I, _S, _I, _I],
@@ -162,8 +177,9 @@ SIGNATURES = {
     "fi_frame_download_redo": [_P, _P, _S],
     "fi_frame_set_fused": [_P, _I],
     "fi_frame_autotune": [_P, _P, _S, ctypes.POINTER(_S)],
-    "fi_frame_records": [_P, ctypes.POINTER(_P), ctypes.POINTER(_S)],
+    "fi_frame_records": [_P, _I, ctypes.POINTER(_P), ctypes.POINTER(_S)],
     "fi_frame_set_records": [_P, _I],
+    "fi_frame_prepare": [_P, ctypes.POINTER(PrepareStats)],
     "fi_frame_set_temporal_extrapolation": [_P, _I, _F],
     "fi_seeder_create": [_I, ctypes.POINTER(_P)],
     "fi_seeder_destroy": [_P],

@@ -256,6 +256,11 @@ int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* cent
     });
 }
 
+int fi_icgn_export(fi_icgn* h, int what, void* out, size_t bytes)
+{
+    return guarded([&] { need(h); h->export_data(what, out, bytes); });
+}
+
 // ---- frame driver ----
 int fi_frame_create(int radius, float conv, float stop, int device, fi_frame** out)
 {
@@ -301,9 +306,9 @@ int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n)
     return guarded([&] { need(h); h->download_redo(out, n); });
 }
 
-int fi_frame_records(fi_frame* h, const void** records, size_t* n_records)
+int fi_frame_records(fi_frame* h, int age, const void** records, size_t* n_records)
 {
-    return guarded([&] { need(h); h->records(records, n_records); });
+    return guarded([&] { need(h); h->records(age, records, n_records); });
 }
 
 int fi_frame_set_records(fi_frame* h, int on)
@@ -319,6 +324,15 @@ int fi_frame_set_temporal_extrapolation(fi_frame* h, int on, float max_delta)
         h->temporal = on != 0;
         h->temporal_max_delta = max_delta;
         h->invalidate_graphs();
+    });
+}
+
+int fi_frame_prepare(fi_frame* h, fi_prepare_stats* stats)
+{
+    return guarded([&] {
+        need(h);
+        const fi_prepare_stats ps = h->prepare_now();
+        if (stats) *stats = ps;
     });
 }
 

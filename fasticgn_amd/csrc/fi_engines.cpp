@@ -3,6 +3,7 @@
 #include "fi_seeder_engine.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstring>
 #include <map>
@@ -452,11 +453,60 @@ void fi_icgn::compute(fi_poi* pois, size_t n, const float* offsets)
     FI_HIP(hipMemcpyAsync(h_pois.p, d_pois.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
     FI_HIP(hipStreamSynchronize(stream));
     std::memcpy(pois, h_pois.p, n * sizeof(fi_poi));
+    prep_n = n;
     if (offsets) {
         prep_key.clear();  // the records hold this call's offsets
     } else {
         prep_key.swap(key);
         prep_version = ref_version;
+    }
+}
+
+void fi_icgn::export_data(int what, void* out, size_t bytes)
+{
+    if (out == nullptr && bytes > 0) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");
+    DeviceGuard g(device);
+    const int w = ref_raw.view.width, h = ref_raw.view.height;
+    const size_t px = (size_t)w * h;
+    auto copy_out = [&](const void* src, size_t want) {
+        if (bytes != want)
+            fail(FI_E_INVALID_ARGUMENT, "export needs " + std::to_string(want) + " bytes, got " + std::to_string(bytes));
+        if (want) FI_HIP(hipMemcpy(out, src, want, hipMemcpyDeviceToHost));
+    };
+    switch (what) {
+    case FI_EXPORT_REF_F32:
+    case FI_EXPORT_GRAD_X:
+    case FI_EXPORT_GRAD_Y:
+        if (!grads_ready) fail(FI_E_LOGIC, "export of the reference side needs prepareRef");
+        copy_out(what == FI_EXPORT_REF_F32 ? ref_f.p : (what == FI_EXPORT_GRAD_X ? gx.p : gy.p), px * sizeof(float));
+        break;
+    case FI_EXPORT_TAR_F32:
+        if (!coef_ready) fail(FI_E_LOGIC, "export of the target side needs prepareTar");
+        copy_out(tar_f.p, (size_t)tar_raw.view.width * tar_raw.view.height * sizeof(float));
+        break;
+    case FI_EXPORT_BSPLINE: {
+        // table rows hold four planes of w float4 (plane k = coef[k][0..3]);
+        // out is texel-major: 16 floats coef[k][l] per texel, k*4 + l
+        if (!coef_ready) fail(FI_E_LOGIC, "export of the B-spline table needs prepareTar");
+        const int tw = tar_raw.view.width, th = tar_raw.view.height;
+        const size_t want = (size_t)16 * tw * th * sizeof(float);
+        if (bytes != want)
+            fail(FI_E_INVALID_ARGUMENT, "export needs " + std::to_string(want) + " bytes, got " + std::to_string(bytes));
+        std::vector<float> tab((size_t)16 * tw * th);
+        FI_HIP(hipMemcpy(tab.data(), coef.p, tab.size() * sizeof(float), hipMemcpyDeviceToHost));
+        float* o = static_cast<float*>(out);
+        for (int y = 0; y < th; y++)
+            for (int k = 0; k < 4; k++)
+                for (int x = 0; x < tw; x++)
+                    std::memcpy(o + 16 * ((size_t)y * tw + x) + 4 * k, tab.data() + 4 * ((size_t)(4 * y + k) * tw + x),
+                                4 * sizeof(float));
+        break;
+    }
+    case FI_EXPORT_PREP:
+        copy_out(prep.p, prep_n * sizeof(PoiPrep));
+        break;
+    default:
+        fail(FI_E_INVALID_ARGUMENT, "unknown export " + std::to_string(what));
     }
 }
 
@@ -487,7 +537,6 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
     rec_stream = make_stream();
     for (auto& e : rec_ev) FI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     FI_HIP(hipEventCreateWithFlags(&pack_ev, hipEventDisableTiming));
-    iter_total.reserve(1);
     threshold.reserve(1);
     stager = std::thread([this] { stage_loop(); });
 }
@@ -654,7 +703,7 @@ void fi_frame::reset_sequence()
 {
     have_prev = false;
     have_prev2 = false;
-    rec_frame = -1;
+    rec_held[0] = rec_held[1] = -1;
     baseline_ready = false;
     baseline_bad = NAN;
     frame_index = 0;
@@ -714,14 +763,15 @@ void fi_frame::prepare()
         lay = layout.data();
     }
 
+    using clock = std::chrono::steady_clock;
+    const auto t_start = clock::now();
+    // device buffers first (the allocation share of a cold frame)
     ref_f.reserve(px);
     gx.reserve(px);
     gy.reserve(px);
-    launch_to_float(ref_raw.view, ref_f.p, stream);
-    launch_gradient(ref_f.p, w, h, gx.p, gy.p, stream);
     const size_t cap = std::max<size_t>(n, 1);
     tmpl.reserve(cap);
-    iters.reserve(cap);
+    iters.reserve(3 * cap);
     seed_ok.reserve(cap);
     cur.reserve(cap);
     prev.reserve(cap);
@@ -734,25 +784,45 @@ void fi_frame::prepare()
     counters.reserve(nclass() + 1);
     h_counters.reserve(nclass() + 1);
     queue.reserve((size_t)kQueueWords * std::max<size_t>(nclass(), 1));
+    class_n.reserve(std::max<size_t>(nclass(), 1));
+    if (!order.empty()) d_order.reserve(n);
+    for (auto& cp : classes) {
+        FrameClass& c = *cp;
+        c.tw.reserve(4 * (size_t)c.r);
+        c.spectra.reserve(c.n * (size_t)2 * c.r * (c.r + 1) * 2);
+        c.slots.reserve(c.n);
+    }
+    if (records_on)
+        for (int k = 0; k < 2; k++) {
+            d_rec[k].reserve(std::max<size_t>(n * (size_t)36, 36));
+            h_rec[k].reserve(std::max<size_t>(n * (size_t)36, 36));
+        }
+    const auto t_alloc = clock::now();
+    // host data: template layout, class sizes, layout order, DFT twiddles
+    // (the host vectors live until the closing synchronisation)
     if (n > 0) FI_HIP(hipMemcpyAsync(tmpl.p, lay, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
     std::vector<unsigned> sizes(nclass());
     for (size_t c = 0; c < nclass(); c++) sizes[c] = (unsigned)classes[c]->n;
-    class_n.reserve(std::max<size_t>(nclass(), 1));
     if (!sizes.empty())
         FI_HIP(hipMemcpyAsync(class_n.p, sizes.data(), sizes.size() * sizeof(unsigned), hipMemcpyHostToDevice,
                               stream));
-    if (!order.empty()) {
-        d_order.reserve(n);
+    if (!order.empty())
         FI_HIP(hipMemcpyAsync(d_order.p, order.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, stream));
+    std::vector<std::vector<float>> tws(nclass());
+    for (size_t ci = 0; ci < nclass(); ci++) {
+        FrameClass& c = *classes[ci];
+        tws[ci].resize(4 * c.r);
+        dft_twiddles(2 * c.r, tws[ci].data());
+        FI_HIP(hipMemcpyAsync(c.tw.p, tws[ci].data(), tws[ci].size() * sizeof(float), hipMemcpyHostToDevice,
+                              stream));
     }
+    // reference f32 and Gradient2D4, FFTCC reference spectra, per-POI records
+    FI_HIP(hipEventRecord(ev[4], stream));
+    launch_to_float(ref_raw.view, ref_f.p, stream);
+    launch_gradient(ref_f.p, w, h, gx.p, gy.p, stream);
+    FI_HIP(hipEventRecord(ev[5], stream));
     for (auto& cp : classes) {
         FrameClass& c = *cp;
-        std::vector<float> t(4 * c.r);
-        dft_twiddles(2 * c.r, t.data());
-        c.tw.reserve(t.size());
-        FI_HIP(hipMemcpyAsync(c.tw.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-        c.spectra.reserve(c.n * (size_t)2 * c.r * (c.r + 1) * 2);
-        c.slots.reserve(c.n);
         FftccParams fp{};
         fp.ref = ref_f.p;
         fp.tar = ref_f.p;
@@ -766,6 +836,10 @@ void fi_frame::prepare()
         fp.slots = c.slots.p;
         fp.twiddles = c.tw.p;
         launch_fftcc(fp, stream);
+    }
+    FI_HIP(hipEventRecord(ev[6], stream));
+    for (auto& cp : classes) {
+        FrameClass& c = *cp;
         IcgnParams ip{};
         ip.ref = ref_f.p;
         ip.gx = gx.p;
@@ -778,13 +852,31 @@ void fi_frame::prepare()
         ip.rx = c.r;
         ip.ry = c.r;
         launch_icgn_prep(ip, stream);
-        // the twiddle staging vector must outlive its copy
-        FI_HIP(hipStreamSynchronize(stream));
     }
+    FI_HIP(hipEventRecord(ev[7], stream));
     FI_HIP(hipGetLastError());
     FI_HIP(hipStreamSynchronize(stream));
+    const auto t_end = clock::now();
+    fi_prepare_stats ps{};
+    ps.total_ms = std::chrono::duration<double, std::milli>(t_end - t_start).count();
+    ps.alloc_ms = std::chrono::duration<double, std::milli>(t_alloc - t_start).count();
+    float ms = 0.f;
+    FI_HIP(hipEventElapsedTime(&ms, ev[4], ev[5]));
+    ps.reference_ms = ms;
+    FI_HIP(hipEventElapsedTime(&ms, ev[5], ev[6]));
+    ps.fftcc_ms = ms;
+    FI_HIP(hipEventElapsedTime(&ms, ev[6], ev[7]));
+    ps.hessian_ms = ms;
+    last_prepare = ps;
     invalidate_graphs();
     prepared = true;
+}
+
+fi_prepare_stats fi_frame::prepare_now()
+{
+    DeviceGuard g(device);
+    if (!prepared) prepare();
+    return last_prepare;
 }
 
 void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* slot_dev,
@@ -808,7 +900,7 @@ void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, co
 }
 
 void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* prep_index,
-                         const unsigned* count_dev)
+                         const unsigned* count_dev, int region)
 {
     if (count <= 0) return;
     const int mode = c.sampling >= 0 ? c.sampling : sampling;
@@ -831,7 +923,7 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     ip.ry = c.r;
     ip.conv = conv;
     ip.stop = stop;
-    ip.iters = iters.p + c.off;
+    ip.iters = iters.p + (size_t)region * n + c.off;
     int waves = c.waves, spl = c.spl;
     if (waves == 0 && shape_waves && icgn_shape_covers(shape_waves, shape_spl, c.r, c.r)) {
         waves = shape_waves;
@@ -847,7 +939,6 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
         launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), kQueueWords, stream);
     }
     icgn_launch(ip, waves, spl, stream);
-    launch_sum_iters(ip.iters, count, iter_total.p, stream, count_dev);
 }
 
 // The fewest-waves generic shape whose bands cover every class radius.
@@ -867,7 +958,7 @@ bool fi_frame::fused_shape(int& waves, int& spl) const
 
 // One launch over all classes: the kernel reads each POI's subset_radius
 // (self-adaptive); the LDS tile and shape are those of the largest radius.
-void fi_frame::icgn_fused(fi_poi* base)
+void fi_frame::icgn_fused(fi_poi* base, int region)
 {
     const int mode = fused.sampling >= 0 ? fused.sampling : sampling;
     IcgnParams ip{};
@@ -888,7 +979,7 @@ void fi_frame::icgn_fused(fi_poi* base)
     ip.self_adaptive = 1;
     ip.conv = conv;
     ip.stop = stop;
-    ip.iters = iters.p;
+    ip.iters = iters.p + (size_t)region * n;
     int waves = fused.waves, spl = fused.spl;
     if (waves == 0 && !fused_shape(waves, spl))
         fail(FI_E_UNSUPPORTED, "no IC-GN lane shape covers every radius class");
@@ -899,16 +990,15 @@ void fi_frame::icgn_fused(fi_poi* base)
         launch_zero_words(reinterpret_cast<uint32_t*>(ip.queue), kQueueWords, stream);
     }
     icgn_launch(ip, waves, spl, stream);
-    launch_sum_iters(ip.iters, (int)n, iter_total.p, stream);
 }
 
-void fi_frame::icgn_all(fi_poi* base)
+void fi_frame::icgn_all(fi_poi* base, int region)
 {
     if (fuse && nclass() > 1) {
-        icgn_fused(base);
+        icgn_fused(base, region);
         return;
     }
-    for (auto& c : classes) icgn_pass(*c, base + c->off, (int)c->n, nullptr);
+    for (auto& c : classes) icgn_pass(*c, base + c->off, (int)c->n, nullptr, nullptr, region);
 }
 
 void fi_frame::fftcc_all(fi_poi* base)
@@ -933,7 +1023,7 @@ void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* 
     fi_poi* rep = repair.p + c.off + lo;
     launch_gather(tmpl.p + c.off, idx, k, rep, stream, count_dev, redo.p + c.off);
     fftcc_pass(c, rep, k, idx, count_dev);
-    icgn_pass(c, rep, k, idx, count_dev);
+    icgn_pass(c, rep, k, idx, count_dev, 1);
     launch_merge_repair(cur.p + c.off, rep, idx, k, counters.p + ci, stream, count_dev);
     FI_HIP(hipGetLastError());
 }
@@ -1025,9 +1115,9 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         // a fill kernel, not memsets: this runs inside captured frame graphs
         // FFTCC and learned seeds start from the template: copied here, in the
         // reset launch
-        launch_frame_reset(iter_total.p, reinterpret_cast<uint32_t*>(counters.p),
-                           (C + 1) * sizeof(FrameCounters) / 4, redo.p, n, mode == 1 ? 1 : 0, stream,
-                           mode == 2 ? nullptr : tmpl.p, cur.p, n * sizeof(fi_poi));
+        launch_frame_reset(reinterpret_cast<uint32_t*>(counters.p), (C + 1) * sizeof(FrameCounters) / 4, iters.p,
+                           3 * n, redo.p, n, mode == 1 ? 1 : 0, stream, mode == 2 ? nullptr : tmpl.p, cur.p,
+                           n * sizeof(fi_poi));
         if (timed) FI_HIP(hipEventRecord(ev[0], stream));
         // the B-spline table, when a launch samples it (HBM-write-bound), on a
         // second stream beside the seeding (FFTCC or the seeder read only the
@@ -1096,11 +1186,10 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             for (int ci = 0; ci < C; ci++)
                 if (classes[ci]->n > 0)
                     icgn_pass(*classes[ci], cur.p + classes[ci]->off, (int)classes[ci]->n, nullptr,
-                              &counters.p[ci].fallback_n);
-            launch_merge_frame_fallback(save.p, cur.p, cnt, stream, flag);
-            launch_fill_bytes(redo.p, n, 1, stream, flag);
+                              &counters.p[ci].fallback_n, 2);
+            launch_merge_frame_fallback(save.p, cur.p, cnt, stream, flag, redo.p);
         }
-        launch_qa(cur.p, cnt, counters.p + C, stream, iter_total.p);
+        launch_qa(cur.p, cnt, counters.p + C, stream, iters.p, 3 * n);
         FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
                               stream));
         FI_HIP(hipGetLastError());
@@ -1198,7 +1287,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     }
     if (redo_qa) {
         launch_zero_words(reinterpret_cast<uint32_t*>(counters.p + C), sizeof(FrameCounters) / 4, stream);
-        launch_qa(cur.p, cnt, counters.p + C, stream, iter_total.p);
+        launch_qa(cur.p, cnt, counters.p + C, stream, iters.p, 3 * n);
         FI_HIP(hipGetLastError());
         qa = read_counters(nullptr);
     }
@@ -1272,8 +1361,7 @@ void fi_frame::queue_records(const fi_poi* src)
     FI_HIP(hipStreamWaitEvent(rec_stream, pack_ev, 0));
     if (bytes) FI_HIP(hipMemcpyAsync(h_rec[s].p, d_rec[s].p, bytes, hipMemcpyDeviceToHost, rec_stream));
     FI_HIP(hipEventRecord(rec_ev[s], rec_stream));
-    rec_last = s;
-    rec_frame = frame_index;
+    rec_held[s] = frame_index;
 }
 
 void fi_frame::set_records(bool on)
@@ -1281,15 +1369,20 @@ void fi_frame::set_records(bool on)
     records_on = on;
 }
 
-void fi_frame::records(const void** out, size_t* count)
+void fi_frame::records(int age, const void** out, size_t* count)
 {
     if (!have_prev) fail(FI_E_LOGIC, "no frame has been run");
     if (out == nullptr) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");
+    if (age < 0 || age > 1) fail(FI_E_INVALID_ARGUMENT, "records age must be 0 (last frame) or 1");
     DeviceGuard g(device);
-    if (rec_frame != frame_index - 1) {
+    const int want = frame_index - 1 - age;
+    int s = rec_held[0] == want ? 0 : (rec_held[1] == want ? 1 : -1);
+    if (s < 0) {
+        if (age != 0 || want < 0)
+            fail(FI_E_LOGIC, "the records of that frame are not held (fi_frame_set_records keeps two frames)");
         // not packed by its run: pack the latest results (prev after the rotation)
         const int keep = frame_index;
-        frame_index--;
+        frame_index = want;
         try {
             queue_records(prev.p);
         } catch (...) {
@@ -1297,9 +1390,10 @@ void fi_frame::records(const void** out, size_t* count)
             throw;
         }
         frame_index = keep;
+        s = rec_held[0] == want ? 0 : 1;
     }
-    FI_HIP(hipEventSynchronize(rec_ev[rec_last]));
-    *out = h_rec[rec_last].p;
+    FI_HIP(hipEventSynchronize(rec_ev[s]));
+    *out = h_rec[s].p;
     if (count) *count = n;
 }
 
@@ -1484,7 +1578,6 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
             fused.queue_blocks = -1;
         }
     }
-    FI_HIP(hipMemsetAsync(iter_total.p, 0, sizeof(unsigned long long), stream));
     FI_HIP(hipStreamSynchronize(stream));
     invalidate_graphs();
     if (report_n) *report_n = std::min(m, report_len);

@@ -84,6 +84,9 @@ struct fi_icgn {
     void prepare_tar();
     // offsets: n (x, y) pairs (compute(POI2D*, Point2D&)), or nullptr
     void compute(fi_poi* pois, size_t n, const float* offsets = nullptr);
+    size_t prep_n = 0;  // records of the last compute
+    // Copies a device-side array out (fi_icgn_export).
+    void export_data(int what, void* out, size_t bytes);
 };
 
 // One radius class of the frame driver's template: POIs [off, off + n) of the
@@ -157,8 +160,7 @@ struct fi_frame {
     // counters[c] for the scans and repairs of class c, counters[classes] for QA
     fi::DevArray<fi::FrameCounters> counters;
     fi::PinnedArray<fi::FrameCounters> h_counters;
-    fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry
-    fi::DevArray<unsigned long long> iter_total;
+    fi::DevArray<uint32_t> iters;  // executed iterations per IC-GN launch entry, 3 x n (see icgn_pass)
     fi::DevArray<int> queue;       // persistent-queue claim counters, kQueueWords per class
     fi::PinnedArray<fi_poi> h_out;
     fi::DevArray<uint8_t> redo;    // per POI: re-run from FFTCC in the last frame
@@ -176,12 +178,14 @@ struct fi_frame {
     fi::PinnedArray<uint8_t> h_rec[2];
     hipEvent_t rec_ev[2] = {}, pack_ev = nullptr;
     hipStream_t rec_stream = nullptr;
-    int rec_next = 0, rec_last = -1, rec_frame = -1;
+    int rec_next = 0;
+    int rec_held[2] = { -1, -1 };    // frame index whose records each slot holds
     size_t n = 0;
     bool ref_set = false, pois_set = false, prepared = false, tar_ready = false;
     bool have_prev = false, have_prev2 = false, baseline_ready = false;
     float baseline_bad = NAN;
     float threshold_host = NAN;      // the value last written to `threshold`
+    fi_prepare_stats last_prepare{}; // timing of the last reference-side preparation
     int frame_index = 0;
     hipEvent_t ev[8] = {};
     struct Graph {
@@ -198,13 +202,17 @@ struct fi_frame {
     void upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12);
     void run(unsigned policy, fi_frame_stats* stats);
     void download(fi_poi* out, size_t n);
-    // The last frame's raw records in pinned host memory (valid until the
-    // second run after it); packs and copies them now if the frame did not.
-    void records(const void** out, size_t* count);
+    // Raw records of the frame `age` runs back (0: the last) in pinned host
+    // memory, valid until the second run after that frame; the last frame's
+    // are packed and copied now if its run did not queue them.
+    void records(int age, const void** out, size_t* count);
     void set_records(bool on);
     void download_redo(uint8_t* out, size_t n);
     void reset_sequence();
     void invalidate_graphs();
+    // Prepares the reference side now if it is not prepared; the timing of
+    // the last preparation.
+    fi_prepare_stats prepare_now();
     // Times the IC-GN lane shapes, sampling modes and launch modes on the
     // uploaded target per radius class and keeps the fastest; returns the
     // number of configurations timed.
@@ -215,11 +223,14 @@ private:
     void queue_records(const fi_poi* src);
     void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev,
                     const unsigned* count_dev = nullptr);
+    // region: which third of `iters` the launch writes its per-entry
+    // iteration counts to (0 first pass, 1 POI-level repair, 2 whole-frame
+    // fallback); k_qa sums all three
     void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index,
-                   const unsigned* count_dev = nullptr);
-    void icgn_fused(fi_poi* base);
+                   const unsigned* count_dev = nullptr, int region = 0);
+    void icgn_fused(fi_poi* base, int region = 0);
     bool fused_shape(int& waves, int& spl) const;
-    void icgn_all(fi_poi* base);
+    void icgn_all(fi_poi* base, int region = 0);
     void fftcc_all(fi_poi* base);
     void repair_class(const FrameClass& c, int m, int lo = 0, const unsigned* count_dev = nullptr);
     int repair_cap(const FrameClass& c) const;

@@ -141,26 +141,27 @@ void launch_scan(const fi_poi* cur, const fi_poi* prev, int n, int collect,
 // flags (optional): flags[idx[j]] = 1 for every gathered entry
 void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, hipStream_t s,
                    const unsigned* count = nullptr, uint8_t* flags = nullptr);
-// gate (optional, device): merge only when *gate != 0
+// gate (optional, device): merge only when *gate != 0; flags (optional):
+// flags[0..n) = 1 when merged
 void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s,
-                                 const unsigned* gate = nullptr);
+                                 const unsigned* gate = nullptr, uint8_t* flags = nullptr);
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
                          FrameCounters* counters, hipStream_t s, const unsigned* count = nullptr);
-// iter_total (optional): copied into counters->iterations
-void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s,
-               const unsigned long long* iter_total = nullptr);
+// counters->iterations += sum of iters[0..n_iters) (the frame's per-entry
+// executed iteration counts)
+void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s, const uint32_t* iters = nullptr,
+               size_t n_iters = 0);
 // learned-seed fallback candidates (see k_policy.hip)
 void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float threshold, FrameCounters* counters,
                       uint32_t* candidates, hipStream_t s);
-// *total += sum of iters[0..n)
-void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s,
-                      const unsigned* count = nullptr);
 // p[0..n) = 0 (32-bit words), by a kernel
 void launch_zero_words(uint32_t* p, size_t n, hipStream_t s);
 // p[0..n) = v (bytes); with gate (device), only when *gate != 0
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s, const unsigned* gate = nullptr);
-// *total = 0, words[0..n_words) = 0, flags[0..n) = v: one launch
-void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
-                        uint8_t v, hipStream_t s, const void* src = nullptr, void* dst = nullptr, size_t bytes = 0);
+// words[0..n_words) = 0, words2[0..n_words2) = 0, flags[0..n) = v, and src
+// copied to dst when given: one launch
+void launch_frame_reset(uint32_t* words, size_t n_words, uint32_t* words2, size_t n_words2, uint8_t* flags,
+                        size_t n, uint8_t v, hipStream_t s, const void* src = nullptr, void* dst = nullptr,
+                        size_t bytes = 0);
 
 }  // namespace fi

@@ -3,8 +3,8 @@
 //
 //   k_icgn_prep  once per reference layout: reference subset mean and norm,
 //                steepest-descent Hessian (compensated FP32 sums), FP64
-//                partial-pivot inversion and sum (ref - mean); stored as a
-//                176-byte PoiPrep record.
+//                partial-pivot inversion (one column per lane) and
+//                sum (ref - mean); stored as a 176-byte PoiPrep record.
 //   k_icgn       once per frame (k_icgn_impl.h): the inverse-compositional
 //                Gauss-Newton loop, one block per POI.
 //
@@ -15,44 +15,17 @@
 
 namespace fi {
 
-struct SubsetGeom {
-    int rx, ry, sw, sh, ns;
-    uint32_t div_magic;  // ceil(2^32 / sw): row = umulhi(s, magic)
-};
-
-__device__ __forceinline__ SubsetGeom make_geom(int rx, int ry)
-{
-    SubsetGeom g;
-    g.rx = rx;
-    g.ry = ry;
-    g.sw = 2 * rx + 1;
-    g.sh = 2 * ry + 1;
-    g.ns = g.sw * g.sh;
-    g.div_magic = (uint32_t)((0x100000000ull + (unsigned long long)g.sw - 1) / (unsigned long long)g.sw);
-    return g;
-}
-
-__device__ __forceinline__ void sample_coords(const SubsetGeom& g, int s, int& c, int& r)
-{
-    r = (int)__umulhi((uint32_t)s, g.div_magic);
-    c = s - r * g.sw;
-}
-
 // ---------------------------------------------------------------------------
 // FP64 6x6 inverse, partial-pivot LU with the same structure as the CPU
 // reference's Eigen PartialPivLU (largest |a_ik|, first on ties; a zero pivot
 // column is skipped so a singular Hessian yields inf/NaN exactly as there).
+// Every lane factors (the same values: the pivot choice is uniform across the
+// wave); lane c < 6 then solves for column c of the inverse.
 // ---------------------------------------------------------------------------
-__device__ void inverse6_f64(const double* a, double* inv)
+__device__ __forceinline__ void lu6_f64(double (&m)[6][6], int (&perm)[6])
 {
-    double m[6][6];
-    int perm[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) {
-        perm[i] = i;
-#pragma unroll
-        for (int j = 0; j < 6; j++) m[i][j] = a[i * 6 + j];
-    }
+    for (int i = 0; i < 6; i++) perm[i] = i;
 #pragma unroll
     for (int k = 0; k < 6; k++) {
         int piv = k;
@@ -62,53 +35,66 @@ __device__ void inverse6_f64(const double* a, double* inv)
             const double s = fabs(m[i][k]);
             if (s > best) { best = s; piv = i; }
         }
-        if (best != 0.0) {
-            if (piv != k) {
+        const bool nz = best != 0.0;
+        // every lane holds the same matrix, so the pivot row is uniform: one
+        // scalar branch per candidate row, and only the pivot row's swap runs
+        const int pu = __builtin_amdgcn_readfirstlane(nz ? piv : k);
+#pragma unroll
+        for (int i = k + 1; i < 6; i++) {
+            if (i == pu) {
 #pragma unroll
                 for (int j = 0; j < 6; j++) {
-                    // select-based swap keeps the matrix in registers
-#pragma unroll
-                    for (int i = k + 1; i < 6; i++)
-                        if (i == piv) { const double t = m[k][j]; m[k][j] = m[i][j]; m[i][j] = t; }
+                    const double a = m[k][j];
+                    m[k][j] = m[i][j];
+                    m[i][j] = a;
                 }
-#pragma unroll
-                for (int i = k + 1; i < 6; i++)
-                    if (i == piv) { const int t = perm[k]; perm[k] = perm[i]; perm[i] = t; }
+                const int pa = perm[k];
+                perm[k] = perm[i];
+                perm[i] = pa;
             }
+        }
 #pragma unroll
-            for (int i = k + 1; i < 6; i++) m[i][k] /= m[k][k];
+        for (int i = k + 1; i < 6; i++) {
+            const double l = m[i][k] / m[k][k];
+            m[i][k] = nz ? l : m[i][k];
         }
 #pragma unroll
         for (int i = k + 1; i < 6; i++)
 #pragma unroll
             for (int j = k + 1; j < 6; j++) m[i][j] -= m[i][k] * m[k][j];
     }
+}
+
+// Column c of the inverse from the factors: forward then back substitution on
+// the permuted unit vector.
+__device__ __forceinline__ void solve6_col_f64(const double (&m)[6][6], const int (&perm)[6], int c, double (&x)[6])
+{
 #pragma unroll
-    for (int c = 0; c < 6; c++) {
-        double x[6];
+    for (int i = 0; i < 6; i++) x[i] = (perm[i] == c) ? 1.0 : 0.0;
 #pragma unroll
-        for (int i = 0; i < 6; i++) x[i] = (perm[i] == c) ? 1.0 : 0.0;
+    for (int k = 0; k < 6; k++)
 #pragma unroll
-        for (int k = 0; k < 6; k++)
+        for (int i = k + 1; i < 6; i++) x[i] -= m[i][k] * x[k];
 #pragma unroll
-            for (int i = k + 1; i < 6; i++) x[i] -= m[i][k] * x[k];
+    for (int k = 5; k >= 0; k--) {
+        x[k] /= m[k][k];
 #pragma unroll
-        for (int k = 5; k >= 0; k--) {
-            x[k] /= m[k][k];
-#pragma unroll
-            for (int i = 0; i < k; i++) x[i] -= m[i][k] * x[k];
-        }
-#pragma unroll
-        for (int i = 0; i < 6; i++) inv[i * 6 + c] = x[i];
+        for (int i = 0; i < k; i++) x[i] -= m[i][k] * x[k];
     }
 }
 
 // ---------------------------------------------------------------------------
 // Reference-side prep: mean, norm, Hessian, inverse.
 // ---------------------------------------------------------------------------
-// Three streaming passes over the subset (sum, centred sum of squares,
-// Hessian); samples are re-read from L1/L2 instead of held in VGPRs, so one
-// kernel serves every radius.
+// One wave per POI.  The subset's columns map to lanes and, when the subset is
+// at most 32 wide, the wave's two (or more) lane groups take alternate rows;
+// rows are streamed in batches of kPrepRows with every load of a batch in
+// flight at once.  Pass 1 accumulates the reference sum and the 21 Hessian
+// entries as per-lane FMA chains (at most 63 terms), reduced across lanes by
+// the transposing TwoSum tree six sums at a time; pass 2 (the rows again, from
+// cache) the centred sums of squares, reduced the same way.
+constexpr int kPrepRows = 8;
+
 __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
 {
     const int lane = threadIdx.x;
@@ -130,68 +116,113 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
         if (lane == 0) { out->status = 0; out->rx = rx; out->ry = ry; }
         return;
     }
-    const SubsetGeom g = make_geom(rx, ry);
+    const int sw = 2 * rx + 1, sh = 2 * ry + 1, ns = sw * sh;
     const int ulx = (int)(x - rx), uly = (int)(y - ry);  // Subset2D::fill, src/oc_subset.cpp:39-44
     const int px = (int)x, py = (int)y;                   // gradients at (int)x + local, :206-210
+    const int groups = 64 / sw;                           // lane groups, one row each per step
+    const int grp = lane / sw, col = lane - grp * sw;
+    const bool on = grp < groups;
+    const int c0 = on ? col : 0, g0 = on ? grp : 0;
+    const float* ref_col = p.ref + (size_t)(uly + g0) * w + (ulx + c0);
+    const size_t go = (size_t)(py - ry + g0) * w + (px - rx + c0);
+    const float* gx_col = p.gx + go;
+    const float* gy_col = p.gy + go;
+    const float xl = (float)(c0 - rx) - ox;
+    const size_t step = (size_t)groups * w;  // rows between a lane's samples
+    const int rows = on ? (sh - g0 + groups - 1) / groups : 0;  // this lane's rows
+    const int last = rows > 0 ? rows - 1 : 0;
 
-    CSum sum_r;
-    for (int s = lane; s < g.ns; s += 64) {
-        int c, r;
-        sample_coords(g, s, c, r);
-        sum_r.add(p.ref[(size_t)(uly + r) * w + (ulx + c)]);
+    float sum_r = 0.f;
+    float hs[21];
+#pragma unroll
+    for (int q = 0; q < 21; q++) hs[q] = 0.f;
+    for (int k0 = 0; k0 < rows; k0 += kPrepRows) {
+        float rv[kPrepRows], gxv[kPrepRows], gyv[kPrepRows];
+        // every load is issued (rows past the lane's last re-read its last
+        // row, in bounds) and the values are masked afterwards: no branch
+        // separates the loads of a batch
+#pragma unroll
+        for (int k = 0; k < kPrepRows; k++) {
+            const size_t o = (size_t)min(k0 + k, last) * step;
+            rv[k] = ref_col[o];
+            gxv[k] = gx_col[o];
+            gyv[k] = gy_col[o];
+        }
+#pragma unroll
+        for (int k = 0; k < kPrepRows; k++) {
+            const bool ok = k0 + k < rows;
+            rv[k] = ok ? rv[k] : 0.f;
+            gxv[k] = ok ? gxv[k] : 0.f;
+            gyv[k] = ok ? gyv[k] : 0.f;
+        }
+#pragma unroll
+        for (int k = 0; k < kPrepRows; k++) {
+            // rows past the lane's last give zero terms (all three loads are 0)
+            sum_r += rv[k];
+            const float yl = (float)(g0 + (k0 + k) * groups - ry) - oy;
+            const float sd[6] = { gxv[k], gxv[k] * xl, gxv[k] * yl, gyv[k], gyv[k] * xl, gyv[k] * yl };
+            // entry (a, e <= a) at a (a + 1) / 2 + e
+#pragma unroll
+            for (int a = 0; a < 6; a++)
+#pragma unroll
+                for (int e = 0; e <= a; e++)
+                    hs[a * (a + 1) / 2 + e] = __builtin_fmaf(sd[a], sd[e], hs[a * (a + 1) / 2 + e]);
+        }
     }
-    const float mean = wave_sum_comp(sum_r) / (float)g.ns;
-    CSum sq, rs;
-    for (int s = lane; s < g.ns; s += 64) {
-        int c, r;
-        sample_coords(g, s, c, r);
-        const float d = p.ref[(size_t)(uly + r) * w + (ulx + c)] - mean;
-        sq.add(d * d);
-        rs.add(d);
-    }
-    const float norm = sqrtf(wave_sum_comp(sq));
-    const float ref_sum = wave_sum_comp(rs);
+    // 21 Hessian entries + the reference sum: four transposing reductions
+    float t[4][6];
+#pragma unroll
+    for (int q = 0; q < 21; q++) t[q / 6][q % 6] = hs[q];
+    t[3][3] = sum_r;
+    t[3][4] = 0.f;
+    t[3][5] = 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; g++) wave_sum6(t[g]);
+    const float mean = t[3][3] / (float)ns;
 
-    CSum hs[21];
-    for (int s = lane; s < g.ns; s += 64) {
-        int c, r;
-        sample_coords(g, s, c, r);
-        const size_t gi = (size_t)(py + r - ry) * w + (px + c - rx);
-        const float gxv = p.gx[gi], gyv = p.gy[gi];
-        const float xl = (float)(c - rx) - ox, yl = (float)(r - ry) - oy;
-        const float sd[6] = { gxv, gxv * xl, gxv * yl, gyv, gyv * xl, gyv * yl };
-        int q = 0;
+    float sq = 0.f, rs = 0.f;
+    for (int k0 = 0; k0 < rows; k0 += kPrepRows) {
+        float rv[kPrepRows];
 #pragma unroll
-        for (int a = 0; a < 6; a++)
+        for (int k = 0; k < kPrepRows; k++) rv[k] = ref_col[(size_t)min(k0 + k, last) * step];
 #pragma unroll
-            for (int e = 0; e <= a; e++) hs[q++].add(sd[a] * sd[e]);
+        for (int k = 0; k < kPrepRows; k++) rv[k] = k0 + k < rows ? rv[k] : mean;  // past the end: d = 0
+#pragma unroll
+        for (int k = 0; k < kPrepRows; k++) {
+            const float d = rv[k] - mean;
+            sq = __builtin_fmaf(d, d, sq);
+            rs += d;
+        }
     }
-    double H[36];
-    {
-        int q = 0;
-#pragma unroll
-        for (int a = 0; a < 6; a++)
-#pragma unroll
-            for (int e = 0; e <= a; e++) {
-                const double v = (double)wave_sum_comp(hs[q++]);
-                H[a * 6 + e] = v;
-                H[e * 6 + a] = v;
-            }
-    }
-    double Hi[36];
-    inverse6_f64(H, Hi);
+    float u[3] = { sq, rs, 0.f };
+    wave_sum3(u);
+    const float norm = sqrtf(u[0]);
+    const float ref_sum = u[1];
 
+    double m[6][6];
+#pragma unroll
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int e = 0; e <= a; e++) {
+            const int q = a * (a + 1) / 2 + e;
+            const double v = (double)t[q / 6][q % 6];
+            m[a][e] = v;
+            m[e][a] = v;
+        }
+    int perm[6];
+    lu6_f64(m, perm);
+    if (lane < 6) {
+        double xc[6];
+        solve6_col_f64(m, perm, lane, xc);
+#pragma unroll
+        for (int r = 0; r < 6; r++) out->hinv[r * 6 + lane] = (float)xc[r];
+    }
     if (lane == 0) {
         out->mean = mean;
         out->norm = norm;
         out->status = 1;
         out->rx = rx;
         out->ry = ry;
-        float4* dst = reinterpret_cast<float4*>(out->hinv);
-#pragma unroll
-        for (int q = 0; q < 9; q++)
-            dst[q] = make_float4((float)Hi[4 * q], (float)Hi[4 * q + 1], (float)Hi[4 * q + 2],
-                                 (float)Hi[4 * q + 3]);
         out->ref_sum = ref_sum;
     }
 }

@@ -234,11 +234,14 @@ __global__ void k_gather(const fi_poi* __restrict__ src, const uint32_t* __restr
     }
 }
 
+// The whole-frame fallback's per-POI merge; flags (optional): every POI went
+// through FFTCC.
 __global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restrict__ scratch, int n,
-                              const unsigned* gate)
+                              const unsigned* gate, uint8_t* __restrict__ flags)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || (gate && *gate == 0u)) return;
+    if (flags) flags[i] = 1u;
     if (!take_frame_fallback(skip[i], scratch[i])) scratch[i] = skip[i];
 }
 
@@ -257,16 +260,18 @@ __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restric
 }
 
 // computeFrameQa (:629-700): valid = finite zncc >= 0; iteration statistics
-// over finite iteration counts; bad per isBadPoiForFallback.
+// over finite iteration counts; bad per isBadPoiForFallback.  Also the frame's
+// executed IC-GN iterations: the sum of iters[0..n_iters), the per-entry
+// counts every IC-GN launch of the frame wrote (zeroed at the frame's start).
 __global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict__ cur, int n,
                                                         FrameCounters* counters,
-                                                        const unsigned long long* __restrict__ iter_total)
+                                                        const uint32_t* __restrict__ iters, size_t n_iters)
 {
-    // the iteration total rides along with the counters (one read-back)
-    if (iter_total && blockIdx.x == 0 && threadIdx.x == 0) counters->iterations = *iter_total;
-    // grid-stride: a few hundred blocks, so the five counters see one atomic
+    // grid-stride: a few hundred blocks, so the six counters see one atomic
     // per block (same-address atomics serialise)
-    unsigned v[4] = { 0u, 0u, 0u, 0u }, mx[1] = { 0u };
+    unsigned v[5] = { 0u, 0u, 0u, 0u, 0u }, mx[1] = { 0u };
+    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_iters; j += (size_t)gridDim.x * blockDim.x)
+        v[4] += iters[j];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const fi_poi& p = cur[i];
         const float z = p.result[2];
@@ -279,26 +284,15 @@ __global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict_
             mx[0] = max(mx[0], (unsigned)(int)(it + 0.5f));
         }
     }
-    block_reduce<4>(v, mx);
+    block_reduce<5>(v, mx);
     if (threadIdx.x == 0) {
+        if (v[4]) atomicAdd(&counters->iterations, (unsigned long long)v[4]);
         if (v[0]) atomicAdd(&counters->valid, (unsigned long long)v[0]);
         if (v[1]) atomicAdd(&counters->bad, (unsigned long long)v[1]);
         if (v[2]) atomicAdd(&counters->iter_finite, (unsigned long long)v[2]);
         if (v[3]) atomicAdd(&counters->iter_sum, (unsigned long long)v[3]);
         if (mx[0]) atomicMax(&counters->iter_max, mx[0]);
     }
-}
-
-// Sum of per-entry executed iteration counts into *total.
-__global__ __launch_bounds__(kPolicyThreads) void k_sum_iters(const uint32_t* __restrict__ iters, int n,
-                                                               unsigned long long* total, const unsigned* count)
-{
-    n = bounded(n, count);
-    unsigned long long acc = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc += iters[i];
-    unsigned v[1] = { (unsigned)acc }, mx[1] = { 0u };
-    block_reduce<1>(v, mx);
-    if (threadIdx.x == 0 && v[0]) atomicAdd(total, (unsigned long long)v[0]);
 }
 
 static dim3 grid_for(int n) { return dim3((n + kPolicyThreads - 1) / kPolicyThreads); }
@@ -319,17 +313,17 @@ __global__ void k_fill_bytes(uint8_t* __restrict__ p, size_t n, uint8_t v, const
         p[i] = v;
 }
 
-// Start of a frame in one launch: zero the iteration total and the counters,
-// set every per-POI FFTCC-redo flag to v and, when src is given, copy bytes
-// (a multiple of 4; both buffers 16-byte aligned) from src to dst: the POI
-// template into the frame's queue, without a copy node of its own.
-__global__ void k_frame_reset(unsigned long long* __restrict__ total, uint32_t* __restrict__ words, size_t n_words,
-                              uint8_t* __restrict__ flags, size_t n, uint8_t v, const void* __restrict__ src,
-                              void* __restrict__ dst, size_t bytes)
+// Start of a frame in one launch: zero the counters and the per-entry
+// iteration counts, set every per-POI FFTCC-redo flag to v and, when src is
+// given, copy bytes (a multiple of 4; both buffers 16-byte aligned) from src to
+// dst: the POI template into the frame's queue, without a copy node of its own.
+__global__ void k_frame_reset(uint32_t* __restrict__ words, size_t n_words, uint32_t* __restrict__ words2,
+                              size_t n_words2, uint8_t* __restrict__ flags, size_t n, uint8_t v,
+                              const void* __restrict__ src, void* __restrict__ dst, size_t bytes)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
-    if (t == 0) *total = 0ull;
     for (size_t i = t; i < n_words; i += stride) words[i] = 0u;
+    for (size_t i = t; i < n_words2; i += stride) words2[i] = 0u;
     for (size_t i = t; i < n; i += stride) flags[i] = v;
     if (src) {
         const size_t n16 = bytes / 16;
@@ -342,12 +336,12 @@ __global__ void k_frame_reset(unsigned long long* __restrict__ total, uint32_t* 
     }
 }
 
-void launch_frame_reset(unsigned long long* total, uint32_t* words, size_t n_words, uint8_t* flags, size_t n,
-                        uint8_t v, hipStream_t s, const void* src, void* dst, size_t bytes)
+void launch_frame_reset(uint32_t* words, size_t n_words, uint32_t* words2, size_t n_words2, uint8_t* flags,
+                        size_t n, uint8_t v, hipStream_t s, const void* src, void* dst, size_t bytes)
 {
-    const size_t m = std::max(std::max(n_words, n), src ? bytes / 16 : 0);
+    const size_t m = std::max(std::max(std::max(n_words, n_words2), n), src ? bytes / 16 : 0);
     hipLaunchKernelGGL(k_frame_reset, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + 255) / 256, 1024))),
-                       dim3(256), 0, s, total, words, n_words, flags, n, v, src, dst, bytes);
+                       dim3(256), 0, s, words, n_words, words2, n_words2, flags, n, v, src, dst, bytes);
 }
 
 void launch_fill_bytes(uint8_t* p, size_t n, uint8_t v, hipStream_t s, const unsigned* gate)
@@ -457,9 +451,9 @@ void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, h
 }
 
 void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s,
-                                 const unsigned* gate)
+                                 const unsigned* gate, uint8_t* flags)
 {
-    if (n > 0) hipLaunchKernelGGL(k_merge_frame, grid_for(n), dim3(256), 0, s, skip, scratch_inout, n, gate);
+    if (n > 0) hipLaunchKernelGGL(k_merge_frame, grid_for(n), dim3(256), 0, s, skip, scratch_inout, n, gate, flags);
 }
 
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
@@ -470,19 +464,14 @@ void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* id
                            counters, count);
 }
 
-void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s,
-               const unsigned long long* iter_total)
+void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s, const uint32_t* iters,
+               size_t n_iters)
 {
-    hipLaunchKernelGGL(k_qa, dim3(std::max(1, std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks))),
-                       dim3(kPolicyThreads), 0, s, cur, n, counters, iter_total);
+    const size_t m = std::max((size_t)n, n_iters);
+    hipLaunchKernelGGL(k_qa, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + kPolicyThreads - 1) / kPolicyThreads,
+                                                                                    kReduceBlocks))),
+                       dim3(kPolicyThreads), 0, s, cur, n, counters, iters, n_iters);
 }
 
-void launch_sum_iters(const uint32_t* iters, int n, unsigned long long* total, hipStream_t s,
-                      const unsigned* count)
-{
-    if (n > 0)
-        hipLaunchKernelGGL(k_sum_iters, dim3(std::min((n + kPolicyThreads - 1) / kPolicyThreads, kReduceBlocks)),
-                           dim3(kPolicyThreads), 0, s, iters, n, total, count);
-}
 
 }  // namespace fi

@@ -41,6 +41,13 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_set_pois", self._h, nat.ptr(q), len(q))
         self.n = len(q)
 
+    def prepare(self) -> dict:
+        """Prepare the reference side now (if needed); the timing of the last
+        preparation (fi_prepare_stats)."""
+        st = nat.PrepareStats()
+        nat.call("fi_frame_prepare", self._h, ctypes.byref(st))
+        return st.as_dict()
+
     def upload_target(self, tar: Image2D) -> None:
         """Queue a target for the next run(); the copy to the device proceeds
         in the background (at most two targets queued)."""
@@ -58,17 +65,21 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))
         return out
 
-    def records(self) -> np.ndarray:
-        """The last frame's 36-byte output records (x, y, u, v, ux, uy, vx,
-        vy, zncc; rawio.RAW_DTYPE), packed on the device, as a copy."""
+    def records(self, age: int = 0, copy: bool = True) -> np.ndarray:
+        """36-byte output records (x, y, u, v, ux, uy, vx, vy, zncc;
+        rawio.RAW_DTYPE) of the last frame (age 0) or the one before (age 1,
+        with set_records on), packed on the device.  copy=False returns a
+        view of the driver's pinned buffer, valid until the second run after
+        that frame."""
         from .rawio import RAW_DTYPE
         p = ctypes.c_void_p()
         m = ctypes.c_size_t(0)
-        nat.call("fi_frame_records", self._h, ctypes.byref(p), ctypes.byref(m))
+        nat.call("fi_frame_records", self._h, int(age), ctypes.byref(p), ctypes.byref(m))
         if m.value == 0:
             return np.zeros(0, dtype=RAW_DTYPE)
         buf = (ctypes.c_char * (36 * m.value)).from_address(p.value)
-        return np.frombuffer(buf, dtype=RAW_DTYPE).copy()
+        view = np.frombuffer(buf, dtype=RAW_DTYPE)
+        return view.copy() if copy else view
 
     def set_records(self, on: bool = True) -> None:
         """Pack every frame's records on the device and copy them to host

@@ -239,6 +239,23 @@ class ICGN2D1(_Handle):
             raise ValueError("center_offsets must be one (x, y) pair per POI")
         nat.call("fi_icgn_compute_offset", self._h, nat.ptr(q), len(q), nat.ptr(off))
 
+    def export(self, what: str, shape=None, n_records: int = 0) -> np.ndarray:
+        """A device-side array of the engine (fi_icgn_export): 'ref', 'gx',
+        'gy', 'tar' (h x w f32), 'bspline' (h x w x 16 f32, coef[k][l] at
+        k * 4 + l) with shape = (h, w), or 'prep' (the last compute's
+        n_records reference records, PREP_DTYPE)."""
+        kinds = {"ref": nat.FI_EXPORT_REF_F32, "gx": nat.FI_EXPORT_GRAD_X, "gy": nat.FI_EXPORT_GRAD_Y,
+                 "tar": nat.FI_EXPORT_TAR_F32, "bspline": nat.FI_EXPORT_BSPLINE, "prep": nat.FI_EXPORT_PREP}
+        if what not in kinds:
+            raise ValueError(f"export must be one of {sorted(kinds)}")
+        if what == "prep":
+            out = np.zeros(n_records, dtype=nat.PREP_DTYPE)
+        else:
+            h, w = shape
+            out = np.zeros((h, w, 16) if what == "bspline" else (h, w), dtype=np.float32)
+        nat.call("fi_icgn_export", self._h, kinds[what], nat.ptr(out), out.nbytes)
+        return out
+
 
 def format_raw(queue: np.ndarray) -> bytes:
     """36-byte records x,y (int32) u,v,ux,uy,vx,vy,zncc (float32), little endian

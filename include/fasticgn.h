@@ -191,6 +191,26 @@ int fi_icgn_compute(fi_icgn* h, fi_poi* pois, size_t n);
  * per-POI statuses (-5 for NaN, -3 out of range). */
 int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* center_offsets);
 
+/* Copies one of the engine's device-side arrays to host memory, for tests of
+ * the preparation stages against the oracle; bytes must be the array's exact
+ * size.  No reference counterpart (the reference keeps them private):
+ *   FI_EXPORT_REF_F32   reference as f32 (w * h floats)         after prepareRef
+ *   FI_EXPORT_GRAD_X/Y  Gradient2D4 of the reference (w * h)    after prepareRef
+ *   FI_EXPORT_TAR_F32   target as f32 (w * h)                   after prepareTar
+ *   FI_EXPORT_BSPLINE   BicubicBspline table, texel-major, 16 floats coef[k][l]
+ *                       (k * 4 + l) per texel (w * h * 16)      after prepareTar
+ *   FI_EXPORT_PREP      the last compute's per-POI reference records, 176 bytes
+ *                       each: mean, norm (f32), status, rx (int32), inverse
+ *                       Hessian (36 f32, row-major), sum (ref - mean) (f32),
+ *                       ry (int32), 8 bytes of padding */
+#define FI_EXPORT_REF_F32 0
+#define FI_EXPORT_GRAD_X 1
+#define FI_EXPORT_GRAD_Y 2
+#define FI_EXPORT_TAR_F32 3
+#define FI_EXPORT_BSPLINE 4
+#define FI_EXPORT_PREP 5
+int fi_icgn_export(fi_icgn* h, int what, void* out, size_t bytes);
+
 /* ---- multi-frame driver --------------------------------------------------
  * The fixed-reference sequence of opencorr_cli_cuda_mvp.cpp:3834-4509 with the
  * frame policy of processFrame (:3024-3210): the reference, its gradients,
@@ -229,15 +249,17 @@ int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
  * (an FFTCC-seeded frame, the whole-frame fallback, POI-level repair or the
  * learned seed's fallback), else 0. */
 int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n);
-/* The last frame's output records (formatRawBinary, cli:2498-2536, no output
+/* Output records of a frame (formatRawBinary, cli:2498-2536, no output
  * filter): n 36-byte records x, y (int32), u, v, ux, uy, vx, vy, zncc
- * (float32) in template order, packed on the device.  *records points into
- * pinned memory the driver owns; it stays valid until the second fi_frame_run
- * after the frame it holds.  With fi_frame_set_records on, every run packs its
- * records and copies them to host memory on a copy stream that overlaps the
- * next frame, and this call only waits for that copy; otherwise it packs and
- * copies the last frame now.  *n_records (may be NULL) = n. */
-int fi_frame_records(fi_frame* h, const void** records, size_t* n_records);
+ * (float32) in template order, packed on the device.  age 0 = the last frame
+ * run, 1 = the one before it.  *records points into pinned memory the driver
+ * owns; it stays valid until the second fi_frame_run after the frame it
+ * holds.  With fi_frame_set_records on, every run packs its records and
+ * copies them to host memory on a copy stream that overlaps the next frame,
+ * and this call only waits for that copy (so the records of age 1, read
+ * after the next run, cost the host no wait); otherwise the last frame's
+ * (age 0 only) are packed and copied now.  *n_records (may be NULL) = n. */
+int fi_frame_records(fi_frame* h, int age, const void** records, size_t* n_records);
 int fi_frame_set_records(fi_frame* h, int on);
 /* Two-frame temporal extrapolation of the seeds (seedPoiQueueFromPrevious
  * with --temporal-extrapolation, cli:560-624, 3364-3371): on a frame seeded
@@ -248,6 +270,20 @@ int fi_frame_set_records(fi_frame* h, int on);
  * subset radius).  Off by default, as in the reference.  fi_frame_stats
  * counts the extrapolated seeds. */
 int fi_frame_set_temporal_extrapolation(fi_frame* h, int on, float max_delta);
+/* Reference-side preparation of the driver (prepareRef of both engines,
+ * cli:4095-4104): device buffers, reference f32 + Gradient2D4, FFTCC reference
+ * spectra and the per-POI IC-GN records (mean, norm, inverse Hessian).  The
+ * first fi_frame_run after a new reference or template does it by itself;
+ * fi_frame_prepare does it now (if needed) and reports the timing of the last
+ * preparation. */
+typedef struct fi_prepare_stats {
+    double total_ms;      /* host wall time of the preparation */
+    double alloc_ms;      /* device buffer allocation (host wall time) */
+    double reference_ms;  /* reference f32 + gradients (device, HIP events) */
+    double fftcc_ms;      /* FFTCC reference spectra (device) */
+    double hessian_ms;    /* per-POI records: mean, norm, FP64 inverse Hessian (device) */
+} fi_prepare_stats;
+int fi_frame_prepare(fi_frame* h, fi_prepare_stats* stats);
 /* Forget the previous frames (next run seeds by FFTCC again). */
 int fi_frame_reset_sequence(fi_frame* h);
 /* IC-GN lane-to-POI shape for this driver (see fi_icgn_set_shape). */

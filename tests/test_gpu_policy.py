@@ -143,9 +143,16 @@ def test_device_records_equal_formatted_pois(oracle):
         d.set_reference(fa.Image2D(seq.ref, depth12="right"))
         d.set_pois(seq.pois)
         d.upload_target(fa.Image2D(seq.frame(0), depth12="right"))
+        last = None
         for k in range(1, 6):
             d.upload_target(fa.Image2D(seq.frame(k), depth12="right"))
             d.run()
+            if last is not None:  # the frame before, its copy overlapped this frame
+                assert d.records(1).tobytes() == last, k
             rec = d.records()
             res = d.download()
-            assert rec.tobytes() == rawio.format_raw(res).tobytes(), k
+            last = rawio.format_raw(res).tobytes()
+            assert rec.tobytes() == last, k
+        with pytest.raises(fa.LogicError):
+            d.reset_sequence()
+            d.records()
