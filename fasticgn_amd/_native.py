@@ -23,7 +23,8 @@ FI_E_UNSUPPORTED = -5
 FI_E_NO_MEMORY = -6
 
 FI_PIXEL_U8, FI_PIXEL_U16, FI_PIXEL_F32 = 0, 1, 2
-FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT, FI_DEPTH12_LEFT = 0, 1, 2
+FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT, FI_DEPTH12_LEFT, FI_DEPTH12_AUTO, FI_DEPTH_AUTO, FI_DEPTH16 = 0, 1, 2, 3, 4, 5
+FI_NORMALIZE_NONE, FI_NORMALIZE_TO8, FI_NORMALIZE_TO16, FI_NORMALIZE_UNIT = 0x00, 0x10, 0x20, 0x30
 FI_POLICY_SKIP_FFTCC = 1
 FI_POLICY_POI_FALLBACK = 2
 FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
@@ -75,6 +76,15 @@ class FrameStats(ctypes.Structure):
         ("temporal_seed_count", ctypes.c_int32),
         ("temporal_seed_used", ctypes.c_int32),
     ]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class IngestInfo(ctypes.Structure):
+    """fi_ingest_info (include/fasticgn.h)."""
+    _fields_ = [("effective_bits", ctypes.c_int32), ("left_aligned", ctypes.c_int32),
+                ("source_max", ctypes.c_double), ("scale", ctypes.c_double)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -135,6 +145,7 @@ SIGNATURES = {
     "fi_device_synchronize": [_I],
     "fi_last_error": [ctypes.c_char_p, _S],
     "fi_dft_twiddles": [_I, _P],
+    "fi_image_resolve": [_P, _I, _I, _S, _I, _I, ctypes.POINTER(IngestInfo)],
     "fi_conv_threshold2": [_F, ctypes.POINTER(ctypes.c_double)],
     "fi_fftcc_create": [_I, _I, _I, ctypes.POINTER(_P)],
     "fi_fftcc_destroy": [_P],
@@ -149,6 +160,7 @@ SIGNATURES = {
     "fi_icgn_destroy": [_P],
     "fi_icgn_set_iteration": [_P, _F, _F],
     "fi_icgn_set_self_adaptive": [_P, _I],
+    "fi_icgn_set_subset": [_P, _I, _I],
     "fi_icgn_set_shape": [_P, _I, _I],
     "fi_icgn_set_sampling": [_P, _I],
     "fi_icgn_set_ref": [_P, _P, _I, _I, _S, _I, _I],

@@ -86,6 +86,23 @@ int fi_last_error(char* buf, size_t len)
     return (int)m.size();
 }
 
+int fi_image_resolve(const void* px, int width, int height, size_t stride_bytes, int pixel, int mode,
+                     fi_ingest_info* out)
+{
+    return guarded([&] {
+        check_image_args(px, width, height, stride_bytes, pixel, mode);
+        if (!out) fail(FI_E_INVALID_ARGUMENT, "null output");
+        unsigned mx = 0;
+        if (pixel == FI_PIXEL_U16)
+            for (int y = 0; y < height; y++) {
+                const uint16_t* row = reinterpret_cast<const uint16_t*>(static_cast<const uint8_t*>(px) +
+                                                                       (size_t)y * stride_bytes);
+                for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
+            }
+        *out = resolve_ingest(pixel, mode, mx);
+    });
+}
+
 int fi_dft_twiddles(int n, float* out)
 {
     return guarded([&] {
@@ -117,14 +134,14 @@ int fi_fftcc_destroy(fi_fftcc* h)
     return guarded([&] { delete h; });
 }
 
-int fi_fftcc_set_ref(fi_fftcc* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+int fi_fftcc_set_ref(fi_fftcc* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
+    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, mode); });
 }
 
-int fi_fftcc_set_tar(fi_fftcc* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+int fi_fftcc_set_tar(fi_fftcc* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->set_tar(px, w, hh, stride, pixel, d12); });
+    return guarded([&] { need(h); h->set_tar(px, w, hh, stride, pixel, mode); });
 }
 
 int fi_fftcc_prepare_ref_clear(fi_fftcc* h)
@@ -180,6 +197,18 @@ int fi_icgn_set_iteration(fi_icgn* h, float conv, float stop)
     });
 }
 
+int fi_icgn_set_subset(fi_icgn* h, int rx, int ry)
+{
+    return guarded([&] {
+        need(h);
+        if (rx < 1 || ry < 1) fail(FI_E_INVALID_ARGUMENT, "ICGN2D1 subset radii must be positive");
+        if (rx > kIcgnMaxRadius || ry > kIcgnMaxRadius)
+            fail(FI_E_UNSUPPORTED, "ICGN on the device supports radii 1.." + std::to_string(kIcgnMaxRadius));
+        h->rx = rx;
+        h->ry = ry;
+    });
+}
+
 int fi_icgn_set_self_adaptive(fi_icgn* h, int on)
 {
     return guarded([&] {
@@ -213,14 +242,14 @@ int fi_icgn_set_sampling(fi_icgn* h, int mode)
     });
 }
 
-int fi_icgn_set_ref(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+int fi_icgn_set_ref(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
+    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, mode); });
 }
 
-int fi_icgn_set_tar(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+int fi_icgn_set_tar(fi_icgn* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->set_tar(px, w, hh, stride, pixel, d12); });
+    return guarded([&] { need(h); h->set_tar(px, w, hh, stride, pixel, mode); });
 }
 
 int fi_icgn_prepare_ref(fi_icgn* h)
@@ -276,9 +305,9 @@ int fi_frame_destroy(fi_frame* h)
     return guarded([&] { delete h; });
 }
 
-int fi_frame_set_ref(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+int fi_frame_set_ref(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, d12); });
+    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, mode); });
 }
 
 int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n)
@@ -286,9 +315,9 @@ int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n)
     return guarded([&] { need(h); h->set_pois(pois, n); });
 }
 
-int fi_frame_upload_target(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int d12)
+int fi_frame_upload_target(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->upload_target(px, w, hh, stride, pixel, d12); });
+    return guarded([&] { need(h); h->upload_target(px, w, hh, stride, pixel, mode); });
 }
 
 int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats)

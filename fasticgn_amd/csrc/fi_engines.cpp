@@ -103,10 +103,10 @@ fi_fftcc::~fi_fftcc()
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-void fi_fftcc::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+void fi_fftcc::set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
     DeviceGuard g(device);
-    ref.upload(px, w, h, stride, pixel, depth12, stream);
+    ref.upload(px, w, h, stride, pixel, mode, stream);
     ref_f.reserve((size_t)w * h);
     launch_to_float(ref.view, ref_f.p, stream);
     FI_HIP(hipGetLastError());
@@ -114,10 +114,10 @@ void fi_fftcc::set_ref(const void* px, int w, int h, size_t stride, int pixel, i
     ready = false;  // layout retained for a lazy rebuild (:188-195)
 }
 
-void fi_fftcc::set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+void fi_fftcc::set_tar(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
     DeviceGuard g(device);
-    tar.upload(px, w, h, stride, pixel, depth12, stream);
+    tar.upload(px, w, h, stride, pixel, mode, stream);
     tar_f.reserve((size_t)w * h);
     launch_to_float(tar.view, tar_f.p, stream);
     FI_HIP(hipGetLastError());
@@ -301,19 +301,19 @@ fi_icgn::~fi_icgn()
     if (stream) (void)hipStreamDestroy(stream);
 }
 
-void fi_icgn::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+void fi_icgn::set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
     check_image_offsets(w, h);
     DeviceGuard g(device);
-    ref_raw.upload(px, w, h, stride, pixel, depth12, stream);
+    ref_raw.upload(px, w, h, stride, pixel, mode, stream);
     grads_ready = false;
     ref_version++;
 }
 
-void fi_icgn::set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+void fi_icgn::set_tar(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
     DeviceGuard g(device);
-    tar_raw.upload(px, w, h, stride, pixel, depth12, stream);
+    tar_raw.upload(px, w, h, stride, pixel, mode, stream);
     coef_ready = false;
 }
 
@@ -589,7 +589,7 @@ void fi_frame::stage_loop()
         std::string msg;
         try {
             DeviceGuard g(device);
-            t.img.stage(job.px, job.w, job.h, job.stride, job.pixel, job.depth12);
+            t.img.stage(job.px, job.w, job.h, job.stride, job.pixel, job.mode);
             t.img.dma(copy_stream);
             FI_HIP(hipEventRecord(t.ready, copy_stream));
         } catch (const Error& e) {
@@ -642,11 +642,11 @@ void fi_frame::invalidate_graphs()
     graphs.clear();
 }
 
-void fi_frame::set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+void fi_frame::set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
     check_image_offsets(w, h);
     DeviceGuard g(device);
-    ref_raw.upload(px, w, h, stride, pixel, depth12, stream);
+    ref_raw.upload(px, w, h, stride, pixel, mode, stream);
     ref_set = true;
     prepared = false;
     reset_sequence();
@@ -674,13 +674,13 @@ void fi_frame::set_self_adaptive(bool on)
 // copied to pinned memory and sent to the device by the staging thread, so the
 // copy overlaps a frame that is running.  The caller's pixels must stay valid
 // until the next run (or upload) returns.
-void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12)
+void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
     DeviceGuard g(device);
     if (ref_set && (w != ref_raw.view.width || h != ref_raw.view.height))
         fail(FI_E_INVALID_ARGUMENT, "reference " + size_str(ref_raw.view.width, ref_raw.view.height) +
                                         " and target " + size_str(w, h) + " sizes differ");
-    check_image_args(px, w, h, stride, pixel, depth12);
+    check_image_args(px, w, h, stride, pixel, mode);
     // 12-bit range: checked by the staging thread as it copies the rows
     // (reported by the run that takes this target)
     if (pending.size() >= 2) fail(FI_E_LOGIC, "two targets are already queued; run a frame first");
@@ -691,7 +691,7 @@ void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pi
         stage_cv.wait(lk, [&] { return tslot[s].state != 1; });
         tslot[s].state = 1;
         tslot[s].err_code = 0;
-        stage_jobs.push_back({ s, px, w, h, stride, pixel, depth12 });
+        stage_jobs.push_back({ s, px, w, h, stride, pixel, mode });
     }
     stage_cv.notify_all();
     pending.push_back(s);

@@ -40,8 +40,8 @@ struct fi_fftcc {
 
     fi_fftcc(int rx, int ry, int dev);
     ~fi_fftcc();
-    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
-    void set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode);
+    void set_tar(const void* px, int w, int h, size_t stride, int pixel, int mode);
     void prepare_ref_clear();
     void prepare_ref(const fi_poi* layout, size_t n);
     void compute(fi_poi* pois, size_t n);
@@ -78,8 +78,8 @@ struct fi_icgn {
 
     fi_icgn(int rx, int ry, float conv, float stop, int dev);
     ~fi_icgn();
-    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
-    void set_tar(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode);
+    void set_tar(const void* px, int w, int h, size_t stride, int pixel, int mode);
     void prepare_ref();
     void prepare_tar();
     // offsets: n (x, y) pairs (compute(POI2D*, Point2D&)), or nullptr
@@ -144,7 +144,7 @@ struct fi_frame {
         const void* px;
         int w, h;
         size_t stride;
-        int pixel, depth12;
+        int pixel, mode;
     };
     std::deque<StageJob> stage_jobs;
     bool stage_stop = false;
@@ -196,10 +196,10 @@ struct fi_frame {
 
     fi_frame(int radius, float conv, float stop, int dev);
     ~fi_frame();
-    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode);
     void set_pois(const fi_poi* pois, size_t n);
     void set_self_adaptive(bool on);
-    void upload_target(const void* px, int w, int h, size_t stride, int pixel, int depth12);
+    void upload_target(const void* px, int w, int h, size_t stride, int pixel, int mode);
     void run(unsigned policy, fi_frame_stats* stats);
     void download(fi_poi* out, size_t n);
     // Raw records of the frame `age` runs back (0: the last) in pinned host

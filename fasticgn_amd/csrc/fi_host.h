@@ -94,18 +94,21 @@ struct DeviceImage {
     PinnedArray<uint8_t> stage_buf;  // host staging: one contiguous DMA per upload
     DevImage view;
     bool set = false;
-    void upload(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12,
+    void upload(const void* px, int width, int height, size_t stride_bytes, int pixel, int mode,
                 hipStream_t s);
     // The two halves of upload: pack the rows into the pinned staging buffer
     // (host), then one DMA on stream s that is not waited for.  stage rejects a
     // right-aligned 12-bit image with a value above 4095 (src/oc_image.cpp:172-178).
-    void stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12);
+    void stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int mode);
     void dma(hipStream_t s);
     size_t staged_bytes = 0;
+    fi_ingest_info info{};  // how the last staged image is converted
 };
 
+// mode: FI_DEPTH_* | FI_NORMALIZE_* (include/fasticgn.h)
 void check_image_args(const void* px, int width, int height, size_t stride_bytes, int pixel,
-                      int depth12);
+                      int mode);
+fi_ingest_info resolve_ingest(int pixel, int mode, unsigned max_value);
 
 // Selects the handle's device for the duration of a call.
 struct DeviceGuard {

@@ -19,7 +19,7 @@ static size_t bytes_per_pixel(int pixel)
 }
 
 void check_image_args(const void* px, int width, int height, size_t stride_bytes, int pixel,
-                      int depth12)
+                      int mode)
 {
     if (px == nullptr) fail(FI_E_INVALID_ARGUMENT, "image pointer is null");
     if (width < 1 || height < 1)
@@ -27,37 +27,75 @@ void check_image_args(const void* px, int width, int height, size_t stride_bytes
                                         "x" + std::to_string(height));
     if (pixel != kU8 && pixel != kU16 && pixel != kF32)
         fail(FI_E_INVALID_ARGUMENT, "unknown pixel format " + std::to_string(pixel));
-    if (depth12 != FI_DEPTH_NATIVE && depth12 != FI_DEPTH12_RIGHT && depth12 != FI_DEPTH12_LEFT)
-        fail(FI_E_INVALID_ARGUMENT, "unknown 12-bit alignment " + std::to_string(depth12));
-    if (depth12 != FI_DEPTH_NATIVE && pixel != kU16)
-        fail(FI_E_INVALID_ARGUMENT, "12-bit alignment applies to 16-bit pixels only");
+    const int depth = mode & FI_DEPTH_MASK, norm = mode & FI_NORMALIZE_MASK;
+    if ((mode & ~(FI_DEPTH_MASK | FI_NORMALIZE_MASK)) || depth > FI_DEPTH16 || norm > FI_NORMALIZE_UNIT)
+        fail(FI_E_INVALID_ARGUMENT, "unknown image input mode " + std::to_string(mode));
+    if (pixel == kF32 && mode != FI_DEPTH_NATIVE)
+        fail(FI_E_INVALID_ARGUMENT, "float pixels are taken as stored (no depth or normalise mode)");
     if (stride_bytes < (size_t)width * bytes_per_pixel(pixel))
         fail(FI_E_INVALID_ARGUMENT, "row stride " + std::to_string(stride_bytes) +
                                         " is shorter than a row");
 }
 
-void DeviceImage::stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int depth12)
+// loadImage2DMat (src/oc_image.cpp:125-217) on decoded pixels: the effective
+// depth and alignment (inferred from the maximum where the mode says so) and
+// the scale, formed in FP64 as there and applied on the device as
+// float(pixel) * float(scale).  A right-aligned 12-bit image above 4095 is
+// rejected (:172-178).
+fi_ingest_info resolve_ingest(int pixel, int mode, unsigned max_value)
 {
-    check_image_args(px, width, height, stride_bytes, pixel, depth12);
+    fi_ingest_info in{};
+    const int depth = mode & FI_DEPTH_MASK, norm = mode & FI_NORMALIZE_MASK;
+    in.source_max = (double)max_value;
+    in.effective_bits = pixel == kU8 ? 8 : pixel == kF32 ? 32 : 16;
+    in.left_aligned = 0;
+    if (pixel != kF32) {
+        if (depth == FI_DEPTH12_RIGHT) in.effective_bits = 12;
+        else if (depth == FI_DEPTH12_LEFT) { in.effective_bits = 12; in.left_aligned = 1; }
+        else if (depth == FI_DEPTH12_AUTO) { in.effective_bits = 12; in.left_aligned = max_value > 4095; }
+        else if (depth == FI_DEPTH16) in.effective_bits = 16;
+        else if (depth == FI_DEPTH_AUTO && pixel == kU16) in.effective_bits = max_value <= 4095 ? 12 : 16;
+        if (in.effective_bits == 12 && !in.left_aligned && max_value > 4095)
+            fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
+    }
+    double scale = in.left_aligned ? 1.0 / 16.0 : 1.0;
+    const double logical = in.effective_bits == 8 ? 255.0 : in.effective_bits == 12 ? 4095.0 : 65535.0;
+    if (norm == FI_NORMALIZE_TO8) scale *= 255.0 / logical;
+    else if (norm == FI_NORMALIZE_TO16) scale *= 65535.0 / logical;
+    else if (norm == FI_NORMALIZE_UNIT) scale *= 1.0 / logical;
+    in.scale = scale;
+    return in;
+}
+
+// The modes that need the data's maximum (a range check or an inference).
+static bool needs_max(int pixel, int mode)
+{
+    const int depth = mode & FI_DEPTH_MASK;
+    return pixel == kU16 && (depth == FI_DEPTH12_RIGHT || depth == FI_DEPTH12_AUTO || depth == FI_DEPTH_AUTO);
+}
+// (8-bit data never exceeds 4095: its maximum decides nothing)
+
+void DeviceImage::stage(const void* px, int width, int height, size_t stride_bytes, int pixel, int mode)
+{
+    check_image_args(px, width, height, stride_bytes, pixel, mode);
     const size_t row_bytes = (size_t)width * bytes_per_pixel(pixel);
     const size_t total = row_bytes * height;
     // rows packed into pinned memory (the copy out of pageable memory is a
     // plain memcpy; the DMA then runs at the link rate)
     this->stage_buf.reserve(total);
-    // a right-aligned 12-bit image is range checked on the staged rows (each
-    // still in cache after its copy): one pass over the caller's pixels
-    const bool check12 = depth12 == FI_DEPTH12_RIGHT && pixel == FI_PIXEL_U16;
+    // the maximum a mode needs is taken on the staged rows (each still in
+    // cache after its copy): one pass over the caller's pixels
+    const bool scan = needs_max(pixel, mode);
     uint16_t mx = 0;
     for (int y = 0; y < height; y++) {
         uint8_t* dst = stage_buf.p + (size_t)y * row_bytes;
         std::memcpy(dst, static_cast<const uint8_t*>(px) + (size_t)y * stride_bytes, row_bytes);
-        if (check12) {
+        if (scan) {
             const uint16_t* row = reinterpret_cast<const uint16_t*>(dst);
             for (int x = 0; x < width; x++) mx = row[x] > mx ? row[x] : mx;
         }
     }
-    // src/oc_image.cpp:172-178: a right-aligned 12-bit image above 4095 is rejected
-    if (mx > 4095) fail(FI_E_INVALID_ARGUMENT, "12-bit right-aligned input has values above 4095");
+    info = resolve_ingest(pixel, mode, mx);
     raw.reserve(total);
     staged_bytes = total;
     view.data = raw.p;
@@ -65,7 +103,7 @@ void DeviceImage::stage(const void* px, int width, int height, size_t stride_byt
     view.height = height;
     view.pixel = pixel;
     view.pitch = (size_t)width;
-    view.scale = (depth12 == FI_DEPTH12_LEFT) ? (1.f / 16.f) : 1.f;
+    view.scale = (float)info.scale;
 }
 
 void DeviceImage::dma(hipStream_t s)
@@ -75,9 +113,9 @@ void DeviceImage::dma(hipStream_t s)
 }
 
 void DeviceImage::upload(const void* px, int width, int height, size_t stride_bytes, int pixel,
-                         int depth12, hipStream_t s)
+                         int mode, hipStream_t s)
 {
-    stage(px, width, height, stride_bytes, pixel, depth12);
+    stage(px, width, height, stride_bytes, pixel, mode);
     dma(s);
     FI_HIP(hipStreamSynchronize(s));  // the staging buffer is reused by the next upload
 }

@@ -51,15 +51,23 @@ def _check_queue(queue: np.ndarray) -> np.ndarray:
 
 
 class Image2D:
-    """A grey image: row-major uint8, uint16 or float32 pixels.
+    """A grey image: row-major uint8, uint16 or float32 pixels, and the input
+    mode that converts them (loadImage2DMat, src/oc_image.cpp:125-217).
 
-    depth12 selects the 12-bit handling of 16-bit input
-    (src/oc_image.cpp:163-185): 'native', 'right' (values must be <= 4095) or
-    'left' (divided by 16)."""
+    depth12: 'native' (the stored type's depth), 'right' (12-bit, values must
+    be <= 4095), 'left' (12-bit, divided by 16), 'auto12' (12-bit,
+    left-aligned when the maximum exceeds 4095), '16' (16-bit) or 'auto'
+    (16-bit data up to 4095 is 12-bit right-aligned, else 16-bit; 8-bit data
+    8-bit).  normalize: 'none',
+    'to8', 'to16' or 'unit' (0..255, 0..65535 or 0..1 from the logical maximum
+    of the effective depth)."""
 
-    _DEPTH = {"native": nat.FI_DEPTH_NATIVE, "right": nat.FI_DEPTH12_RIGHT, "left": nat.FI_DEPTH12_LEFT}
+    _DEPTH = {"native": nat.FI_DEPTH_NATIVE, "right": nat.FI_DEPTH12_RIGHT, "left": nat.FI_DEPTH12_LEFT,
+              "auto12": nat.FI_DEPTH12_AUTO, "auto": nat.FI_DEPTH_AUTO, "16": nat.FI_DEPTH16}
+    _NORMALIZE = {"none": nat.FI_NORMALIZE_NONE, "to8": nat.FI_NORMALIZE_TO8, "to16": nat.FI_NORMALIZE_TO16,
+                  "unit": nat.FI_NORMALIZE_UNIT}
 
-    def __init__(self, pixels: np.ndarray, depth12: str = "native"):
+    def __init__(self, pixels: np.ndarray, depth12: str = "native", normalize: str = "none"):
         a = np.ascontiguousarray(pixels)
         if a.ndim != 2:
             raise ValueError("Image2D needs a 2-D array")
@@ -73,19 +81,97 @@ class Image2D:
             raise TypeError(f"unsupported pixel type {a.dtype}")
         if depth12 not in self._DEPTH:
             raise ValueError(f"depth12 must be one of {sorted(self._DEPTH)}")
+        if normalize not in self._NORMALIZE:
+            raise ValueError(f"normalize must be one of {sorted(self._NORMALIZE)}")
         self.data = a
-        self.depth12 = self._DEPTH[depth12]
+        self.mode = self._DEPTH[depth12] | self._NORMALIZE[normalize]
         self.height, self.width = a.shape
+        self.file_path = ""
+
+    @property
+    def depth12(self) -> int:
+        return self.mode
 
     def args(self):
-        return (nat.ptr(self.data), self.width, self.height, self.data.strides[0],
-                self.pixel, self.depth12)
+        return (nat.ptr(self.data), self.width, self.height, self.data.strides[0], self.pixel, self.mode)
+
+    def info(self) -> dict:
+        """The effective depth, alignment and scale of this image's mode
+        (ImageLoadInfo), as an upload resolves them (fi_image_resolve)."""
+        out = nat.IngestInfo()
+        nat.call("fi_image_resolve", *self.args(), ctypes.byref(out))
+        return out.as_dict()
 
     def as_float(self) -> np.ndarray:
-        a = self.data.astype(np.float32)
-        if self.depth12 == nat.FI_DEPTH12_LEFT:
-            a *= np.float32(1.0 / 16.0)
-        return a
+        """The pixels as the device converts them: float(p) * float(scale)."""
+        scale = np.float32(self.info()["scale"])
+        return (self.data.astype(np.float32) * scale).astype(np.float32)
+
+    @classmethod
+    def load(cls, path, depth: str = "compat8", alignment: str = "auto", normalize: str = "none") -> "Image2D":
+        """Image2D(file_path) / loadWithOptions (src/oc_image.cpp:229-285) for
+        PGM files (P2 / P5, 8- or 16-bit).  depth 'compat8' (the reference's
+        default: 8-bit, 16-bit data reduced to its high byte), '12' (with
+        alignment 'right', 'left' or 'auto'), '16' or 'auto'."""
+        px = read_pgm(path)
+        if depth == "compat8":
+            if px.dtype == np.uint16:
+                px = (px >> 8).astype(np.uint8)
+            d = "native"
+        elif depth == "12":
+            d = {"right": "right", "left": "left", "auto": "auto12"}[alignment]
+        elif depth == "16":
+            d = "16"
+        elif depth == "auto":
+            d = "auto" if px.dtype == np.uint16 else "native"
+        else:
+            raise ValueError("depth must be 'compat8', '12', '16' or 'auto'")
+        img = cls(px, d, normalize)
+        img.file_path = str(path)
+        return img
+
+
+def read_pgm(path) -> np.ndarray:
+    """Decodes a PGM file (P5 binary or P2 ASCII; maxval up to 255 gives
+    uint8, up to 65535 big-endian uint16)."""
+    blob = open(path, "rb").read()
+    fields, pos = [], 0
+    while len(fields) < 4:
+        while pos < len(blob) and blob[pos:pos + 1].isspace():
+            pos += 1
+        if pos < len(blob) and blob[pos:pos + 1] == b"#":
+            while pos < len(blob) and blob[pos:pos + 1] not in (b"\n", b"\r"):
+                pos += 1
+            continue
+        start = pos
+        while pos < len(blob) and not blob[pos:pos + 1].isspace():
+            pos += 1
+        if start == pos:
+            raise ValueError(f"Fail to load file: {path}")
+        fields.append(blob[start:pos])
+    magic, w, h, maxval = fields[0], int(fields[1]), int(fields[2]), int(fields[3])
+    if magic not in (b"P5", b"P2") or w < 1 or h < 1 or not 0 < maxval < 65536:
+        raise ValueError(f"Fail to load file: {path}")
+    dtype = np.uint8 if maxval < 256 else np.dtype(">u2")
+    if magic == b"P5":
+        n = w * h * np.dtype(dtype).itemsize
+        data = np.frombuffer(blob, dtype=dtype, count=w * h, offset=pos + 1) if len(blob) >= pos + 1 + n else None
+        if data is None:
+            raise ValueError(f"Fail to load file: {path}")
+    else:
+        data = np.array(blob[pos:].split()[: w * h], dtype=np.int64)
+        if data.size != w * h:
+            raise ValueError(f"Fail to load file: {path}")
+    return data.reshape(h, w).astype(np.uint8 if maxval < 256 else np.uint16)
+
+
+def write_pgm(path, px: np.ndarray) -> None:
+    """A binary PGM (P5) of uint8 or uint16 pixels."""
+    px = np.ascontiguousarray(px)
+    maxval = 255 if px.dtype == np.uint8 else 65535
+    with open(path, "wb") as fh:
+        fh.write(f"P5\n{px.shape[1]} {px.shape[0]}\n{maxval}\n".encode("ascii"))
+        fh.write(px.astype(">u2").tobytes() if px.dtype == np.uint16 else px.tobytes())
 
 
 class _Handle:

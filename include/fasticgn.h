@@ -21,9 +21,11 @@
  *   - Calls are synchronous: results are in caller memory when a call returns.
  *     The caller owns every buffer it passes.  One handle is never used from
  *     two host threads at once; distinct handles may be.
- *   - Images are row-major.  pixel: FI_PIXEL_U8 / U16 / F32.  depth12:
- *     FI_DEPTH_NATIVE, FI_DEPTH12_RIGHT (values must be <= 4095) or
- *     FI_DEPTH12_LEFT (divided by 16), src/oc_image.cpp:163-185.
+ *   - Images are row-major.  pixel: FI_PIXEL_U8 / U16 / F32.  The image
+ *     argument named depth12 is the input mode of loadImage2DMat
+ *     (src/oc_image.cpp:125-217): a depth FI_DEPTH_* or-ed with a normalise
+ *     FI_NORMALIZE_* (below); the pixels stay as uploaded in HBM and the
+ *     kernels convert them as they load them.
  */
 #ifndef FASTICGN_H
 #define FASTICGN_H
@@ -48,9 +50,39 @@ extern "C" {
 #define FI_PIXEL_U8 0
 #define FI_PIXEL_U16 1
 #define FI_PIXEL_F32 2
+/* Input mode (ImageLoadOptions, src/oc_image.h:26-53; loadImage2DMat,
+ * src/oc_image.cpp:125-217) = depth | normalise.
+ *   FI_DEPTH_NATIVE   the stored type's depth: 8-bit data 8-bit, 16-bit data 16-bit
+ *   FI_DEPTH12_RIGHT  12-bit right-aligned (values above 4095 are an error)
+ *   FI_DEPTH12_LEFT   12-bit left-aligned (divided by 16)
+ *   FI_DEPTH12_AUTO   12-bit, left-aligned when the maximum exceeds 4095
+ *   FI_DEPTH16        16-bit (ImageInputDepthMode::Depth16)
+ *   FI_DEPTH_AUTO     inferred: 8-bit data 8-bit; 16-bit data up to 4095
+ *                     12-bit right-aligned, else 16-bit (ImageInputDepthMode::Auto)
+ *   FI_NORMALIZE_TO8 / TO16 / UNIT scale to 0..255 / 0..65535 / 0..1 from the
+ *   logical maximum of the effective depth (255, 4095 or 65535).  The scale is
+ *   formed in FP64 and applied as float(pixel) * float(scale).  Depth modes
+ *   take 8- or 16-bit pixels; float pixels take FI_DEPTH_NATIVE only. */
 #define FI_DEPTH_NATIVE 0
 #define FI_DEPTH12_RIGHT 1
 #define FI_DEPTH12_LEFT 2
+#define FI_DEPTH12_AUTO 3
+#define FI_DEPTH_AUTO 4
+#define FI_DEPTH16 5
+#define FI_DEPTH_MASK 0x0f
+#define FI_NORMALIZE_NONE 0x00
+#define FI_NORMALIZE_TO8 0x10
+#define FI_NORMALIZE_TO16 0x20
+#define FI_NORMALIZE_UNIT 0x30
+#define FI_NORMALIZE_MASK 0xf0
+
+/* How an image is converted (ImageLoadInfo, src/oc_image.h:55-73). */
+typedef struct fi_ingest_info {
+    int32_t effective_bits;   /* 8, 12, 16, or 32 for float pixels */
+    int32_t left_aligned;     /* 12-bit data divided by 16 */
+    double source_max;        /* maximum pixel value (16-bit data, when the mode needs it), else 0 */
+    double scale;             /* the FP64 scale; pixels become float(p) * float(scale) */
+} fi_ingest_info;
 
 /* opencorr::POI2D (src/oc_poi.h:102-108): Point2D x,y; DeformationVector2D
  * p[12] = u ux uy uxx uxy uyy v vx vy vxx vxy vyy; Result2D r[6] = u0 v0 zncc
@@ -105,6 +137,11 @@ int fi_device_synchronize(int device);
 /* Copies the message of the calling thread's last failure (NUL-terminated,
  * truncated to len).  Returns the length of the full message. */
 int fi_last_error(char* buf, size_t len);
+/* Resolves an input mode for the given pixels (the effective depth and
+ * alignment, inferred from the maximum where the mode says so, and the scale)
+ * without uploading them; FI_E_INVALID_ARGUMENT as an upload would fail. */
+int fi_image_resolve(const void* px, int width, int height, size_t stride_bytes, int pixel, int mode,
+                     fi_ingest_info* out);
 /* The FFTCC DFT twiddle table for length n (2n floats: cos, -sin), as the
  * device kernels use it.  For tests of the DFT specification. */
 int fi_dft_twiddles(int n, float* out);
@@ -148,6 +185,9 @@ int fi_icgn_create(int rx, int ry, float conv, float stop, int device, fi_icgn**
 int fi_icgn_destroy(fi_icgn* h);
 int fi_icgn_set_iteration(fi_icgn* h, float conv, float stop);       /* :120-124 */
 int fi_icgn_set_self_adaptive(fi_icgn* h, int on);                   /* src/oc_dic.h:62 */
+/* DIC::setSubset (src/oc_dic.cpp:28-32): the subset radii of later computes
+ * (1..31 on the device, FI_E_UNSUPPORTED above). */
+int fi_icgn_set_subset(fi_icgn* h, int rx, int ry);
 /* Lane-to-POI shape of the IC-GN kernel: waves per POI (1, 2, 4) and rows per
  * band (the subset's columns map to lanes; 64 * waves / (2rx+1) bands of rows
  * stack vertically and must cover the 2ry+1 rows); 0, 0 restores the default.

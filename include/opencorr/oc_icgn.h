@@ -1,0 +1,5 @@
+// oc_icgn.h -- the reference's header name for the OpenCorr-shaped API of this
+// library (src/oc_icgn.h): callers written against OpenCorr include it unchanged
+// with -I include/opencorr.
+#pragma once
+#include "../fasticgn_opencorr.hpp"

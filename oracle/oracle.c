@@ -12,7 +12,7 @@
  *   src/oc_icgn.cpp:39-43,168-387        ICGN2D1::compute(POI2D*)   or_icgn_compute_*
  *   src/oc_subset.cpp:39-53              Subset2D fill/zeroMeanNorm
  *   src/oc_deformation.cpp:94-128        Deformation2D1 warp/update
- *   src/oc_image.cpp:163-185             12-bit alignment           or_image_to_float
+ *   src/oc_image.cpp:125-217             depth, alignment, normalise or_image_to_float
  *
  * Third-party arithmetic the reference delegates (absent here, restated):
  *   FFTW3f >= 3.3 (README.md:51): fftwf_plan_dft_r2c_2d / c2r_2d, unnormalised.
@@ -79,24 +79,55 @@ static void or_bc_matrix_fill(int f64, void* out)
  * Image ingest.  u8 as is; u16 as is (right aligned 12-bit must stay <= 4095)
  * or divided by 16 when left aligned (src/oc_image.cpp:163-185).
  * ------------------------------------------------------------------------- */
+/* loadImage2DMat (src/oc_image.cpp:125-217) on decoded pixels.  mode = depth
+ * (bits 0-3: 0 as stored, 1 12-bit right, 2 12-bit left, 3 12-bit with the
+ * alignment inferred (left when the maximum exceeds 4095), 4 depth inferred:
+ * 16-bit data up to 4095 is 12-bit right-aligned, else 16-bit, 5 16-bit) | normalise
+ * (bits 4-7: 0 none, 1 to 8 bits, 2 to 16 bits, 3 to [0, 1], each relative to
+ * the logical maximum 255 / 4095 / 65535 of the effective depth).  The scale
+ * is formed in FP64 as there and applied as float(pixel) * float(scale).
+ * Returns 0, -1 for right-aligned 12-bit data above 4095, -2 for a mode the
+ * pixel type does not take (float pixels are taken as stored). */
 int or_image_to_float(const void* px, int pixel, int width, int height,
-                      size_t stride_bytes, int depth12, float* out)
+                      size_t stride_bytes, int mode, float* out)
 {
+    const int depth = mode & 15, norm = (mode >> 4) & 15;
+    if (depth > 5 || norm > 3 || (mode & ~0xff)) return -2;
+    if (pixel == 2 && (depth != 0 || norm != 0)) return -2;
+    unsigned mx = 0;
+    for (int y = 0; y < height; y++) {
+        const uint8_t* row = (const uint8_t*)px + (size_t)y * stride_bytes;
+        for (int x = 0; x < width; x++) {
+            unsigned v = 0;
+            if (pixel == 0) v = row[x];
+            else if (pixel == 1) { uint16_t s; memcpy(&s, row + 2 * (size_t)x, 2); v = s; }
+            if (v > mx) mx = v;
+        }
+    }
+    /* effective depth (8, 12, 16, or 0 for float) and alignment */
+    int eff = pixel == 0 ? 8 : pixel == 2 ? 0 : 16, left = 0;
+    if (pixel != 2) {
+        if (depth == 1) eff = 12;
+        else if (depth == 2) { eff = 12; left = 1; }
+        else if (depth == 3) { eff = 12; left = mx > 4095; }
+        else if (depth == 5) eff = 16;
+        else if (depth == 4 && pixel == 1) eff = mx <= 4095 ? 12 : 16;
+        if (eff == 12 && !left && mx > 4095) return -1;
+    }
+    double scale = left ? 1.0 / 16.0 : 1.0;
+    const double logical = eff == 8 ? 255.0 : eff == 12 ? 4095.0 : 65535.0;
+    if (norm == 1) scale *= 255.0 / logical;
+    else if (norm == 2) scale *= 65535.0 / logical;
+    else if (norm == 3) scale *= 1.0 / logical;
+    const float s32 = (float)scale;
     for (int y = 0; y < height; y++) {
         const uint8_t* row = (const uint8_t*)px + (size_t)y * stride_bytes;
         for (int x = 0; x < width; x++) {
             float v;
             if (pixel == 0) v = (float)row[x];
-            else if (pixel == 1) {
-                uint16_t s;
-                memcpy(&s, row + 2 * (size_t)x, 2);
-                if (depth12 == 1 && s > 4095) return -1;
-                v = (float)s;
-                if (depth12 == 2) v = (float)((double)v * (1.0 / 16.0));
-            } else {
-                memcpy(&v, row + 4 * (size_t)x, 4);
-            }
-            out[(size_t)y * width + x] = v;
+            else if (pixel == 1) { uint16_t s; memcpy(&s, row + 2 * (size_t)x, 2); v = (float)s; }
+            else memcpy(&v, row + 4 * (size_t)x, 4);
+            out[(size_t)y * width + x] = v * s32;
         }
     }
     return 0;

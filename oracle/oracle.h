@@ -36,10 +36,13 @@ typedef struct or_poi {
 } or_poi;
 
 /* ---- image ingest (src/oc_image.cpp:125-217) ---------------------------- */
-/* pixel: 0 = u8, 1 = u16, 2 = f32.  depth12: 0 = none, 1 = right aligned,
- * 2 = left aligned (divide by 16).  Output row-major float, width*height. */
+/* pixel: 0 = u8, 1 = u16, 2 = f32.  mode: depth (0 as stored, 1 12-bit
+ * right, 2 12-bit left, 3 12-bit inferred alignment, 4 inferred depth) |
+ * normalise << 4 (0 none, 1 to 8 bits, 2 to 16 bits, 3 unit).  Output
+ * row-major float, width*height.  0, -1 (right-aligned 12-bit above 4095) or
+ * -2 (mode not valid for the pixel type). */
 int or_image_to_float(const void* px, int pixel, int width, int height,
-                      size_t stride_bytes, int depth12, float* out);
+                      size_t stride_bytes, int mode, float* out);
 
 /* ---- Gradient2D4 (src/oc_gradient.cpp:37-79) ----------------------------- */
 void or_gradient_f32(const float* img, int w, int h, float* gx, float* gy, int threads);

@@ -78,14 +78,23 @@ def threads() -> int:
     return int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
 
 
-def as_f32(img: np.ndarray, depth12: str = "native") -> np.ndarray:
+DEPTH_MODES = {"native": 0, "right": 1, "left": 2, "auto12": 3, "auto": 4, "16": 5}
+NORMALIZE_MODES = {"none": 0, "to8": 1, "to16": 2, "unit": 3}
+
+
+def as_f32(img: np.ndarray, depth12: str = "native", normalize: str = "none") -> np.ndarray:
+    """loadImage2DMat on decoded pixels: depth12 'native' (as stored),
+    'right' / 'left' (12-bit), 'auto12' (12-bit, alignment inferred) or 'auto'
+    (depth inferred); normalize 'none', 'to8', 'to16' or 'unit'."""
     a = np.ascontiguousarray(img)
     pixel = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.float32): 2}[a.dtype]
-    d = {"native": 0, "right": 1, "left": 2}[depth12]
+    mode = DEPTH_MODES[depth12] | (NORMALIZE_MODES[normalize] << 4)
     out = np.empty(a.shape, dtype=np.float32)
-    rc = lib().or_image_to_float(_p(a), pixel, a.shape[1], a.shape[0], a.strides[0], d, _p(out))
-    if rc != 0:
+    rc = lib().or_image_to_float(_p(a), pixel, a.shape[1], a.shape[0], a.strides[0], mode, _p(out))
+    if rc == -1:
         raise ValueError("12-bit right-aligned input has values above 4095")
+    if rc != 0:
+        raise ValueError(f"input mode {depth12}/{normalize} does not apply to {a.dtype} pixels")
     return out
 
 

@@ -167,3 +167,21 @@ def test_12bit_left_alignment_matches_right(oracle, u16_scene):
     for k in range(3):
         assert out["left"][k].tobytes() == out["right"][k].tobytes(), k
     assert (out["left"][0]["result"][:, 2] > 0.9).mean() > 0.9
+
+
+@pytest.mark.parametrize("depth,norm", [("native", "unit"), ("auto", "to16"), ("auto12", "none"), ("16", "to8"),
+                                        ("left", "unit")])
+def test_input_modes_convert_as_the_oracle(oracle, u16_scene, depth, norm):
+    """The device's conversion of each input mode (loadImage2DMat,
+    src/oc_image.cpp:125-217) equals the oracle's, bit for bit, for the
+    reference and the target, and B-spline tables built from them too."""
+    ref, tar = u16_scene
+    if depth in ("auto12", "left"):
+        ref, tar = (ref << 4).astype(np.uint16), (tar << 4).astype(np.uint16)
+    h, w = ref.shape
+    with engine(fa.Image2D(ref, depth, norm), fa.Image2D(tar, depth, norm)) as ic:
+        got_r, got_t = ic.export("ref", (h, w)), ic.export("tar", (h, w))
+        got_b = ic.export("bspline", (h, w))
+    want_r, want_t = oracle.as_f32(ref, depth, norm), oracle.as_f32(tar, depth, norm)
+    assert got_r.tobytes() == want_r.tobytes() and got_t.tobytes() == want_t.tobytes()
+    assert got_b.tobytes() == oracle.bspline(want_t).tobytes()
