@@ -171,6 +171,7 @@ SIGNATURES = {
     "fi_icgn_compute": [_P, _P, _S],
     "fi_icgn_compute_offset": [_P, _P, _S, _P],
     "fi_icgn_export": [_P, _I, _P, _S],
+    "fi_icgn_compute_noalloc": [_P, _P, _S],
     "fi_frame_create": [_I, _F, _F, _I, ctypes.POINTER(_P)],
     "fi_frame_destroy": [_P],
     "fi_frame_set_ref": [_P, _P, _I, _I, _S, _I, _I],

@@ -285,6 +285,11 @@ int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* cent
     });
 }
 
+int fi_icgn_compute_noalloc(fi_icgn* h, fi_poi* device_pois, size_t n)
+{
+    return guarded([&] { need(h); h->compute_noalloc(device_pois, n); });
+}
+
 int fi_icgn_export(fi_icgn* h, int what, void* out, size_t bytes)
 {
     return guarded([&] { need(h); h->export_data(what, out, bytes); });

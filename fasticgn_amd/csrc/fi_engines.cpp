@@ -462,6 +462,46 @@ void fi_icgn::compute(fi_poi* pois, size_t n, const float* offsets)
     }
 }
 
+// compute(vector<POI2D>&) on POIs the caller keeps in device memory, with
+// the per-POI reference records in the library's static scratch
+// (k_noalloc.hip) instead of the handle's buffers: no device or pinned
+// allocation, no host staging.  Uniform subsets only (self-adaptive subsets
+// need a host-side grouping), at most kNoAllocMaxPois POIs.
+void fi_icgn::compute_noalloc(fi_poi* d_pois, size_t n)
+{
+    if (!grads_ready || !coef_ready)
+        fail(FI_E_LOGIC, "ICGN2D1 compute requires prepareRef and prepareTar after the images are set");
+    if (ref_raw.view.width != tar_raw.view.width || ref_raw.view.height != tar_raw.view.height)
+        fail(FI_E_INVALID_ARGUMENT, "ICGN2D1 reference and target sizes differ");
+    if (self_adaptive) fail(FI_E_UNSUPPORTED, "the no-allocation compute takes uniform subsets only");
+    if (n > kNoAllocMaxPois)
+        fail(FI_E_INVALID_ARGUMENT, "the no-allocation compute takes at most " + std::to_string(kNoAllocMaxPois) +
+                                        " POIs per call, got " + std::to_string(n));
+    if (n == 0) return;
+    if (d_pois == nullptr) fail(FI_E_INVALID_ARGUMENT, "POI pointer is null");
+    DeviceGuard g(device);
+    IcgnParams p{};
+    p.ref = ref_f.p;
+    p.gx = gx.p;
+    p.gy = gy.p;
+    p.coef = coef.p;
+    p.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
+    p.group = sampling == FI_SAMPLING_GROUP ? 1 : 0;
+    p.w = ref_raw.view.width;
+    p.h = ref_raw.view.height;
+    p.pois = d_pois;
+    p.prep = noalloc_prep_scratch(device);
+    p.conv = conv;
+    p.stop = stop;
+    p.n = (int)n;
+    p.rx = rx;
+    p.ry = ry;
+    launch_icgn_prep(p, stream);
+    icgn_launch(p, shape_waves, shape_spl, stream);
+    FI_HIP(hipGetLastError());
+    FI_HIP(hipStreamSynchronize(stream));
+}
+
 void fi_icgn::export_data(int what, void* out, size_t bytes)
 {
     if (out == nullptr && bytes > 0) fail(FI_E_INVALID_ARGUMENT, "output pointer is null");

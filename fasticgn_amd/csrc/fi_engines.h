@@ -85,6 +85,9 @@ struct fi_icgn {
     // offsets: n (x, y) pairs (compute(POI2D*, Point2D&)), or nullptr
     void compute(fi_poi* pois, size_t n, const float* offsets = nullptr);
     size_t prep_n = 0;  // records of the last compute
+    // compute on POIs in device memory with the records in the library's
+    // static scratch: no allocation (fi_icgn_compute_noalloc)
+    void compute_noalloc(fi_poi* d_pois, size_t n);
     // Copies a device-side array out (fi_icgn_export).
     void export_data(int what, void* out, size_t bytes);
 };

@@ -123,6 +123,12 @@ void dft_twiddles(int n, float* out);
 // conv_threshold2(conv), which is (float)sqrt(s2) >= conv (k_icgn.hip).
 double conv_threshold2(float conv);
 
+// Scratch of fi_icgn_compute_noalloc (k_noalloc.hip): PoiPrep records for at
+// most kNoAllocMaxPois POIs per call (11.5 MB per GPU), in a file-scope device
+// array; its address on `device` (the current device), taken on first use.
+constexpr size_t kNoAllocMaxPois = 65536;
+PoiPrep* noalloc_prep_scratch(int device);
+
 // Frame-policy kernels (k_policy.hip).
 // prev2 (optional): the frame before prev, for the two-frame extrapolation
 // (steps up to max_delta px); frame->temporal counts the extrapolated seeds.

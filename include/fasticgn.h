@@ -231,6 +231,18 @@ int fi_icgn_compute(fi_icgn* h, fi_poi* pois, size_t n);
  * per-POI statuses (-5 for NaN, -3 out of range). */
 int fi_icgn_compute_offset(fi_icgn* h, fi_poi* pois, size_t n, const float* center_offsets);
 
+/* compute(vector<POI2D>&) on n POIs the caller keeps in device memory (updated
+ * in place), allocating nothing: no device or pinned memory, no host staging.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The per-POI reference records live in a static device array of
+ * the library that holds at most 65,536 POIs (11.5 MB per GPU, held until
+ * the process ends); a larger n is FI_E_INVALID_ARGUMENT.  Every call on a GPU
+ * shares that GPU's array, so calls that overlap on one GPU race.  Uniform
+ * subsets only (self-adaptive subsets are FI_E_UNSUPPORTED); prepareRef and
+ * prepareTar must have run.  Synchronous on the handle's stream. */
+int fi_icgn_compute_noalloc(fi_icgn* h, fi_poi* device_pois, size_t n);
+
 /* Copies one of the engine's device-side arrays to host memory, for tests of
  * the preparation stages against the oracle; bytes must be the array's exact
  * size.  No reference counterpart (the reference keeps them private):
