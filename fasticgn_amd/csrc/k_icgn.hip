@@ -327,13 +327,16 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 // for c > 0, (float)sqrt(s2) >= c exactly when sqrt(s2) reaches the midpoint
 // m of c and the float below it (an exact tie aside), and m, with at most 25
 // significant bits, squares exactly in FP64.  c = 0 passes every norm, c < 0
-// every norm but NaN, c = NaN none.
+// every norm but NaN, c = NaN none.  For c = +inf the float below is FLT_MAX
+// and the midpoint is 2^128 - 2^103, where (float)sqrt rounds up to inf (the
+// tie goes to the even significand, which is inf's).
 double conv_threshold2(float c)
 {
     if (c != c) return (double)c;
     if (c < 0.f) return -HUGE_VAL;
     if (c == 0.f) return 0.0;
-    const double m = 0.5 * ((double)nextafterf(c, 0.f) + (double)c);
+    const double below = (double)nextafterf(c, 0.f);
+    const double m = c == HUGE_VALF ? below + 0x1p103 : 0.5 * (below + (double)c);
     return m * m;
 }
 

@@ -563,6 +563,9 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     // this wave only
     hl += 36 * wave;
     if (lane < 36) hl[lane] = pr->hinv[lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 
     const Bands bd = make_bands(rx, ry, NT, tid);
     const int R = bd.R, ovl = bd.ovl;

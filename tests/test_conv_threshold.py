@@ -25,8 +25,11 @@ def reference_continues(s2: float, c: float) -> bool:
 
 
 def midpoint_below(c: float) -> float:
+    """The midpoint of c and the float below it; for +inf, of FLT_MAX and
+    2^128, the first root that rounds to inf."""
     c32 = np.float32(c)
-    return 0.5 * (float(np.nextafter(c32, np.float32(0))) + float(c32))
+    below = float(np.nextafter(c32, np.float32(0)))
+    return below + 2.0 ** 103 if np.isinf(c32) else 0.5 * (below + float(c32))
 
 
 def candidates(t: float, rng: np.random.Generator) -> list:
@@ -51,7 +54,7 @@ CRITERIA = [1e-3, 1e-2, 0.5, 1.0, 7.0, 3e-7, 1e-30, 1e-40, 3.4e38, np.inf, 0.0, 
 def test_threshold_is_the_rounded_root_test(c):
     rng = np.random.default_rng(7)
     t = threshold(c)
-    m = midpoint_below(c) if np.isfinite(c) and c > 0 else None
+    m = midpoint_below(c) if c > 0 else None
     checked = 0
     for s2 in candidates(t, rng):
         if m is not None and np.isfinite(s2) and np.sqrt(np.float64(s2)) == m:
@@ -65,7 +68,11 @@ def test_threshold_special_values():
     assert threshold(0.0) == 0.0
     assert threshold(-1.0) == -np.inf
     assert np.isnan(threshold(np.nan))
-    assert threshold(np.inf) == np.inf
+    # +inf: every root from 2^128 - 2^103 on rounds to inf, so s2 from its square
+    m = 2.0 ** 128 - 2.0 ** 103
+    assert threshold(np.inf) == m * m
+    assert reference_continues(m * m, np.inf)
+    assert not reference_continues(float(np.nextafter(m, 0.0)) ** 2, np.inf)
     # the criterion itself, squared, is above the threshold; the float below is not
     c = np.float32(1e-3)
     assert float(c) ** 2 >= threshold(float(c))
