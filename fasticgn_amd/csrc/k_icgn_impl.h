@@ -252,14 +252,16 @@ constexpr int kGroupRowSlack = 8;  // rows of vertical spread the four neighbour
 constexpr int kGroupMargin = FI_GROUP_MARGIN;
 
 // bias + sample at (gx, gy).  Entry (row, col) of a tile whose pixel (0, 0)
-// is image pixel (tx0, ty0)
-// holds the polynomial of pixel column tx0 + 1 + col; cb = -(ty0 + 1) *
-// kGroupStride - (tx0 + 1) places the sample's first row.  The caller
-// guarantees the neighbourhood lies in the tile.
+// is image pixel (tx0, ty0) holds the polynomial of pixel column tx0 + 1 +
+// col; cb = -(ty0 + 1) * kGroupStride - (tx0 + 1) places the sample's first
+// row.  The caller guarantees the neighbourhood lies in the tile.
 __device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx, float gy, float bias)
 {
+    static_assert(kGroupStride * sizeof(float4) == 1024, "a row of entries is 1 KiB: a shift by 10");
     const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
-    const float4* q = ct + (__mul24((int)gy, kGroupStride) + (int)gx + cb);
+    const uint32_t rowb = ((uint32_t)(int)gy << 10) + (uint32_t)(cb * 16);
+    const float4* q = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ct) +
+                                                      (((uint32_t)(int)gx << 4) + rowb));
     auto horner = [&](const float4 c) {
         return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, dx, c.z), dx, c.y), dx, c.x);
     };
@@ -751,7 +753,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         const float szz = __builtin_fmaf(-s1[0], tm, s1[1]);  // sum of squared centred target
         const float inv_tar_norm = __builtin_amdgcn_rsqf(szz);
         // e = 0 * inf when the target subset is flat, as in the reference
-        const float scale = (szz > 0.f) ? ref_norm * inv_tar_norm : __builtin_nanf("");
+        const float scale = __int_as_float(__builtin_amdgcn_readfirstlane(
+            __float_as_int((szz > 0.f) ? ref_norm * inv_tar_norm : __builtin_nanf(""))));
         zncc = __builtin_fmaf(-tm, pc->ref_sum, s1[2]) * inv_tar_norm * inv_ref_norm;
 
         // pass 2: e = (t - tbar) * (nr / nt) - r and the numerator sum SD_j e
@@ -782,33 +785,53 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         float dp[6];
 #pragma unroll
         for (int a = 0; a < 6; a++) dp[a] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dpl), a));
-        // W <- W * W(dp)^-1 (:333-340); the 3x3 inverse by cofactors down column 0
-        const float D00 = 1.f + dp[1], D01 = dp[2], D02 = dp[0];
-        const float D10 = dp[4], D11 = 1.f + dp[5], D12 = dp[3];
-        const float D20 = 0.f, D21 = 0.f, D22 = 1.f;
-        const float c00 = D11 * D22 - D12 * D21, c01 = D12 * D20 - D10 * D22, c02 = D10 * D21 - D11 * D20;
-        const float c10 = D21 * D02 - D22 * D01, c11 = D22 * D00 - D20 * D02, c12 = D20 * D01 - D21 * D00;
-        const float c20 = D01 * D12 - D02 * D11, c21 = D02 * D10 - D00 * D12, c22 = D00 * D11 - D01 * D10;
-        const float det = c00 * D00 + (c10 * D10 + c20 * D20);
-        const float invdet = __builtin_amdgcn_rcpf(det);
-        const float I00 = c00 * invdet, I01 = c10 * invdet, I02 = c20 * invdet;
-        const float I10 = c01 * invdet, I11 = c11 * invdet, I12 = c21 * invdet;
-        const float I20 = c02 * invdet, I21 = c12 * invdet, I22 = c22 * invdet;
-        const float N00 = W00 * I00 + (W01 * I10 + W02 * I20), N01 = W00 * I01 + (W01 * I11 + W02 * I21);
-        const float N02 = W00 * I02 + (W01 * I12 + W02 * I22), N10 = W10 * I00 + (W11 * I10 + W12 * I20);
-        const float N11 = W10 * I01 + (W11 * I11 + W12 * I21), N12 = W10 * I02 + (W11 * I12 + W12 * I22);
-        const float N20 = W20 * I00 + (W21 * I10 + W22 * I20), N21 = W20 * I01 + (W21 * I11 + W22 * I21);
-        const float N22 = W20 * I02 + (W21 * I12 + W22 * I22);
-        W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12; W20 = N20; W21 = N21; W22 = N22;
-
         // convergence norm (:343-352), accumulated in FP64; the loop compares
-        // its square with the launcher's threshold, the root is taken once
+        // its square with the launcher's threshold, the root is taken once.
+        // It is finite exactly when every dp is (a float squared cannot
+        // overflow FP64).
         cn = (double)dp[0] * dp[0];
         cn += (double)dp[1] * dp[1] * rx2;
         cn += (double)dp[2] * dp[2] * ry2;
         cn += (double)dp[3] * dp[3];
         cn += (double)dp[4] * dp[4] * rx2;
         cn += (double)dp[5] * dp[5] * ry2;
+
+        // W <- W * W(dp)^-1 (:333-340); the 3x3 inverse by cofactors down column 0
+        const float D00 = 1.f + dp[1], D01 = dp[2], D02 = dp[0];
+        const float D10 = dp[4], D11 = 1.f + dp[5], D12 = dp[3];
+        if (cn <= __DBL_MAX__) {
+            // Finite dp: the terms of the 3x3 product that hold a zero of the
+            // affine last rows (0 0 1) are exact zeros and drop out, leaving
+            // every other product and sum with the rounding of the full form.
+            const float c20 = D01 * D12 - D02 * D11, c21 = D02 * D10 - D00 * D12;
+            const float det = D11 * D00 + -(D01 * D10);
+            const float invdet = __builtin_amdgcn_rcpf(det);
+            const float I00 = D11 * invdet, I01 = -D01 * invdet, I02 = c20 * invdet;
+            const float I10 = -D10 * invdet, I11 = D00 * invdet, I12 = c21 * invdet;
+            const float I22 = (D00 * D11 - D01 * D10) * invdet;
+            const float N00 = W00 * I00 + W01 * I10, N01 = W00 * I01 + W01 * I11;
+            const float N02 = W00 * I02 + (W01 * I12 + W02 * I22);
+            const float N10 = W10 * I00 + W11 * I10, N11 = W10 * I01 + W11 * I11;
+            const float N12 = W10 * I02 + (W11 * I12 + W12 * I22);
+            W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12;
+            W22 = W22 * I22;
+        } else {
+            const float D20 = 0.f, D21 = 0.f, D22 = 1.f;
+            const float c00 = D11 * D22 - D12 * D21, c01 = D12 * D20 - D10 * D22, c02 = D10 * D21 - D11 * D20;
+            const float c10 = D21 * D02 - D22 * D01, c11 = D22 * D00 - D20 * D02, c12 = D20 * D01 - D21 * D00;
+            const float c20 = D01 * D12 - D02 * D11, c21 = D02 * D10 - D00 * D12, c22 = D00 * D11 - D01 * D10;
+            const float det = c00 * D00 + (c10 * D10 + c20 * D20);
+            const float invdet = __builtin_amdgcn_rcpf(det);
+            const float I00 = c00 * invdet, I01 = c10 * invdet, I02 = c20 * invdet;
+            const float I10 = c01 * invdet, I11 = c11 * invdet, I12 = c21 * invdet;
+            const float I20 = c02 * invdet, I21 = c12 * invdet, I22 = c22 * invdet;
+            const float N00 = W00 * I00 + (W01 * I10 + W02 * I20), N01 = W00 * I01 + (W01 * I11 + W02 * I21);
+            const float N02 = W00 * I02 + (W01 * I12 + W02 * I22), N10 = W10 * I00 + (W11 * I10 + W12 * I20);
+            const float N11 = W10 * I01 + (W11 * I11 + W12 * I21), N12 = W10 * I02 + (W11 * I12 + W12 * I22);
+            const float N20 = W20 * I00 + (W21 * I10 + W22 * I20), N21 = W20 * I01 + (W21 * I11 + W22 * I21);
+            const float N22 = W20 * I02 + (W21 * I12 + W22 * I22);
+            W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12; W20 = N20; W21 = N21; W22 = N22;
+        }
     } while ((float)it < stop && cn >= p.conv2);
     const float conv = (float)sqrt(cn);
 
