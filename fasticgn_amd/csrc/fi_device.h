@@ -12,11 +12,12 @@
 namespace fi {
 
 // DPP controls (GCN/CDNA encoding): quad_perm xor 1, quad_perm xor 2,
-// row_ror 4, row_ror 8.
+// row_ror 4, row_ror 8, row_half_mirror.
 constexpr int kDppQuadXor1 = 0xB1;
 constexpr int kDppQuadXor2 = 0x4E;
 constexpr int kDppRowRor4 = 0x124;
 constexpr int kDppRowRor8 = 0x128;
+constexpr int kDppRowHalfMirror = 0x141;
 
 template <int CTRL>
 __device__ __forceinline__ float dpp_f(float v)

@@ -338,9 +338,10 @@ __device__ __forceinline__ void stage_tile(float* tile, const float* tar, int w,
 }
 
 // Per-lane partial sums -> wave totals, identical on every lane.  Transposing
-// exchanges (permlane32_swap, permlane16_swap) fold pairs of values so each
-// register carries up to four values, one per 16-lane row; DPP then reduces
-// inside the rows and lane reads collect the row totals.
+// exchanges (permlane32_swap, permlane16_swap, and a rotate by 8 inside the
+// rows with selects) fold pairs of values so each register carries up to
+// eight values, one per 8-lane half row; DPP then reduces inside the groups
+// and lane reads collect the totals.
 __device__ __forceinline__ float2 swap32(float a, float b)
 {
     const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
@@ -401,7 +402,24 @@ __device__ __forceinline__ CPair row_sum16(CPair v)
     return v;
 }
 
-__device__ __forceinline__ float row_total(CPair z, int lane) { return lane_read(z.s, lane) + lane_read(z.c, lane); }
+// fold8(p, q) of two fold16 results: in every row, lanes 0-7 carry p and
+// lanes 8-15 q, each summed over two lanes 8 apart
+__device__ __forceinline__ CPair fold8(CPair p, CPair q)
+{
+    const bool lo = (__lane_id() & 8) == 0;
+    const CPair a = { lo ? p.s : q.s, lo ? p.c : q.c };
+    const CPair x = { lo ? q.s : p.s, lo ? q.c : p.c };
+    return cadd(a, { dpp_f<kDppRowRor8>(x.s), dpp_f<kDppRowRor8>(x.c) });
+}
+
+// sum of the 8 lanes of each half row (every lane of the half holds it)
+__device__ __forceinline__ CPair half_sum8(CPair v)
+{
+    v = cdpp<kDppQuadXor1>(v);
+    v = cdpp<kDppQuadXor2>(v);
+    v = cdpp<kDppRowHalfMirror>(v);
+    return v;
+}
 
 // Per-lane partial sums -> wave totals with a compensated (TwoSum) tree.
 __device__ __forceinline__ void wave_sum3(float (&v)[3])
@@ -409,9 +427,10 @@ __device__ __forceinline__ void wave_sum3(float (&v)[3])
     const CPair w0 = fold32(v[0], v[1]);
     const CPair w1 = fold32(v[2], v[2]);
     const CPair z = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 2
-    v[0] = row_total(z, 0);
-    v[2] = row_total(z, 16);
-    v[1] = row_total(z, 32);
+    const float t = z.s + z.c;
+    v[0] = lane_read(t, 0);
+    v[2] = lane_read(t, 16);
+    v[1] = lane_read(t, 32);
 }
 
 __device__ __forceinline__ void wave_sum6(float (&v)[6])
@@ -419,14 +438,15 @@ __device__ __forceinline__ void wave_sum6(float (&v)[6])
     const CPair w0 = fold32(v[0], v[1]);
     const CPair w1 = fold32(v[2], v[3]);
     const CPair w2 = fold32(v[4], v[5]);
-    const CPair z0 = row_sum16(fold16(w0, w1));  // rows: 0, 2, 1, 3
-    const CPair z1 = row_sum16(fold16(w2, w2));  // rows: 4, 4, 5, 5
-    v[0] = row_total(z0, 0);
-    v[2] = row_total(z0, 16);
-    v[1] = row_total(z0, 32);
-    v[3] = row_total(z0, 48);
-    v[4] = row_total(z1, 0);
-    v[5] = row_total(z1, 32);
+    // rows 0, 2, 1, 3 beside rows 4, 4, 5, 5: half rows 0|4, 2|4, 1|5, 3|5
+    const CPair z = half_sum8(fold8(fold16(w0, w1), fold16(w2, w2)));
+    const float t = z.s + z.c;
+    v[0] = lane_read(t, 0);
+    v[4] = lane_read(t, 8);
+    v[2] = lane_read(t, 16);
+    v[1] = lane_read(t, 32);
+    v[5] = lane_read(t, 40);
+    v[3] = lane_read(t, 48);
 }
 
 __device__ __forceinline__ void wave_sum(float (&v)[3]) { wave_sum3(v); }
