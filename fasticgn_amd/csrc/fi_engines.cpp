@@ -61,11 +61,17 @@ size_t max_lds_per_block(int device)
     return (size_t)v;
 }
 
-hipStream_t make_stream()
+// An engine's stream goes back to the block pool with the engine's buffers
+// once it is idle; one that fails to synchronise is destroyed instead.
+void retire_stream(int device, hipStream_t s, RecycleScope& recycle)
 {
-    hipStream_t s = nullptr;
-    FI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    return s;
+    if (!s) return;
+    if (hipStreamSynchronize(s) == hipSuccess) {
+        pool_stream_release(device, s);
+        recycle.arm();
+    } else {
+        (void)hipStreamDestroy(s);
+    }
 }
 
 }  // namespace
@@ -91,17 +97,14 @@ fi_fftcc::fi_fftcc(int rx, int ry, int dev) : device(dev)
     DeviceGuard g(device);
     if (fftcc_lds_bytes(r, kFftccUncached) > max_lds_per_block(device))
         fail(FI_E_UNSUPPORTED, "FFTCC window of radius " + std::to_string(r) + " exceeds the LDS per block");
-    stream = make_stream();
+    stream = pool_stream();
     std::vector<float> t(2 * 2 * r);
     dft_twiddles(2 * r, t.data());
     tw.reserve(t.size());
     FI_HIP(hipMemcpy(tw.p, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
 }
 
-fi_fftcc::~fi_fftcc()
-{
-    if (stream) (void)hipStreamDestroy(stream);
-}
+fi_fftcc::~fi_fftcc() { retire_stream(device, stream, recycle_); }
 
 void fi_fftcc::set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
@@ -293,13 +296,10 @@ fi_icgn::fi_icgn(int rx_, int ry_, float conv_, float stop_, int dev)
     if (rx > kIcgnMaxRadius || ry > kIcgnMaxRadius)
         fail(FI_E_UNSUPPORTED, "ICGN on the device supports radii 1.." + std::to_string(kIcgnMaxRadius));
     DeviceGuard g(device);
-    stream = make_stream();
+    stream = pool_stream();
 }
 
-fi_icgn::~fi_icgn()
-{
-    if (stream) (void)hipStreamDestroy(stream);
-}
+fi_icgn::~fi_icgn() { retire_stream(device, stream, recycle_); }
 
 void fi_icgn::set_ref(const void* px, int w, int h, size_t stride, int pixel, int mode)
 {
@@ -567,14 +567,15 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
     DeviceGuard g(device);
     if (fftcc_lds_bytes(r, kFftccCached) > max_lds_per_block(device))
         fail(FI_E_UNSUPPORTED, "FFTCC window exceeds the LDS per block");
-    stream = make_stream();
-    copy_stream = make_stream();
-    aux_stream = make_stream();
+    stream = pool_stream();
+    copy_stream = pool_stream();
+    aux_stream = pool_stream();
     FI_HIP(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    FI_HIP(hipEventCreateWithFlags(&tmpl_ev, hipEventDisableTiming));
     FI_HIP(hipEventCreate(&join_ev));
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
     for (auto& t : tslot) FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
-    rec_stream = make_stream();
+    rec_stream = pool_stream();
     for (auto& e : rec_ev) FI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     FI_HIP(hipEventCreateWithFlags(&pack_ev, hipEventDisableTiming));
     threshold.reserve(1);
@@ -597,15 +598,23 @@ fi_frame::~fi_frame()
     for (auto& e : rec_ev)
         if (e) (void)hipEventDestroy(e);
     if (pack_ev) (void)hipEventDestroy(pack_ev);
-    if (rec_stream) {
-        (void)hipStreamSynchronize(rec_stream);
-        (void)hipStreamDestroy(rec_stream);
-    }
     if (fork_ev) (void)hipEventDestroy(fork_ev);
     if (join_ev) (void)hipEventDestroy(join_ev);
-    if (aux_stream) (void)hipStreamDestroy(aux_stream);
-    if (copy_stream) (void)hipStreamDestroy(copy_stream);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (tmpl_ev) (void)hipEventDestroy(tmpl_ev);
+    // Every stream idle: nothing queued reads the buffers any more, so they
+    // and the streams go back to the block pool.  A stream that fails to
+    // synchronise is destroyed and the buffers are freed instead.
+    bool idle = true;
+    for (hipStream_t s : { stream, copy_stream, aux_stream, rec_stream })
+        if (s) idle = hipStreamSynchronize(s) == hipSuccess && idle;
+    for (hipStream_t s : { stream, copy_stream, aux_stream, rec_stream }) {
+        if (!s) continue;
+        if (idle)
+            pool_stream_release(device, s);
+        else
+            (void)hipStreamDestroy(s);
+    }
+    if (idle) recycle_.arm();
 }
 
 // Staging thread: packs a queued target into its slot's pinned buffer and
@@ -696,7 +705,11 @@ void fi_frame::set_pois(const fi_poi* pois, size_t count)
 {
     if (count > 0 && pois == nullptr) fail(FI_E_INVALID_ARGUMENT, "POI pointer is null");
     if (count > (size_t)INT_MAX) fail(FI_E_INVALID_ARGUMENT, "queue too long");
-    tmpl_host.assign(pois, pois + count);
+    {
+        DeviceGuard g(device);
+        tmpl_host.reserve(std::max<size_t>(count, 1));
+    }
+    if (count) std::memcpy(tmpl_host.p, pois, count * sizeof(fi_poi));
     n = count;
     pois_set = true;
     prepared = false;
@@ -767,7 +780,7 @@ void fi_frame::prepare()
     if (self_adaptive) {
         const size_t lds = max_lds_per_block(device);
         for (size_t i = 0; i < n; i++) {
-            const int rx = safe_int(tmpl_host[i].subset_radius[0]), ry = safe_int(tmpl_host[i].subset_radius[1]);
+            const int rx = safe_int(tmpl_host.p[i].subset_radius[0]), ry = safe_int(tmpl_host.p[i].subset_radius[1]);
             if (rx != ry)
                 fail(FI_E_UNSUPPORTED, "frame driver subsets are square (FFTCC2D): POI " + std::to_string(i) +
                                            " has subset_radius " + std::to_string(rx) + "x" + std::to_string(ry));
@@ -796,10 +809,10 @@ void fi_frame::prepare()
         classes.back()->n++;
     }
     std::vector<fi_poi> layout;
-    const fi_poi* lay = tmpl_host.data();
+    const fi_poi* lay = tmpl_host.p;
     if (!order.empty()) {
         layout.resize(n);
-        for (size_t i = 0; i < n; i++) layout[i] = tmpl_host[order[i]];
+        for (size_t i = 0; i < n; i++) layout[i] = tmpl_host.p[order[i]];
         lay = layout.data();
     }
 
@@ -838,9 +851,14 @@ void fi_frame::prepare()
             h_rec[k].reserve(std::max<size_t>(n * (size_t)36, 36));
         }
     const auto t_alloc = clock::now();
-    // host data: template layout, class sizes, layout order, DFT twiddles
-    // (the host vectors live until the closing synchronisation)
-    if (n > 0) FI_HIP(hipMemcpyAsync(tmpl.p, lay, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
+    // host data: template layout (on copy_stream, beside the reference
+    // conversion and gradients; pinned unless reordered), class sizes, layout
+    // order, DFT twiddles (the host vectors live until the closing
+    // synchronisation, which follows the template's copy)
+    if (n > 0) {
+        FI_HIP(hipMemcpyAsync(tmpl.p, lay, n * sizeof(fi_poi), hipMemcpyHostToDevice, copy_stream));
+        FI_HIP(hipEventRecord(tmpl_ev, copy_stream));
+    }
     std::vector<unsigned> sizes(nclass());
     for (size_t c = 0; c < nclass(); c++) sizes[c] = (unsigned)classes[c]->n;
     if (!sizes.empty())
@@ -861,6 +879,7 @@ void fi_frame::prepare()
     launch_to_float(ref_raw.view, ref_f.p, stream);
     launch_gradient(ref_f.p, w, h, gx.p, gy.p, stream);
     FI_HIP(hipEventRecord(ev[5], stream));
+    if (n > 0) FI_HIP(hipStreamWaitEvent(stream, tmpl_ev, 0));
     for (auto& cp : classes) {
         FrameClass& c = *cp;
         FftccParams fp{};

@@ -23,6 +23,7 @@ void check_shape(int waves, int spl);
 }
 
 struct fi_fftcc {
+    fi::RecycleScope recycle_;  // first member: see fi_frame
     int device = 0;
     int r = 0;
     hipStream_t stream = nullptr;
@@ -56,6 +57,7 @@ private:
 };
 
 struct fi_icgn {
+    fi::RecycleScope recycle_;  // first member: see fi_frame
     int device = 0;
     int rx = 0, ry = 0;
     float conv = 0.001f, stop = 10.f;
@@ -106,6 +108,9 @@ struct FrameClass {
 };
 
 struct fi_frame {
+    // first member, destroyed last: armed by the destructor once the streams
+    // are idle, so the buffers and streams go back to the block pool
+    fi::RecycleScope recycle_;
     int device = 0;
     int r = 0;                     // subset radius of every POI unless self_adaptive
     bool self_adaptive = false;    // per-POI subset_radius (square) instead of r
@@ -139,6 +144,7 @@ struct fi_frame {
     hipStream_t copy_stream = nullptr;
     hipStream_t aux_stream = nullptr;      // B-spline table beside the seeding
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+    hipEvent_t tmpl_ev = nullptr;  // template upload done (copy_stream)
     std::thread stager;
     std::mutex stage_mu;
     std::condition_variable stage_cv;
@@ -153,7 +159,7 @@ struct fi_frame {
     bool stage_stop = false;
     fi::DevArray<float> ref_f, gx, gy, coef, tar_f;
     fi::DevArray<fi::PoiPrep> prep;
-    std::vector<fi_poi> tmpl_host;        // template in the caller's order
+    fi::PinnedArray<fi_poi> tmpl_host;    // template in the caller's order (n POIs)
     std::vector<uint32_t> order;          // layout position -> template index (empty: identity)
     std::vector<std::unique_ptr<FrameClass>> classes;
     // cur: the frame being run; prev, prev2: the two frames before it

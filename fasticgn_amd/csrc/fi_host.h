@@ -31,11 +31,51 @@ struct Error : std::runtime_error {
 
 void set_last_error(const std::string& msg);
 
+// Block pool (fi_runtime.cpp).  Device and pinned host blocks, allocated in
+// size classes (eight per octave above 64 KiB), are kept after release by an
+// owner that has made sure no queued work still reads them, and handed to the
+// next request of the same class on the same device: a driver created for
+// one image pair then takes its buffers without hipMalloc / hipHostMalloc
+// and returns them without hipFree.  At most kPoolKeepDevice / kPoolKeepPinned
+// bytes are kept per device; an allocation that runs out of memory frees the
+// kept blocks of its device and tries again.  Blocks still kept when the
+// process ends are left to the process's teardown.
+enum PoolKind { kPoolDevice = 0, kPoolPinned = 1 };
+constexpr size_t kPoolKeepDevice = size_t(16) << 30;
+constexpr size_t kPoolKeepPinned = size_t(2) << 30;
+size_t pool_round(size_t bytes);
+// allocate a block of pool_round(bytes) on the current device (pinned: host
+// memory) from the pool or the runtime; *dev receives the current device
+void* pool_alloc(PoolKind kind, size_t bytes, int* dev);
+// release a block of pool_alloc: kept when a RecycleScope is active on this
+// thread and the pool has room, freed otherwise
+void pool_release(PoolKind kind, int dev, void* p, size_t bytes);
+// streams (non-blocking) kept the same way: pool_stream_release only for an
+// idle stream
+hipStream_t pool_stream();
+void pool_stream_release(int dev, hipStream_t s);
+int current_device();
+
+// While armed, released DevArray / PinnedArray blocks of this thread go back
+// to the pool.  An owner declares one as its first member (destroyed last)
+// and arms it in its destructor once its streams are idle, so that every
+// member buffer destroyed after the destructor's body is recycled.
+struct RecycleScope {
+    bool armed = false;
+    RecycleScope() = default;
+    RecycleScope(const RecycleScope&) = delete;
+    RecycleScope& operator=(const RecycleScope&) = delete;
+    void arm();
+    ~RecycleScope();
+};
+
 // Device buffer that only grows.
 template <typename T>
 struct DevArray {
     T* p = nullptr;
     size_t cap = 0;
+    size_t bytes = 0;  // block size
+    int dev = 0;
     DevArray() = default;
     DevArray(const DevArray&) = delete;
     DevArray& operator=(const DevArray&) = delete;
@@ -44,19 +84,16 @@ struct DevArray {
     {
         if (n <= cap) return;
         release();
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, n * sizeof(T));
-        if (e != hipSuccess)
-            fail(e == hipErrorOutOfMemory ? FI_E_NO_MEMORY : FI_E_DEVICE,
-                 "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
-        p = static_cast<T*>(q);
-        cap = n;
+        bytes = pool_round(n * sizeof(T));
+        p = static_cast<T*>(pool_alloc(kPoolDevice, bytes, &dev));
+        cap = bytes / sizeof(T);
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) pool_release(kPoolDevice, dev, p, bytes);
         p = nullptr;
         cap = 0;
+        bytes = 0;
     }
 };
 
@@ -65,6 +102,8 @@ template <typename T>
 struct PinnedArray {
     T* p = nullptr;
     size_t cap = 0;
+    size_t bytes = 0;
+    int dev = 0;
     PinnedArray() = default;
     PinnedArray(const PinnedArray&) = delete;
     PinnedArray& operator=(const PinnedArray&) = delete;
@@ -73,16 +112,16 @@ struct PinnedArray {
     {
         if (n <= cap) return;
         release();
-        void* q = nullptr;
-        FI_HIP(hipHostMalloc(&q, n * sizeof(T), hipHostMallocDefault));
-        p = static_cast<T*>(q);
-        cap = n;
+        bytes = pool_round(n * sizeof(T));
+        p = static_cast<T*>(pool_alloc(kPoolPinned, bytes, &dev));
+        cap = bytes / sizeof(T);
     }
     void release()
     {
-        if (p) (void)hipHostFree(p);
+        if (p) pool_release(kPoolPinned, dev, p, bytes);
         p = nullptr;
         cap = 0;
+        bytes = 0;
     }
 };
 

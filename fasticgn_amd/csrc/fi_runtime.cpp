@@ -5,7 +5,170 @@
 
 #include "fi_host.h"
 
+#include <map>
+#include <mutex>
+
 namespace fi {
+
+// ---------------------------------------------------------------------------
+// Block pool
+// ---------------------------------------------------------------------------
+namespace {
+
+struct PoolShelf {
+    std::map<size_t, std::vector<void*>> free;  // block size -> idle blocks
+    size_t kept = 0;
+};
+
+struct Pool {
+    std::mutex mu;
+    std::map<int, PoolShelf> shelf[2];  // [kind][device]
+    std::map<int, std::vector<hipStream_t>> streams;
+};
+
+Pool& pool()
+{
+    static Pool* p = new Pool;  // never destroyed: blocks outlive static teardown
+    return *p;
+}
+
+thread_local int recycle_depth = 0;
+
+void raw_free(PoolKind kind, void* p)
+{
+    if (kind == kPoolDevice)
+        (void)hipFree(p);
+    else
+        (void)hipHostFree(p);
+}
+
+hipError_t raw_alloc(PoolKind kind, void** q, size_t bytes)
+{
+    return kind == kPoolDevice ? hipMalloc(q, bytes) : hipHostMalloc(q, bytes, hipHostMallocDefault);
+}
+
+// free every kept block of (kind, dev)
+void pool_trim(PoolKind kind, int dev)
+{
+    std::vector<void*> drop;
+    {
+        Pool& P = pool();
+        std::lock_guard<std::mutex> lk(P.mu);
+        PoolShelf& sh = P.shelf[kind][dev];
+        for (auto& kv : sh.free)
+            for (void* b : kv.second) drop.push_back(b);
+        sh.free.clear();
+        sh.kept = 0;
+    }
+    for (void* b : drop) raw_free(kind, b);
+}
+
+}  // namespace
+
+int current_device()
+{
+    int d = 0;
+    FI_HIP(hipGetDevice(&d));
+    return d;
+}
+
+size_t pool_round(size_t bytes)
+{
+    if (bytes <= 256) return 256;
+    if (bytes <= (size_t(64) << 10)) {
+        size_t b = 256;
+        while (b < bytes) b <<= 1;
+        return b;
+    }
+    size_t top = size_t(1) << 16;
+    while ((top << 1) <= bytes) top <<= 1;  // top <= bytes < 2 top
+    const size_t step = top >> 3;
+    return (bytes + step - 1) / step * step;
+}
+
+void* pool_alloc(PoolKind kind, size_t bytes, int* dev)
+{
+    const int d = current_device();
+    *dev = d;
+    {
+        Pool& P = pool();
+        std::lock_guard<std::mutex> lk(P.mu);
+        PoolShelf& sh = P.shelf[kind][d];
+        auto it = sh.free.find(bytes);
+        if (it != sh.free.end() && !it->second.empty()) {
+            void* b = it->second.back();
+            it->second.pop_back();
+            sh.kept -= bytes;
+            return b;
+        }
+    }
+    void* q = nullptr;
+    hipError_t e = raw_alloc(kind, &q, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        pool_trim(kind, d);
+        e = raw_alloc(kind, &q, bytes);
+    }
+    if (e != hipSuccess)
+        fail(e == hipErrorOutOfMemory ? FI_E_NO_MEMORY : FI_E_DEVICE,
+             std::string(kind == kPoolDevice ? "hipMalloc" : "hipHostMalloc") + " of " + std::to_string(bytes) +
+                 " bytes failed: " + hipGetErrorString(e));
+    return q;
+}
+
+void pool_release(PoolKind kind, int dev, void* p, size_t bytes)
+{
+    if (recycle_depth > 0) {
+        Pool& P = pool();
+        std::lock_guard<std::mutex> lk(P.mu);
+        PoolShelf& sh = P.shelf[kind][dev];
+        if (sh.kept + bytes <= (kind == kPoolDevice ? kPoolKeepDevice : kPoolKeepPinned)) {
+            sh.free[bytes].push_back(p);
+            sh.kept += bytes;
+            return;
+        }
+    }
+    raw_free(kind, p);
+}
+
+hipStream_t pool_stream()
+{
+    const int d = current_device();
+    {
+        Pool& P = pool();
+        std::lock_guard<std::mutex> lk(P.mu);
+        auto& v = P.streams[d];
+        if (!v.empty()) {
+            hipStream_t s = v.back();
+            v.pop_back();
+            return s;
+        }
+    }
+    hipStream_t s = nullptr;
+    FI_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+void pool_stream_release(int dev, hipStream_t s)
+{
+    if (!s) return;
+    Pool& P = pool();
+    std::lock_guard<std::mutex> lk(P.mu);
+    P.streams[dev].push_back(s);
+}
+
+void RecycleScope::arm()
+{
+    if (!armed) {
+        armed = true;
+        recycle_depth++;
+    }
+}
+
+RecycleScope::~RecycleScope()
+{
+    if (armed) recycle_depth--;
+}
 
 static thread_local std::string g_last_error;
 
