@@ -1,10 +1,12 @@
 // ICGN2D1 on gfx950: the per-POI reference records and the IC-GN launch
 // dispatch over lane shapes.
 //
-//   k_icgn_prep  once per reference layout: reference subset mean and norm,
-//                steepest-descent Hessian (compensated FP32 sums), FP64
-//                partial-pivot inversion (one column per lane) and
-//                sum (ref - mean); stored as a 176-byte PoiPrep record.
+//   k_icgn_prep  once per reference layout, one wave per POI: reference
+//                subset mean and norm, the 21 steepest-descent Hessian sums
+//                (compensated FP32) and sum (ref - mean), into a 176-byte
+//                PoiPrep record;
+//   k_prep_inverse then one lane per POI: FP64 partial-pivot inversion of
+//                the Hessian, stored FP32 in the record.
 //   k_icgn       once per frame (k_icgn_impl.h): the inverse-compositional
 //                Gauss-Newton loop, one block per POI.
 //
@@ -19,8 +21,8 @@ namespace fi {
 // FP64 6x6 inverse, partial-pivot LU with the same structure as the CPU
 // reference's Eigen PartialPivLU (largest |a_ik|, first on ties; a zero pivot
 // column is skipped so a singular Hessian yields inf/NaN exactly as there).
-// Every lane factors (the same values: the pivot choice is uniform across the
-// wave); lane c < 6 then solves for column c of the inverse.
+// One lane per POI: every pivot row differs per lane, so row swaps are
+// selects over the candidate rows (registers are never indexed at run time).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void lu6_f64(double (&m)[6][6], int (&perm)[6])
 {
@@ -36,22 +38,19 @@ __device__ __forceinline__ void lu6_f64(double (&m)[6][6], int (&perm)[6])
             if (s > best) { best = s; piv = i; }
         }
         const bool nz = best != 0.0;
-        // every lane holds the same matrix, so the pivot row is uniform: one
-        // scalar branch per candidate row, and only the pivot row's swap runs
-        const int pu = __builtin_amdgcn_readfirstlane(nz ? piv : k);
+        if (!nz) piv = k;
 #pragma unroll
         for (int i = k + 1; i < 6; i++) {
-            if (i == pu) {
+            const bool sw = piv == i;
 #pragma unroll
-                for (int j = 0; j < 6; j++) {
-                    const double a = m[k][j];
-                    m[k][j] = m[i][j];
-                    m[i][j] = a;
-                }
-                const int pa = perm[k];
-                perm[k] = perm[i];
-                perm[i] = pa;
+            for (int j = 0; j < 6; j++) {
+                const double a = m[k][j], c = m[i][j];
+                m[k][j] = sw ? c : a;
+                m[i][j] = sw ? a : c;
             }
+            const int pa = perm[k], pc = perm[i];
+            perm[k] = sw ? pc : pa;
+            perm[i] = sw ? pa : pc;
         }
 #pragma unroll
         for (int i = k + 1; i < 6; i++) {
@@ -199,23 +198,15 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
     const float norm = sqrtf(u[0]);
     const float ref_sum = u[1];
 
-    double m[6][6];
+    // the 21 Hessian sums (packed lower triangle, entry (a, e <= a) at
+    // a (a + 1) / 2 + e) wait in the record's hinv for k_prep_inverse
+    if (lane == 0) {
+        float4* hv = reinterpret_cast<float4*>(out->hinv);
 #pragma unroll
-    for (int a = 0; a < 6; a++)
-#pragma unroll
-        for (int e = 0; e <= a; e++) {
-            const int q = a * (a + 1) / 2 + e;
-            const double v = (double)t[q / 6][q % 6];
-            m[a][e] = v;
-            m[e][a] = v;
-        }
-    int perm[6];
-    lu6_f64(m, perm);
-    if (lane < 6) {
-        double xc[6];
-        solve6_col_f64(m, perm, lane, xc);
-#pragma unroll
-        for (int r = 0; r < 6; r++) out->hinv[r * 6 + lane] = (float)xc[r];
+        for (int v = 0; v < 5; v++)
+            hv[v] = make_float4(t[(4 * v) / 6][(4 * v) % 6], t[(4 * v + 1) / 6][(4 * v + 1) % 6],
+                                t[(4 * v + 2) / 6][(4 * v + 2) % 6], t[(4 * v + 3) / 6][(4 * v + 3) % 6]);
+        out->hinv[20] = t[3][2];
     }
     if (lane == 0) {
         out->mean = mean;
@@ -225,6 +216,51 @@ __global__ __launch_bounds__(64) void k_icgn_prep(IcgnParams p)
         out->ry = ry;
         out->ref_sum = ref_sum;
     }
+}
+
+// The inverse Hessian of every prepared record, one lane per POI: the FP64
+// LU of the 21 sums k_icgn_prep left in hinv, then the six columns of the
+// inverse, stored FP32 over the sums.
+__global__ __launch_bounds__(64) void k_prep_inverse(IcgnParams p)
+{
+    const int b = (int)(blockIdx.x * 64 + threadIdx.x);
+    if (b >= p.n) return;
+    const int i = p.index ? (int)p.index[b] : b;
+    PoiPrep* rec = p.prep + i;
+    if (rec->status == 0) return;
+    const float4* hv = reinterpret_cast<const float4*>(rec->hinv);
+    float t[24];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        const float4 q = hv[v];
+        t[4 * v] = q.x;
+        t[4 * v + 1] = q.y;
+        t[4 * v + 2] = q.z;
+        t[4 * v + 3] = q.w;
+    }
+    t[20] = rec->hinv[20];
+    double m[6][6];
+#pragma unroll
+    for (int a = 0; a < 6; a++)
+#pragma unroll
+        for (int e = 0; e <= a; e++) {
+            const double v = (double)t[a * (a + 1) / 2 + e];
+            m[a][e] = v;
+            m[e][a] = v;
+        }
+    int perm[6];
+    lu6_f64(m, perm);
+    float inv[36];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        double xc[6];
+        solve6_col_f64(m, perm, c, xc);
+#pragma unroll
+        for (int r = 0; r < 6; r++) inv[r * 6 + c] = (float)xc[r];
+    }
+    float4* ho = reinterpret_cast<float4*>(rec->hinv);
+#pragma unroll
+    for (int v = 0; v < 9; v++) ho[v] = make_float4(inv[4 * v], inv[4 * v + 1], inv[4 * v + 2], inv[4 * v + 3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -316,6 +352,7 @@ void launch_icgn_prep(const IcgnParams& p, hipStream_t s)
 {
     if (p.n <= 0) return;
     hipLaunchKernelGGL(k_icgn_prep, dim3(p.n), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(k_prep_inverse, dim3((p.n + 63) / 64), dim3(64), 0, s, p);
 }
 
 // The per-POI LDS tile: the subset's neighbourhood (2r+1 + 3 pixels) plus
