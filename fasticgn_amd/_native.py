@@ -202,6 +202,7 @@ SIGNATURES = {
     "fi_seeder_set_training_data": [_P, _P, _P, _S],
     "fi_seeder_train_epoch": [_P, _I, _F, ctypes.c_uint64, ctypes.POINTER(_F)],
     "fi_seeder_forward": [_P, _P, _S, _P],
+    "fi_seeder_gradient": [_P, _P, _P, _S, _P, ctypes.POINTER(_F)],
     "fi_seeder_seed": [_P, _P, _P, _I, _I, _P, _S, _P],
     "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
     "fi_write_raw": [_P, _S, ctypes.c_char_p, ctypes.POINTER(_S)],

@@ -529,6 +529,12 @@ int fi_seeder_train_epoch(fi_seeder* h, int batch, float lr, uint64_t seed, floa
     return guarded([&] { need(h); h->train_epoch(batch, lr, seed, mean_loss); });
 }
 
+int fi_seeder_gradient(fi_seeder* h, const uint16_t* x1, const float* targets, size_t n, float* grads,
+                       float* loss)
+{
+    return guarded([&] { need(h); h->gradient(x1, targets, n, grads, loss); });
+}
+
 int fi_seeder_forward(fi_seeder* h, const uint16_t* x1, size_t n, float* out)
 {
     return guarded([&] { need(h); h->forward(x1, n, out); });

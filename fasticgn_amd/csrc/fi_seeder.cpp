@@ -202,26 +202,7 @@ void fi_seeder::train_epoch(int bsz, float lr, uint64_t seed, float* mean_loss)
         // gather the batch: 64 rows of 32 per sample
         launch_gather_rows16(data_x.p, bidx, bsz, 64 * kWidth1, x[0].p, stream);
         launch_gather_rows32(data_t.p, bidx, bsz, kSeedOut, tgt.p, stream);
-        forward_batch(bsz);
-        launch_head_fwd(zb[3].p, params.p + offset_w5(), params.p + offset_b5(), tgt.p, bsz, dout.p,
-                        d_loss.p + st * nloss, stream);
-        launch_head_bwd(zb[3].p, dout.p, params.p + offset_w5(), bsz, grads.p + offset_w5(), grads.p + offset_b5(),
-                        dh4.p, stream);
-        const float* dnext = dh4.p;
-        for (int l = kSeedLayers - 1; l >= 0; l--) {
-            const int rows = bsz * kSeedRows[l], K = kSeedK[l], N = kSeedN[l];
-            launch_relu_bwd(dnext, zb[l].p, rows, N, kGrid[l], dz[l].p, dz32[l].p, stream);
-            // dW = X^T dZ: A(i,k) = X[k K + i]; B(k,n) = dZ[k N + n]; K_gemm = rows
-            const int splits = rows >= 16 * kSplits && rows % (16 * kSplits) == 0 ? kSplits : 1;
-            launch_gemm_bf16(x[l].p, 1, K, dz[l].p, N, 1, grads.p + offset_w(l), K, N, rows, splits,
-                             gemm_parts.p, stream);
-            launch_colsum(dz32[l].p, rows, N, grads.p + offset_b(l), stream);
-            if (l > 0) {
-                // dX = dZ W^T: A(i,n) = dZ[i N + n]; B(n,k) = Wt[n K + k]
-                launch_gemm_bf16(dz[l].p, N, 1, wt[l].p, K, 1, dx[l].p, rows, K, N, 1, nullptr, stream);
-                dnext = dx[l].p;
-            }
-        }
+        loss_and_grad(bsz, d_loss.p + st * nloss);
         adam_t++;
         const float c1 = 1.f / (1.f - std::pow(b1, (float)adam_t));
         const float c2 = 1.f / (1.f - std::pow(b2, (float)adam_t));
@@ -235,6 +216,60 @@ void fi_seeder::train_epoch(int bsz, float lr, uint64_t seed, float* mean_loss)
         double s = 0.0;
         for (size_t i = 0; i < (size_t)nloss * steps; i++) s += h_loss.p[i];
         *mean_loss = steps ? (float)(s / ((double)steps * bsz * kSeedOut)) : 0.f;
+    }
+}
+
+// MSE loss partial sums (loss_parts) and the parameter gradient (grads) of
+// the batch in x[0] / tgt (bsz samples)
+void fi_seeder::loss_and_grad(int bsz, float* loss_parts_out)
+{
+    forward_batch(bsz);
+    launch_head_fwd(zb[3].p, params.p + offset_w5(), params.p + offset_b5(), tgt.p, bsz, dout.p, loss_parts_out,
+                    stream);
+    launch_head_bwd(zb[3].p, dout.p, params.p + offset_w5(), bsz, grads.p + offset_w5(), grads.p + offset_b5(),
+                    dh4.p, stream);
+    const float* dnext = dh4.p;
+    for (int l = kSeedLayers - 1; l >= 0; l--) {
+        const int rows = bsz * kSeedRows[l], K = kSeedK[l], N = kSeedN[l];
+        launch_relu_bwd(dnext, zb[l].p, rows, N, kGrid[l], dz[l].p, dz32[l].p, stream);
+        // dW = X^T dZ: A(i,k) = X[k K + i]; B(k,n) = dZ[k N + n]; K_gemm = rows
+        const int splits = rows >= 16 * kSplits && rows % (16 * kSplits) == 0 ? kSplits : 1;
+        launch_gemm_bf16(x[l].p, 1, K, dz[l].p, N, 1, grads.p + offset_w(l), K, N, rows, splits, gemm_parts.p,
+                         stream);
+        launch_colsum(dz32[l].p, rows, N, grads.p + offset_b(l), stream);
+        if (l > 0) {
+            // dX = dZ W^T: A(i,n) = dZ[i N + n]; B(n,k) = Wt[n K + k]
+            launch_gemm_bf16(dz[l].p, N, 1, wt[l].p, K, 1, dx[l].p, rows, K, N, 1, nullptr, stream);
+            dnext = dx[l].p;
+        }
+    }
+}
+
+void fi_seeder::gradient(const uint16_t* x1, const float* targets, size_t n, float* g, float* loss)
+{
+    if (x1 == nullptr || targets == nullptr || g == nullptr) fail(FI_E_INVALID_ARGUMENT, "null buffer");
+    if (n == 0 || n % 16 != 0 || n > (size_t)INT_MAX / 64)
+        fail(FI_E_INVALID_ARGUMENT, "batch must be a positive multiple of 16");
+    const int bsz = (int)n;
+    if (((size_t)bsz * 64) % kSplits != 0) fail(FI_E_INVALID_ARGUMENT, "batch incompatible with the K split");
+    DeviceGuard gd(device);
+    reserve_batch(bsz);
+    const int nloss = (bsz + 255) / 256;
+    DevArray<float> d_loss;
+    d_loss.reserve((size_t)nloss);
+    FI_HIP(hipMemcpyAsync(x[0].p, x1, n * 64 * kWidth1 * sizeof(uint16_t), hipMemcpyHostToDevice, stream));
+    FI_HIP(hipMemcpyAsync(tgt.p, targets, n * kSeedOut * sizeof(float), hipMemcpyHostToDevice, stream));
+    loss_and_grad(bsz, d_loss.p);
+    FI_HIP(hipGetLastError());
+    const size_t np = seed_param_count();
+    std::vector<float> h_loss((size_t)nloss);
+    FI_HIP(hipMemcpyAsync(g, grads.p, np * sizeof(float), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipMemcpyAsync(h_loss.data(), d_loss.p, (size_t)nloss * sizeof(float), hipMemcpyDeviceToHost, stream));
+    FI_HIP(hipStreamSynchronize(stream));
+    if (loss) {
+        double t = 0.0;
+        for (float v : h_loss) t += v;
+        *loss = (float)(t / ((double)bsz * kSeedOut));
     }
 }
 

@@ -31,6 +31,8 @@ struct fi_seeder {
     void set_training_data(const uint16_t* x1, const float* targets, size_t n);
     void train_epoch(int bsz, float lr, uint64_t seed, float* mean_loss);
     void forward(const uint16_t* x1, size_t n, float* out);
+    void gradient(const uint16_t* x1, const float* targets, size_t n, float* g, float* loss);
+    void loss_and_grad(int bsz, float* loss_parts_out);
     void seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok);
     // device-side seeding for the frame driver (stream-ordered, no sync)
     void run_seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, const uint32_t* index, int n,

@@ -163,6 +163,17 @@ class LearnedSeeder:
                  ctypes.byref(loss))
         return loss.value
 
+    def gradient(self, x1: np.ndarray, targets: np.ndarray) -> tuple[np.ndarray, float]:
+        """(parameter gradient, loss) of the training step on one batch (a
+        multiple of 16 samples); the parameters are unchanged."""
+        x1 = np.ascontiguousarray(x1, dtype=np.uint16)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        assert x1.shape[1:] == (64, 32) and t.shape == (len(x1), N_OUT)
+        g = np.empty(param_count(), dtype=np.float32)
+        loss = ctypes.c_float(0.0)
+        nat.call("fi_seeder_gradient", self._h, nat.ptr(x1), nat.ptr(t), len(x1), nat.ptr(g), ctypes.byref(loss))
+        return g, loss.value
+
     def forward(self, x1: np.ndarray) -> np.ndarray:
         x1 = np.ascontiguousarray(x1, dtype=np.uint16)
         out = np.empty((len(x1), N_OUT), dtype=np.float32)

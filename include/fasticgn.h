@@ -406,6 +406,12 @@ int fi_seeder_set_training_data(fi_seeder* h, const uint16_t* x1, const float* t
 /* One epoch of Adam over a seeded shuffle in batches (multiple of 16; the
  * remainder of the shuffle is dropped).  mean_loss may be NULL. */
 int fi_seeder_train_epoch(fi_seeder* h, int batch, float lr, uint64_t seed, float* mean_loss);
+/* The training step's loss (mean squared error over n*6 scaled outputs) and
+ * its gradient with respect to the parameter vector (n = batch, a multiple of
+ * 16), computed by the kernels fi_seeder_train_epoch runs before its Adam
+ * update; the parameters are not changed.  loss may be NULL. */
+int fi_seeder_gradient(fi_seeder* h, const uint16_t* x1, const float* targets, size_t n, float* grads,
+                       float* loss);
 /* Network outputs (n*6, scaled) for layer-1 rows, through the training
  * kernels. */
 int fi_seeder_forward(fi_seeder* h, const uint16_t* x1, size_t n, float* out);

@@ -1,6 +1,7 @@
 """Learned seeder: the CPU forward restatement pinned by construction, the
-crop layout, and the device kernels (training forward, fused inference,
-training) against the restatement."""
+backward restatement pinned by finite differences, the crop layout, and the
+device kernels (training forward, fused inference, training gradient, Adam
+steps) against the restatements."""
 import ctypes
 
 import numpy as np
@@ -170,3 +171,108 @@ def test_training_reduces_loss():
             last = s.train_epoch(256, 2e-3, 2 + e)
     print("loss", first, "->", last, flush=True)
     assert last < 0.5 * first
+
+
+# ------------------------------------------------- training gradient ------
+
+def small_batch(n=16, seed=5):
+    rng = np.random.default_rng(seed)
+    bits = sd.bf16_bits(rng.normal(0, 1, (n, 64, 32)).astype(np.float32))
+    t = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    return bits, t
+
+
+def test_backward_restatement_matches_finite_differences():
+    """The float64 backward pass of oracle/seeder_ref.py against central
+    differences of its forward loss, on parameters drawn from every block
+    (weights and biases of the four layers and the head) and along random
+    directions; a transposed patch merge or a missing ReLU' fails it."""
+    bits, t = small_batch()
+    p = random_params(4).astype(np.float64)
+    loss, g = seeder_ref.loss_and_grad(p, bits, t, bf16=False)
+    rng = np.random.default_rng(9)
+    blocks, o = [], 0
+    for k, n in sd.LAYERS + [(128, 6)]:
+        blocks += [(o, k * n), (o + k * n, n)]
+        o += k * n + n
+    h = 1e-5
+    worst = 0.0
+    for start, size in blocks:
+        for i in start + rng.choice(size, min(size, 12), replace=False):
+            e = np.zeros_like(p)
+            e[i] = h
+            fd = (seeder_ref.loss_and_grad(p + e, bits, t, bf16=False)[0]
+                  - seeder_ref.loss_and_grad(p - e, bits, t, bf16=False)[0]) / (2 * h)
+            worst = max(worst, abs(fd - g[i]) / (abs(g[i]) + 1e-6 * abs(g).max()))
+    for _ in range(3):
+        v = rng.normal(0, 1, p.size)
+        fd = (seeder_ref.loss_and_grad(p + h * v, bits, t, bf16=False)[0]
+              - seeder_ref.loss_and_grad(p - h * v, bits, t, bf16=False)[0]) / (2 * h)
+        worst = max(worst, abs(fd - g @ v) / abs(g @ v))
+    assert worst < 1e-5, worst
+    # the BF16 restatement is the same pass with rounded GEMM inputs: within
+    # a few BF16 roundings (2^-8 each) of the exact gradient
+    _, gb = seeder_ref.loss_and_grad(p.astype(np.float32), bits, t, bf16=True)
+    assert np.linalg.norm(gb - g) < 5e-2 * np.linalg.norm(g)
+
+
+def test_backward_restatement_merge_is_inverted():
+    y = np.arange(2 * 16 * 3, dtype=np.float64).reshape(32, 3)
+    m = seeder_ref.merge_patches(y, 2, 4)
+    np.testing.assert_array_equal(seeder_ref.unmerge_patches(m, 2, 4), y)
+
+
+@pytest.mark.gpu
+def test_training_gradient_matches_restatement():
+    """The device's training-step gradient (fi_seeder_gradient: forward,
+    MSE loss, head and ReLU' backward, split-K weight GEMMs, bias column
+    sums) against the BF16 restatement, block by block; the loss itself
+    to FP32 rounding."""
+    bits, t = small_batch(64, 6)
+    p = random_params(7)
+    with sd.LearnedSeeder(p) as s:
+        g, loss = s.gradient(bits, t)
+        np.testing.assert_array_equal(s.get_params(), p)  # no update
+    ref_loss, gr = seeder_ref.loss_and_grad(p, bits, t, bf16=True)
+    print("loss", loss, ref_loss, flush=True)
+    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    o = 0
+    for li, (k, n) in enumerate(sd.LAYERS + [(128, 6)]):
+        for name, size in (("W", k * n), ("b", n)):
+            a, b = g[o:o + size], gr[o:o + size]
+            rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+            print(f"layer {li + 1} {name}: relative L2 {rel:.2e}", flush=True)
+            # measured 2e-8 .. 9e-7: FP32 summation order only
+            assert rel < 1e-4, (li, name, rel)
+            o += size
+
+
+@pytest.mark.gpu
+def test_adam_steps_match_restatement():
+    """Two training epochs of one batch each (fi_seeder_train_epoch): the
+    parameters move by Adam with bias correction, b1 0.9, b2 0.999, eps 1e-8,
+    from the gradient fi_seeder_gradient gives at the same parameters."""
+    bits, t = small_batch(64, 8)
+    p0 = random_params(10)
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    with sd.LearnedSeeder(p0) as s:
+        s.set_training_data(bits, t)
+        g0, _ = s.gradient(bits, t)
+        s.train_epoch(64, lr, 1)
+        p1 = s.get_params()
+        g1, _ = s.gradient(bits, t)
+        s.train_epoch(64, lr, 2)
+        p2 = s.get_params()
+    m = (1 - b1) * g0.astype(np.float64)
+    v = (1 - b2) * g0.astype(np.float64) ** 2
+    want1 = p0 - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    m = b1 * m + (1 - b1) * g1
+    v = b2 * v + (1 - b2) * g1.astype(np.float64) ** 2
+    want2 = p1 - lr * (m / (1 - b1 ** 2)) / (np.sqrt(v / (1 - b2 ** 2)) + eps)
+    for got, want in ((p1, want1), (p2, want2)):
+        err = np.abs(got - want)
+        print("adam step max error / lr", float(err.max() / lr), flush=True)
+        # measured 1.5e-5 and 3.2e-5: the training step sums its batch in
+        # shuffled order, so a few gradient bits differ from
+        # fi_seeder_gradient's
+        assert err.max() <= 1e-3 * lr
