@@ -181,9 +181,12 @@ int or_image_to_float(const void* px, int pixel, int width, int height,
  * Zr[k] = (A[j].x + B[j].y, B[j].x - A[j].y); z = ifft(Zr); row 2j = z.x,
  * row 2j+1 = z.y.
  *
- * Window mean and squared norm use row partial sums: for each row the values
- * are summed left to right from 0.0f, and the row sums are added top to
- * bottom from 0.0f (the reference sums serially, src/oc_fftcc.cpp:415-437).
+ * Window mean and squared norm use row partial sums by default: for each row
+ * the values are summed left to right from 0.0f, and the row sums are added
+ * top to bottom from 0.0f (the device's order).  or_fftcc_set_serial_sums(1)
+ * switches to the reference's order, one running sum over the window in row
+ * order (src/oc_fftcc.cpp:280-299, 415-437), so the two orders can be
+ * compared.
  * ------------------------------------------------------------------------- */
 void or_dft_twiddles(int n, float* tw)
 {
@@ -382,13 +385,21 @@ static void cross_power(const float* R, const float* T, int len, float* Z)
     }
 }
 
+static int fftcc_serial_sums = 0;
+
+void or_fftcc_set_serial_sums(int on) { fftcc_serial_sums = on != 0; }
+
 static float window_mean(const float* v, int n)
 {
     float tot = 0.f;
-    for (int y = 0; y < n; y++) {
-        float rs = 0.f;
-        for (int x = 0; x < n; x++) rs += v[y * n + x];
-        tot += rs;
+    if (fftcc_serial_sums) {
+        for (int i = 0; i < n * n; i++) tot += v[i];
+    } else {
+        for (int y = 0; y < n; y++) {
+            float rs = 0.f;
+            for (int x = 0; x < n; x++) rs += v[y * n + x];
+            tot += rs;
+        }
     }
     return tot / (float)(n * n);
 }
@@ -397,6 +408,13 @@ static float window_mean(const float* v, int n)
 static float window_zero_mean_sq(float* v, int n, float mean)
 {
     float tot = 0.f;
+    if (fftcc_serial_sums) {
+        for (int i = 0; i < n * n; i++) {
+            v[i] -= mean;
+            tot += v[i] * v[i];
+        }
+        return tot;
+    }
     for (int y = 0; y < n; y++) {
         float rs = 0.f;
         for (int x = 0; x < n; x++) {

@@ -61,6 +61,9 @@ void or_dft_twiddles(int n, float* tw);
 /* Circular cross-correlation of two zero-mean n x n windows through the
  * DFT specification (R2C, conj(R)*T, C2R), unnormalised like FFTW. */
 void or_fftcc_correlate(const float* ref_win, const float* tar_win, int n, float* out);
+/* window mean / norm summation order: 0 row partials (default, the device's),
+ * 1 one serial sum in row order (the reference's) */
+void or_fftcc_set_serial_sums(int on);
 /* Uncached compute over a queue (FFTCC2D::compute(POI2D*), :396-496). */
 void or_fftcc_compute(const float* ref, const float* tar, int w, int h, int r,
                       or_poi* pois, size_t n, int threads);

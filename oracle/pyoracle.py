@@ -52,6 +52,7 @@ def lib() -> ctypes.CDLL:
             "or_bspline_eval_f64": ([_P, _I, _I, _D, _D], _D),
             "or_dft_twiddles": ([_I, _P], None),
             "or_fftcc_correlate": ([_P, _P, _I, _P], None),
+            "or_fftcc_set_serial_sums": ([_I], None),
             "or_fftcc_compute": ([_P, _P, _I, _I, _I, _P, _S, _I], None),
             "or_fftcc_prepare_ref": ([_P, _I, _I, _I, _P, _S, _P, _P, _P, _P, _P, _I], None),
             "or_fftcc_compute_prepared": ([_P, _I, _I, _I, _P, _P, _P, _P, _S, _P, _I], None),
@@ -140,13 +141,19 @@ def correlate(ref_win: np.ndarray, tar_win: np.ndarray) -> np.ndarray:
     return out
 
 
-def fftcc(ref: np.ndarray, tar: np.ndarray, radius: int, queue: np.ndarray) -> None:
-    """FFTCC2D::compute(POI2D*) over the queue (uncached), in place."""
+def fftcc(ref: np.ndarray, tar: np.ndarray, radius: int, queue: np.ndarray, serial_sums: bool = False) -> None:
+    """FFTCC2D::compute(POI2D*) over the queue (uncached), in place.
+    serial_sums: window mean and norm in the reference's serial order instead
+    of the device's row partials."""
     ref = np.ascontiguousarray(ref, dtype=np.float32)
     tar = np.ascontiguousarray(tar, dtype=np.float32)
     assert queue.dtype == POI_DTYPE and queue.flags.c_contiguous
     h, w = ref.shape
-    lib().or_fftcc_compute(_p(ref), _p(tar), w, h, radius, _p(queue), len(queue), threads())
+    lib().or_fftcc_set_serial_sums(int(serial_sums))
+    try:
+        lib().or_fftcc_compute(_p(ref), _p(tar), w, h, radius, _p(queue), len(queue), threads())
+    finally:
+        lib().or_fftcc_set_serial_sums(0)
 
 
 class PreparedRef:

@@ -392,3 +392,25 @@ def test_icgn_center_offset_closed_form(oracle):
     oracle.icgn(ref, tar, 15, p)
     okp = ok & (p["result"][:, 2] >= 0)
     assert np.abs(p["deformation"][okp, 0] - u[okp]).max() < 0.05
+
+
+def test_fftcc_window_sum_order_reference_vs_device(oracle):
+    """FFTCC's window mean and norm in the reference's order (one serial sum
+    in row order, src/oc_fftcc.cpp:280-299, 415-437) and in the device's
+    (row partials, which the kernel reproduces bit for bit): the integer
+    peaks are identical on the 512^2 u8 scene and a 12-bit scene, and the
+    ZNCC differs by FP32 summation noise only (measured 8.7e-6 and 1.8e-6
+    relative)."""
+    from fasticgn_amd import synth
+    c0 = synth.config0()
+    c2 = synth.config2(size=256, frames=500)
+    for ref, tar, depth in ((c0.ref, c0.tar, "native"), (c2.ref, c2.frame(120), "right")):
+        pois = c0.pois if depth == "native" else c2.pois
+        ref32, tar32 = oracle.as_f32(ref, depth), oracle.as_f32(tar, depth)
+        dev, ser = pois.copy(), pois.copy()
+        oracle.fftcc(ref32, tar32, 15, dev)
+        oracle.fftcc(ref32, tar32, 15, ser, serial_sums=True)
+        assert np.array_equal(dev["deformation"], ser["deformation"])
+        zd, zs = dev["result"][:, 2], ser["result"][:, 2]
+        assert (zd != zs).any()  # the orders do differ in the last bits
+        assert np.max(np.abs(zd - zs) / np.abs(zs)) < 2e-5
