@@ -13,7 +13,8 @@ read_raw_frame) loads it unchanged.
 
 The manifest is rewritten atomically (temporary file + rename) after every
 frame, so an interrupted run leaves a manifest that lists exactly the frames
-whose files are complete.
+whose files are complete.  With an output session (fasticgn_amd/ledger.py)
+the frames and the manifest are published through its intent ledger.
 """
 from __future__ import annotations
 
@@ -72,8 +73,9 @@ class RawWriter:
     def __init__(self, output_dir, ref_path: str = "", tar_list_path: str = "", subset_radius: int = 0,
                  grid_space: int = 0, conv_threshold: float = 0.001, max_iteration: int = 10,
                  policy: int | None = None, depth: str = "native", learned_seed: bool = False,
-                 temporal_extrapolation: bool = False):
+                 temporal_extrapolation: bool = False, session=None):
         from .driver import DEFAULT_POLICY, POI_FALLBACK, SKIP_FFTCC
+        self.session = session
         self.dir = Path(output_dir)
         self.dir.mkdir(parents=True, exist_ok=True)
         policy = DEFAULT_POLICY if policy is None else policy
@@ -114,7 +116,15 @@ class RawWriter:
         k = len(self.frames)
         name = output_name or f"frame_{k:06d}.bin"
         t0 = time.perf_counter()
-        if results.dtype == RAW_DTYPE:
+        published = None
+        if self.session is not None:
+            from .ledger import OutputSpec
+            rec = results if results.dtype == RAW_DTYPE else format_raw(results)
+            payload = np.ascontiguousarray(rec).tobytes()
+            spec = OutputSpec(key="frame/" + name, path=self.dir / name, rows=len(rec), record_bytes=RECORD_BYTES)
+            published = self.session.publish(spec, payload)
+            nbytes = len(payload)
+        elif results.dtype == RAW_DTYPE:
             # records already packed (FrameDriver.records())
             path = self.dir / name
             tmp = path.with_suffix(path.suffix + ".tmp")
@@ -144,12 +154,16 @@ class RawWriter:
         }
         self.frames.append(entry)
         self.flush()
-        return entry
+        return {**entry, "ledger": published} if published else entry
 
     def flush(self) -> Path:
         path = self.dir / RAW_MANIFEST_NAME
+        text = json.dumps({**self.head, "frames": self.frames}, indent=2) + "\n"
+        if self.session is not None:
+            self.session.publish_metadata(path, text.encode())
+            return path
         tmp = path.with_suffix(".json.tmp")
-        tmp.write_text(json.dumps({**self.head, "frames": self.frames}, indent=2) + "\n")
+        tmp.write_text(text)
         os.replace(tmp, path)
         return path
 
