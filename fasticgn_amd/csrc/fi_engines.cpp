@@ -1170,6 +1170,10 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         }
     }
 
+    // records: the slot is chosen (and its previous copy waited for) before
+    // the segment, packed at the segment's end and copied after it
+    const int rs = records_on ? records_slot() : -1;
+
     auto segment = [&](bool timed) {
         // a fill kernel, not memsets: this runs inside captured frame graphs
         // FFTCC and learned seeds start from the template: copied here, in the
@@ -1251,15 +1255,17 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         launch_qa(cur.p, cnt, counters.p + C, stream, iters.p, 3 * n);
         FI_HIP(hipMemcpyAsync(h_counters.p, counters.p, (C + 1) * sizeof(FrameCounters), hipMemcpyDeviceToHost,
                               stream));
+        if (rs >= 0) pack_records(cur.p, rs);
         FI_HIP(hipGetLastError());
     };
 
     if (use_graph) {
-        const uint64_t key[10] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
+        const uint64_t key[11] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
                                    (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tview.data,
                                    (uint64_t)(uintptr_t)(table ? coef.p : nullptr), (uint64_t)(uintptr_t)tar_f.p,
                                    (uint64_t)(uintptr_t)seeder, (uint64_t)(uintptr_t)hist2,
-                                   (uint64_t)(uint32_t)__builtin_bit_cast(uint32_t, max_delta) };
+                                   (uint64_t)(uint32_t)__builtin_bit_cast(uint32_t, max_delta),
+                                   (uint64_t)(uintptr_t)(rs >= 0 ? d_rec[rs].p : nullptr) };
         hipGraphExec_t exec = nullptr;
         for (auto& gr : graphs)
             if (std::memcmp(gr.key, key, sizeof(key)) == 0) exec = gr.exec;
@@ -1288,6 +1294,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     } else {
         segment(true);
     }
+    FI_HIP(hipEventRecord(ev[4], stream));
+    if (rs >= 0) copy_records(rs);
     FI_HIP(hipStreamSynchronize(stream));
     std::vector<FrameCounters> pc(h_counters.p, h_counters.p + C + 1);
     FrameCounters qa = pc[C];
@@ -1349,11 +1357,14 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         launch_qa(cur.p, cnt, counters.p + C, stream, iters.p, 3 * n);
         FI_HIP(hipGetLastError());
         qa = read_counters(nullptr);
+        if (rs >= 0) {
+            // repairs past the cap changed the frame: its records again, same slot
+            FI_HIP(hipStreamWaitEvent(stream, rec_ev[rs], 0));
+            pack_records(cur.p, rs);
+            copy_records(rs);
+        }
     }
-    if (records_on) queue_records(cur.p);
-    FI_HIP(hipEventRecord(ev[4], stream));
     last_slot = ts;
-    FI_HIP(hipEventSynchronize(ev[4]));
 
     const float bad_ratio = cnt ? (float)qa.bad / (float)cnt : 0.f;
     if (!skip) st.pre_repair_bad_ratio = bad_ratio;
@@ -1408,13 +1419,35 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 // copy finished (rec_ev).
 void fi_frame::queue_records(const fi_poi* src)
 {
+    const int s = records_slot();
+    pack_records(src, s);
+    copy_records(s);
+}
+
+// The next record slot, its buffers sized, the compute stream waiting for the
+// slot's previous copy (outside any capture: the wait is on rec_stream work).
+int fi_frame::records_slot()
+{
     const int s = rec_next;
     rec_next ^= 1;
     const size_t bytes = n * (size_t)36;
     d_rec[s].reserve(std::max<size_t>(bytes, 36));
     h_rec[s].reserve(std::max<size_t>(bytes, 36));
     FI_HIP(hipStreamWaitEvent(stream, rec_ev[s], 0));
+    return s;
+}
+
+// the pack kernel, on the compute stream (also inside a captured frame)
+void fi_frame::pack_records(const fi_poi* src, int s)
+{
     launch_pack_raw(src, order.empty() ? nullptr : d_order.p, (int)n, d_rec[s].p, stream);
+}
+
+// the copy of slot s to host memory on rec_stream, after the work queued on
+// the compute stream so far
+void fi_frame::copy_records(int s)
+{
+    const size_t bytes = n * (size_t)36;
     FI_HIP(hipGetLastError());
     FI_HIP(hipEventRecord(pack_ev, stream));
     FI_HIP(hipStreamWaitEvent(rec_stream, pack_ev, 0));

@@ -198,7 +198,7 @@ struct fi_frame {
     int frame_index = 0;
     hipEvent_t ev[8] = {};
     struct Graph {
-        uint64_t key[10];
+        uint64_t key[11];
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;     // instantiated frame graphs (cleared on any change)
@@ -230,6 +230,9 @@ struct fi_frame {
 private:
     void prepare();
     void queue_records(const fi_poi* src);
+    int records_slot();
+    void pack_records(const fi_poi* src, int s);
+    void copy_records(int s);
     void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev,
                     const unsigned* count_dev = nullptr);
     // region: which third of `iters` the launch writes its per-entry
