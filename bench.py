@@ -454,6 +454,7 @@ def main() -> int:
             drv.upload_target(tar_img)
             drv.run(policy=policy)
         drv.run(policy=policy)  # drain: no target queued
+        drv.records(1, copy=False)
         sync()
         stats, checks = [], []
         t0 = time.perf_counter()
@@ -539,6 +540,19 @@ def main() -> int:
             par = compare(o64, r["res"][pick64])
             parity64[name] = {k: par.get(k) for k in keys}
         log(f"[bench] FP64-oracle parity: {parity64}")
+        if "learned" in runs:
+            # the learned frame against the oracle run from the same seeds (the
+            # seeder's output on the sample, FFTCC fallback below the
+            # threshold): iteration counts comparable, unlike against the
+            # FFTCC-seeded oracle above
+            from tests.policy import PolicyOracle
+            seeds = sc.pois[pick].copy()
+            ok = seeder.seed(ref32, tar32, seeds)
+            po = PolicyOracle(pyoracle, sc.ref, sc.pois[pick], sc.radius)
+            o_same, _ = po.learned_frame(sc.tar, seeds, ok, args.seed_zncc)
+            par = compare(o_same, runs["learned"]["res"][pick])
+            parity["learned_same_seeds"] = {k: par.get(k) for k in keys}
+            log(f"[bench] parity learned, oracle from the same seeds: {parity['learned_same_seeds']}")
 
     def gate(name: str) -> bool:
         """The parity gate against the CPU oracle (u, v and the gradients:
@@ -583,6 +597,9 @@ def main() -> int:
                "converged_per_step": mm["valid_count"], "parity_vs_cpu_oracle": parity.get(name),
                "parity_vs_fp64_oracle": parity64.get(name), "error_vs_truth": truth[name]}
         if name == "learned":
+            # against the FFTCC-seeded oracle (the reference pipeline) above;
+            # here against the oracle from the same learned seeds
+            out["parity_vs_cpu_oracle_same_seeds"] = parity.get("learned_same_seeds")
             out.update({"seed_ms": round(mm["seed_ms"], 4), "fftcc_fallback_pois": mm["seed_fallback_count"],
                         "fftcc_fallback_ms": round(mm["fftcc_ms"], 4), "zncc_threshold": args.seed_zncc})
         return out
