@@ -454,7 +454,7 @@ def main() -> int:
             drv.upload_target(tar_img)
             drv.run(policy=policy)
         drv.run(policy=policy)  # drain: no target queued
-        drv.records(1, copy=False)
+        drv.records(0, copy=False)
         sync()
         stats, checks = [], []
         t0 = time.perf_counter()
