@@ -334,7 +334,7 @@ def main() -> int:
     ap.add_argument("--cpu-sample-every", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--shape", default="", help="IC-GN lane-to-POI shape WAVES,ROWS")
-    ap.add_argument("--sampling", default="tune", choices=["tune", "auto", "table", "group"],
+    ap.add_argument("--sampling", default="tune", choices=["tune", "auto", "table", "group", "pair"],
                     help="IC-GN launch: 'tune' = the autotuner's choice on this frame (lane shape, sampling, "
                          "launch mode), or a fixed sampling mode (per-POI LDS tile, table only, or tiles "
                          "shared by four neighbouring POIs)")

@@ -28,8 +28,9 @@ FI_NORMALIZE_NONE, FI_NORMALIZE_TO8, FI_NORMALIZE_TO16, FI_NORMALIZE_UNIT = 0x00
 FI_POLICY_SKIP_FFTCC = 1
 FI_POLICY_POI_FALLBACK = 2
 FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
-FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP = 0, 1, 2
-SAMPLING_MODES = {"auto": FI_SAMPLING_AUTO, "table": FI_SAMPLING_TABLE, "group": FI_SAMPLING_GROUP}
+FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP, FI_SAMPLING_PAIR = 0, 1, 2, 3
+SAMPLING_MODES = {"auto": FI_SAMPLING_AUTO, "table": FI_SAMPLING_TABLE, "group": FI_SAMPLING_GROUP,
+                  "pair": FI_SAMPLING_PAIR}
 FI_POLICY_LEARNED_SEED = 4
 FI_EXPORT_REF_F32, FI_EXPORT_GRAD_X, FI_EXPORT_GRAD_Y, FI_EXPORT_TAR_F32, FI_EXPORT_BSPLINE, FI_EXPORT_PREP = range(6)
 # PoiPrep (fi_params.h), the records FI_EXPORT_PREP copies out: 176 bytes

@@ -229,7 +229,7 @@ int fi_icgn_set_shape(fi_icgn* h, int waves, int spl)
 
 static void check_sampling(int mode)
 {
-    if (mode != FI_SAMPLING_AUTO && mode != FI_SAMPLING_TABLE && mode != FI_SAMPLING_GROUP)
+    if (mode != FI_SAMPLING_AUTO && mode != FI_SAMPLING_TABLE && mode != FI_SAMPLING_GROUP && mode != FI_SAMPLING_PAIR)
         fail(FI_E_INVALID_ARGUMENT, "unknown IC-GN sampling mode " + std::to_string(mode));
 }
 

@@ -25,6 +25,13 @@ int safe_int(float v)
 
 std::string size_str(int w, int h) { return std::to_string(w) + "x" + std::to_string(h); }
 
+// IcgnParams::group for a sampling mode: 1 shared coefficient tiles, 2 two
+// entries per wave over shared tiles, 0 otherwise
+int group_code(int sampling)
+{
+    return sampling == FI_SAMPLING_GROUP ? 1 : sampling == FI_SAMPLING_PAIR ? 2 : 0;
+}
+
 // The IC-GN kernels read the reference, gradient and target images through
 // buffer resources with 32-bit byte offsets.
 void check_image_offsets(int w, int h)
@@ -398,7 +405,7 @@ void fi_icgn::compute(fi_poi* pois, size_t n, const float* offsets)
     p.gy = gy.p;
     p.coef = coef.p;
     p.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
-    p.group = sampling == FI_SAMPLING_GROUP ? 1 : 0;
+    p.group = group_code(sampling);
     p.w = ref_raw.view.width;
     p.h = ref_raw.view.height;
     p.pois = d_pois.p;
@@ -486,7 +493,7 @@ void fi_icgn::compute_noalloc(fi_poi* d_pois, size_t n)
     p.gy = gy.p;
     p.coef = coef.p;
     p.tar = sampling == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
-    p.group = sampling == FI_SAMPLING_GROUP ? 1 : 0;
+    p.group = group_code(sampling);
     p.w = ref_raw.view.width;
     p.h = ref_raw.view.height;
     p.pois = d_pois;
@@ -971,7 +978,7 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     // samples the f32 target when a warp outgrows the tile
     ip.coef = mode == FI_SAMPLING_TABLE ? coef.p : nullptr;
     ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
-    ip.group = mode == FI_SAMPLING_GROUP ? 1 : 0;
+    ip.group = group_code(mode);
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
     ip.pois = queue_base;
@@ -1028,7 +1035,7 @@ void fi_frame::icgn_fused(fi_poi* base, int region)
     // samples the f32 target when a warp outgrows the tile
     ip.coef = mode == FI_SAMPLING_TABLE ? coef.p : nullptr;
     ip.tar = mode == FI_SAMPLING_TABLE ? nullptr : tar_f.p;
-    ip.group = mode == FI_SAMPLING_GROUP ? 1 : 0;
+    ip.group = group_code(mode);
     ip.w = ref_raw.view.width;
     ip.h = ref_raw.view.height;
     ip.pois = base;
@@ -1538,12 +1545,15 @@ int fi_frame::autotune(fi_tune_entry* report, size_t report_len, size_t* report_
         int bw = 0, bs = 0, bsamp = -1, bq = 0;
         size_t best_entry = SIZE_MAX;
         for (int si = 0; si < ns; si++) {
-            for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP }) {
+            for (int samp : { FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP, FI_SAMPLING_PAIR }) {
                 // shared coefficient tiles: one-wave shapes, one block per
-                // four entries (no queue)
+                // four entries (no queue); two entries per wave: once per
+                // class (its layout follows from the radius), where built
                 if (samp == FI_SAMPLING_GROUP && ws[si] != 1) continue;
-                const std::vector<int> qbs =
-                    samp == FI_SAMPLING_GROUP ? std::vector<int>{ 0 } : queue_counts(c.r, ws[si], ss[si], samp);
+                if (samp == FI_SAMPLING_PAIR && (si != 0 || !icgn_pair_supported(c.r, c.r))) continue;
+                const std::vector<int> qbs = samp == FI_SAMPLING_GROUP || samp == FI_SAMPLING_PAIR
+                                                 ? std::vector<int>{ 0 }
+                                                 : queue_counts(c.r, ws[si], ss[si], samp);
                 for (int qb : qbs) {
                     c.waves = ws[si];
                     c.spl = ss[si];

@@ -31,6 +31,10 @@ bool launch_icgn_shape(const IcgnParams& p, int waves, int spl, hipStream_t s);
 // does not cover the radius.
 int icgn_queue_per_cu(const IcgnParams& p, int waves, int spl);
 bool launch_icgn(const IcgnParams& p, hipStream_t s);
+// Whether the two-entries-per-wave kernel (IcgnParams::group == 2) runs
+// uniform subsets of this radius; launches it does not run take the shared-
+// tile kernel.
+bool icgn_pair_supported(int rx, int ry);
 
 // Frame-policy helpers (k_policy.hip).
 struct FrameCounters {

@@ -276,6 +276,9 @@ __global__ __launch_bounds__(64) void k_prep_inverse(IcgnParams p)
 FI_ICGN_EXACT_A(X)
 FI_ICGN_EXACT_B(X)
 #undef X
+#define X(S) extern template void launch_icgn_pair_t<S>(const IcgnParams&, hipStream_t);
+FI_ICGN_PAIR(X)
+#undef X
 
 static bool exact_shape(int waves, int spl)
 {
@@ -406,6 +409,25 @@ bool launch_icgn_shape(const IcgnParams& p0, int waves, int spl, hipStream_t s)
         else if (waves == 4) launch_icgn_offset_t<4>(p, s);
         else return false;
         return true;
+    }
+    if (p.group == 2) {
+        // two entries per wave (the lane shape does not apply): uniform
+        // subsets whose 32-lane band layout is instantiated, in one block per
+        // eight entries; anything else runs as the shared-tile kernel
+        const int R = pair_rows(p.rx, p.ry);
+        if (!p.count && !p.self_adaptive && p.tar != nullptr) {
+            IcgnParams q = p;
+            q.tile_h = 2 * p.ry + 4 + 2 * kGroupMargin + kGroupRowSlack;
+            q.tile_stride = kPairStride;
+#define X(S)                                                                    \
+            if (R == S) {                                                       \
+                if (q.n > 0) launch_icgn_pair_t<S>(q, s);                       \
+                return true;                                                    \
+            }
+            FI_ICGN_PAIR(X)
+#undef X
+        }
+        p.group = 1;
     }
     if (p.group) {
         // shared coefficient tiles serve one-wave shapes on neighbouring

@@ -255,18 +255,22 @@ constexpr int kGroupMargin = FI_GROUP_MARGIN;
 // is image pixel (tx0, ty0) holds the polynomial of pixel column tx0 + 1 +
 // col; cb = -(ty0 + 1) * kGroupStride - (tx0 + 1) places the sample's first
 // row.  The caller guarantees the neighbourhood lies in the tile.
+template <int STRIDE = kGroupStride>
 __device__ __forceinline__ float coef_sample(const float4* ct, int cb, float gx, float gy, float bias)
 {
     static_assert(kGroupStride * sizeof(float4) == 1024, "a row of entries is 1 KiB: a shift by 10");
     const float dx = __builtin_amdgcn_fractf(gx), dy = __builtin_amdgcn_fractf(gy);
-    const uint32_t rowb = ((uint32_t)(int)gy << 10) + (uint32_t)(cb * 16);
+    // rows of 1 KiB: a shift; other strides: one 24-bit multiply-add
+    const uint32_t rowb = STRIDE == kGroupStride
+                              ? ((uint32_t)(int)gy << 10) + (uint32_t)(cb * 16)
+                              : __umul24((uint32_t)(int)gy, (uint32_t)(STRIDE * 16)) + (uint32_t)(cb * 16);
     const float4* q = reinterpret_cast<const float4*>(reinterpret_cast<const char*>(ct) +
                                                       (((uint32_t)(int)gx << 4) + rowb));
     auto horner = [&](const float4 c) {
         return __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, dx, c.z), dx, c.y), dx, c.x);
     };
-    const float r0 = horner(q[0]), r1 = horner(q[kGroupStride]);
-    const float r2 = horner(q[2 * kGroupStride]), r3 = horner(q[3 * kGroupStride]);
+    const float r0 = horner(q[0]), r1 = horner(q[STRIDE]);
+    const float r2 = horner(q[2 * STRIDE]), r3 = horner(q[3 * STRIDE]);
     return ycombine(dy, r0, r1, r2, r3, bias);
 }
 
@@ -284,14 +288,14 @@ __device__ __forceinline__ float4 row_coef(float q0, float q1, float q2, float q
     return v;
 }
 
-template <int WAVES>
+template <int WAVES, int STRIDE = kGroupStride>
 __device__ __forceinline__ void stage_coef(float4* ct, const float* tar, int w, int tx0, int ty0, int tw, int th,
                                            int lane, int wave)
 {
     if (lane >= tw - 3) return;
     constexpr int kRows = 3;
     const float* q = tar + (size_t)(ty0 + wave) * w + (tx0 + lane);
-    float4* d = ct + wave * kGroupStride + lane;
+    float4* d = ct + wave * STRIDE + lane;
     const size_t step = (size_t)WAVES * w;
     int r = wave;
     for (; r + (kRows - 1) * WAVES < th; r += kRows * WAVES) {
@@ -301,14 +305,14 @@ __device__ __forceinline__ void stage_coef(float4* ct, const float* tar, int w, 
 #pragma unroll
             for (int m = 0; m < 4; m++) v[j][m] = q[j * step + m];
 #pragma unroll
-        for (int j = 0; j < kRows; j++) d[j * WAVES * kGroupStride] = row_coef(v[j][0], v[j][1], v[j][2], v[j][3]);
+        for (int j = 0; j < kRows; j++) d[j * WAVES * STRIDE] = row_coef(v[j][0], v[j][1], v[j][2], v[j][3]);
         q += kRows * step;
-        d += kRows * WAVES * kGroupStride;
+        d += kRows * WAVES * STRIDE;
     }
     for (; r < th; r += WAVES) {
         *d = row_coef(q[0], q[1], q[2], q[3]);
         q += step;
-        d += WAVES * kGroupStride;
+        d += WAVES * STRIDE;
     }
 }
 
@@ -1044,6 +1048,433 @@ void k_icgn_group(IcgnParams p)
         icgn_entry<1, SPL, EXACT, true>(p, b, red, flags + wave, reinterpret_cast<float*>(ctile), hl + 36 * wave, X0,
                                         Y0, gw, gh);
 }
+
+// ---------------------------------------------------------------------------
+// Two entries per wave (sampling mode FI_SAMPLING_PAIR, k_icgn_pair): lanes
+// 0-31 run one entry and lanes 32-63 the next, each with the band layout of a
+// 32-lane entry (G = 32 / sw bands of R = SPL rows).  Both halves execute every
+// instruction, so the work of an iteration that does not grow with the subset
+// (the cross-lane sums, the 6x6 solve, the warp update, the convergence test)
+// serves two entries, and the per-entry set-up is shared the same way.  The
+// cross-lane trees never cross the halves (permlane16 swaps, rotations inside
+// 16-lane rows, DPP inside 8-lane half rows); each lane reads its half's totals
+// back with ds_bpermute, which occupies the LDS crossbar instead of VALU issue.
+// The wave iterates while either entry does: a finished entry has written its
+// record, and its lanes then sample at the other entry's coordinates (always a
+// valid path) and their sums are discarded.  Eight neighbouring entries (four
+// waves) share one coefficient tile, kPairStride entries per row.
+// ---------------------------------------------------------------------------
+constexpr int kPairEntries = 8;
+constexpr int kPairStride = 80;  // 1280 B rows: eight POIs 5 px apart at r = 15 need 74 entries
+constexpr int kPairLanes = 32;
+
+// the value of v in the lane at byte address addr (lane * 4)
+__device__ __forceinline__ float lane_fetch(float v, int addr)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(v)));
+}
+
+// the same lane's v in the other half of the wave
+__device__ __forceinline__ float other_half(float v, bool upper)
+{
+    const float2 s = swap32(v, v);  // x: lower half of v in both halves, y: upper half
+    return upper ? s.x : s.y;
+}
+
+// fold16 of raw per-lane values inside each 32-lane half: in each half, the
+// first row carries a and the second b, each summed over two lanes 16 apart
+__device__ __forceinline__ CPair hfold16(float a, float b)
+{
+    const float2 v = swap16(a, b);
+    CPair r;
+    two_sum(v.x, v.y, r.s, r.c);
+    return r;
+}
+
+// Per-lane partials -> totals of each 32-lane half (compensated), read back by
+// every lane of the half; hb = byte address of the half's lane 0.
+__device__ __forceinline__ void half_sum3(float (&v)[3], int hb)
+{
+    const CPair w0 = hfold16(v[0], v[1]);      // rows per half: v0, v1
+    const CPair w1 = hfold16(v[2], v[2]);      // rows per half: v2, v2
+    const CPair z = half_sum8(fold8(w0, w1));  // half rows: v0 | v2, v1 | v2
+    const float t = z.s + z.c;
+    v[0] = lane_fetch(t, hb);
+    v[2] = lane_fetch(t, hb + 32);
+    v[1] = lane_fetch(t, hb + 64);
+}
+
+__device__ __forceinline__ void half_sum6(float (&v)[6], int hb)
+{
+    const CPair w0 = hfold16(v[0], v[1]);
+    const CPair w1 = hfold16(v[2], v[3]);
+    const CPair w2 = hfold16(v[4], v[5]);
+    const CPair z0 = half_sum8(fold8(w0, w1));          // half rows: v0 | v2, v1 | v3
+    const CPair z1 = half_sum8(cdpp<kDppRowRor8>(w2));  // rows: v4, v5
+    const float t0 = z0.s + z0.c, t1 = z1.s + z1.c;
+    v[0] = lane_fetch(t0, hb);
+    v[2] = lane_fetch(t0, hb + 32);
+    v[1] = lane_fetch(t0, hb + 64);
+    v[3] = lane_fetch(t0, hb + 96);
+    v[4] = lane_fetch(t1, hb);
+    v[5] = lane_fetch(t1, hb + 64);
+}
+
+// any lane of this lane's half
+__device__ __forceinline__ bool half_any(bool v, bool upper)
+{
+    const uint64_t m = __builtin_amdgcn_ballot_w64(v);
+    return (upper ? (m >> 32) : (m & 0xffffffffull)) != 0;
+}
+
+// Band layout of a 32-lane entry: the shape k_icgn_pair<SPL> runs (0 = none).
+inline int pair_rows(int rx, int ry)
+{
+    const int sw = 2 * rx + 1, sh = 2 * ry + 1;
+    if (rx < 1 || ry < 1 || sw > kPairLanes) return 0;
+    const int G = kPairLanes / sw;
+    const int R = (sh + G - 1) / G;
+    const int gu = (sh + R - 1) / R;
+    return gu * R - sh <= kMaxOverlap ? R : 0;
+}
+
+// Entry b of the launch on this lane's half (b >= p.n: no entry), slot e of
+// the block.  The entry's set-up loads are issued first, so they are in
+// flight while the block agrees on its coefficient tile: the union of the
+// entries' incoming neighbourhoods plus kGroupMargin (gbox: one box per slot),
+// staged once (p.tile_h rows of kPairStride entries; a union that does not fit
+// leaves the block without a tile).  The tile is never restaged; an iteration
+// whose neighbourhoods leave it samples the target in device memory.  Uniform
+// radius p.rx, p.ry (the host launches no self-adaptive or centre-offset work
+// here).  Both barriers are reached by every thread before any iteration.
+template <int SPL>
+__device__ __forceinline__ void icgn_pair_entry(const IcgnParams& p, const int b, const int e, float* hl,
+                                                float4* ct, int (*gbox)[4])
+{
+    const int lane = threadIdx.x & 63, hlane = lane & 31;
+    const bool upper = lane >= 32;
+    const int hb = (lane & 32) * 4;
+    const bool in = b < p.n;
+    const int i = in ? (p.index ? (int)p.index[b] : b) : 0;
+    fi_poi* poi = p.pois + i;
+    float xc = poi->x, yc = poi->y;
+    const float u_in = poi->deformation[0], ux_in = poi->deformation[1], uy_in = poi->deformation[2];
+    const float v_in = poi->deformation[6], vx_in = poi->deformation[7], vy_in = poi->deformation[8];
+    const float z_in = poi->result[2];
+    const int rx = p.rx, ry = p.ry, w = p.w, h = p.h;
+    // entry gate, src/oc_icgn.cpp:184-191
+    // (a NaN zncc passes: the gate rejects zncc < 0 only)
+    bool live = in && ref_gate_ok(xc, yc, rx, ry, w, h) && fabsf(u_in) < (float)w && fabsf(v_in) < (float)h &&
+                !(z_in < 0.f) && !isnan(u_in) && !isnan(v_in);
+    if (in && !live && hlane == 0) {
+        poi->result[2] = (z_in >= 0.f) ? -3.f : z_in;
+        if (p.iters) p.iters[b] = 0;
+    }
+    const PoiPrep* pr = p.prep + (p.prep_index ? p.prep_index[i] : (uint32_t)i);
+    const float ref_mean = pr->mean, ref_norm = pr->norm, ref_sum = pr->ref_sum;
+    const float inv_ref_norm = 1.f / ref_norm;
+    // the entry's inverse Hessian in LDS (36 floats), for lanes a < 6
+    hl[hlane] = pr->hinv[hlane];
+    if (hlane < 4) hl[32 + hlane] = pr->hinv[32 + hlane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+
+    const Bands bd = make_bands(rx, ry, kPairLanes, hlane);
+    const int ovl = bd.ovl;
+    float W00 = 1.f + ux_in, W01 = uy_in, W02 = u_in;
+    float W10 = vx_in, W11 = 1.f + vy_in, W12 = v_in;
+    float W20 = 0.f, W21 = 0.f, W22 = 1.f;
+    const float wm2 = (float)(w - 2), hm2 = (float)(h - 2);
+    const float rxf = (float)rx, ryf = (float)ry;
+    const float rx2 = (float)rx * (float)rx, ry2 = (float)ry * (float)ry;
+    const float inv_ns = 1.f / ((float)(2 * rx + 1) * (float)(2 * ry + 1));
+
+    // Reference data per slot (zero on masked slots).  A half without an
+    // entry reads nothing: its offsets lie past the buffers' ranges, which the
+    // hardware answers with zeros.
+    float rz[SPL], gxv[SPL], gyv[SPL];
+    const uint32_t pitch = (uint32_t)w * 4u, img_bytes = pitch * (uint32_t)h;
+    const __amdgpu_buffer_rsrc_t rs_ref = table_rsrc(p.ref, img_bytes);
+    const __amdgpu_buffer_rsrc_t rs_gx = table_rsrc(p.gx, img_bytes), rs_gy = table_rsrc(p.gy, img_bytes);
+    uint32_t ro = 0xfffffff0u;
+    if (live) {
+        const int ulx = (int)(xc - rx), uly = (int)(yc - ry);
+        ro = (__umul24((uint32_t)(uly + bd.row0), (uint32_t)w) + (uint32_t)(ulx + bd.col)) * 4u;
+    }
+#pragma unroll
+    for (int k = 0; k < SPL; k++) {
+        const uint32_t so = (uint32_t)k * pitch;
+        const float rv = ld1(rs_ref, ro, so);
+        const float gxl = ld1(rs_gx, ro, so), gyl = ld1(rs_gy, ro, so);
+        const bool valid = bd.active && !(bd.last && k < ovl);
+        rz[k] = valid ? rv - ref_mean : 0.f;
+        gxv[k] = valid ? gxl : 0.f;
+        gyv[k] = valid ? gyl : 0.f;
+    }
+
+    // the block's tile: this entry's incoming neighbourhood (entry_box)
+    {
+        const float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
+        const float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
+        const float cx = xc + W02, cy = yc + W12;
+        const bool has = live && cx - ex >= 1.f + kEdgeMargin && cx + ex < wm2 - kEdgeMargin &&
+                         cy - ey >= 1.f + kEdgeMargin && cy + ey < hm2 - kEdgeMargin;
+        if (hlane == 0) {
+            gbox[e][0] = has ? max((int)(cx - ex - kTileSlack) - 1, 0) : INT_MAX;
+            gbox[e][1] = min((int)(cx + ex + kTileSlack) + 2, w - 1);
+            gbox[e][2] = max((int)(cy - ey - kTileSlack) - 1, 0);
+            gbox[e][3] = min((int)(cy + ey + kTileSlack) + 2, h - 1);
+        }
+    }
+    __syncthreads();
+    int gx0 = INT_MAX, gx1 = INT_MIN, gy0 = INT_MAX, gy1 = INT_MIN;
+#pragma unroll
+    for (int v = 0; v < kPairEntries; v++)
+        if (gbox[v][0] != INT_MAX) {
+            gx0 = min(gx0, gbox[v][0]);
+            gx1 = max(gx1, gbox[v][1]);
+            gy0 = min(gy0, gbox[v][2]);
+            gy1 = max(gy1, gbox[v][3]);
+        }
+    int gw = 0, gh = 0;
+    if (gx0 <= gx1) {
+        gx0 = max(gx0 - kGroupMargin, 0);
+        gx1 = min(gx1 + kGroupMargin, w - 1);
+        gy0 = max(gy0 - kGroupMargin, 0);
+        gy1 = min(gy1 + kGroupMargin, h - 1);
+        if (gx1 - gx0 - 2 <= kPairStride && gy1 - gy0 + 1 <= p.tile_h) {
+            gw = gx1 - gx0 + 1;
+            gh = gy1 - gy0 + 1;
+        }
+    }
+    if (gw > 0) {
+        // columns lane and lane + 64 (a tile row holds up to kPairStride entries)
+        const int wave = threadIdx.x >> 6;
+        stage_coef<kPairEntries / 2, kPairStride>(ct, p.tar, w, gx0, gy0, gw, gh, lane, wave);
+        if (gw - 3 > 64)
+            stage_coef<kPairEntries / 2, kPairStride>(ct + 64, p.tar, w, gx0 + 64, gy0, gw - 64, gh, lane, wave);
+    }
+    __syncthreads();
+
+    const float m_all = bd.active ? 1.f : 0.f;
+    const float m_dup = bd.last ? 0.f : 1.f;
+    const float xl = (float)(bd.col - rx), yl0 = (float)(bd.row0 - ry);
+    const int cbase = -(gy0 + 1) * kPairStride - (gx0 + 1);
+    const float4* tile = ct;
+
+    int it = 0;
+    while (__builtin_amdgcn_ballot_w64(live) != 0) {
+        // a finished (or empty) half takes the other half's centre and warp
+        if (__builtin_amdgcn_ballot_w64(!live) != 0) {
+            const float oxc = other_half(xc, upper), oyc = other_half(yc, upper);
+            const float o00 = other_half(W00, upper), o01 = other_half(W01, upper), o02 = other_half(W02, upper);
+            const float o10 = other_half(W10, upper), o11 = other_half(W11, upper), o12 = other_half(W12, upper);
+            if (!live) {
+                xc = oxc; yc = oyc;
+                W00 = o00; W01 = o01; W02 = o02; W10 = o10; W11 = o11; W12 = o12;
+            }
+        }
+        it += live ? 1 : 0;
+        float xb = xc + __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
+        float yb = yc + __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
+        float ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
+        float ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
+        float cx = xc + W02, cy = yc + W12;
+        const bool box_in = (cx - ex >= 1.f + kEdgeMargin) & (cx + ex < wm2 - kEdgeMargin) &
+                            (cy - ey >= 1.f + kEdgeMargin) & (cy + ey < hm2 - kEdgeMargin);
+        if (__builtin_amdgcn_ballot_w64(!box_in) != 0) {
+            // per-sample test (:290-309): out of range (-3) before NaN (-5)
+            bool oob = false, nan = false;
+#pragma unroll
+            for (int k = 0; k < SPL; k++) {
+                const float gxp = __builtin_fmaf((float)k, W01, xb);
+                const float gyp = __builtin_fmaf((float)k, W11, yb);
+                oob |= (gxp < 1.f) | (gyp < 1.f) | (gxp >= wm2) | (gyp >= hm2);
+                nan |= (gxp != gxp) | (gyp != gyp);
+            }
+            const bool any_oob = half_any(oob, upper), any_nan = half_any(nan, upper);
+            const bool fail = live && (any_oob || any_nan);
+            if (fail && hlane == 0) {
+                poi->result[2] = any_oob ? -3.f : -5.f;
+                if (p.iters) p.iters[b] = (uint32_t)it;
+            }
+            live = live && !fail;
+            if (__builtin_amdgcn_ballot_w64(live) == 0) break;
+            if (__builtin_amdgcn_ballot_w64(fail) != 0) {
+                // the failed half samples at the other half's coordinates
+                const float oxc = other_half(xc, upper), oyc = other_half(yc, upper);
+                const float o00 = other_half(W00, upper), o01 = other_half(W01, upper), o02 = other_half(W02, upper);
+                const float o10 = other_half(W10, upper), o11 = other_half(W11, upper), o12 = other_half(W12, upper);
+                if (fail) {
+                    xc = oxc; yc = oyc;
+                    W00 = o00; W01 = o01; W02 = o02; W10 = o10; W11 = o11; W12 = o12;
+                    xb = xc + __builtin_fmaf(W00, xl, __builtin_fmaf(W01, yl0, W02));
+                    yb = yc + __builtin_fmaf(W10, xl, __builtin_fmaf(W11, yl0, W12));
+                    ex = __builtin_fmaf(fabsf(W00), rxf, fabsf(W01) * ryf);
+                    ey = __builtin_fmaf(fabsf(W10), rxf, fabsf(W11) * ryf);
+                    cx = xc + W02;
+                    cy = yc + W12;
+                }
+            }
+        }
+        const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
+        const int nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
+        const int ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
+        const int ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
+        const bool use_tile = gw > 0 && nx0 >= gx0 && nx1 < gx0 + gw && ny0 >= gy0 && ny1 < gy0 + gh;
+
+        // pass 1: t' = t - ref_mean per slot, sums of t', t'^2, t' r
+        float tpv[SPL];
+        float st = 0.f, stt = 0.f, str = 0.f;
+        auto pass1 = [&](auto sample) {
+#pragma unroll
+            for (int k = 0; k < SPL; k++) {
+                const float gxp = __builtin_fmaf((float)k, W01, xb);
+                const float gyp = __builtin_fmaf((float)k, W11, yb);
+                const float tp = sample(gxp, gyp);
+                const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
+                tpv[k] = tp;
+                st += tpm;
+                stt = __builtin_fmaf(tpm, tp, stt);
+                str = __builtin_fmaf(tp, rz[k], str);
+            }
+        };
+        if (use_tile)
+            pass1([&](float gx, float gy) { return coef_sample<kPairStride>(tile, cbase, gx, gy, -ref_mean); });
+        else
+            pass1([&](float gx, float gy) { return target_sample(p.tar, (uint32_t)w, gx, gy, -ref_mean); });
+        float s1[3] = { st * m_all, stt * m_all, str };
+        half_sum3(s1, hb);
+
+        // ZNSSD terms (Subset2D::zeroMeanNorm, :311-322)
+        const float tm = s1[0] * inv_ns;
+        const float szz = __builtin_fmaf(-s1[0], tm, s1[1]);
+        const float inv_tar_norm = __builtin_amdgcn_rsqf(szz);
+        const float scale = (szz > 0.f) ? ref_norm * inv_tar_norm : __builtin_nanf("");
+        const float zncc = __builtin_fmaf(-tm, ref_sum, s1[2]) * inv_tar_norm * inv_ref_norm;
+
+        // pass 2: e = (t - tbar) * (nr / nt) - r, sums SD_j e; slots in
+        // descending order so AK = sum_k k a_k is a running suffix sum
+        float A = 0.f, AK = 0.f, B = 0.f, BK = 0.f;
+#pragma unroll
+        for (int k = SPL - 1; k >= 0; k--) {
+            const float e = __builtin_fmaf(scale, tpv[k] - tm, -rz[k]);
+            AK += A;
+            BK += B;
+            A = __builtin_fmaf(gxv[k], e, A);
+            B = __builtin_fmaf(gyv[k], e, B);
+        }
+        float bnum[6] = { A, xl * A, __builtin_fmaf(yl0, A, AK), B, xl * B, __builtin_fmaf(yl0, B, BK) };
+        half_sum6(bnum, hb);
+
+        // dp = H^-1 b (:324-331): lane a < 6 of the half forms dp[a]
+        float dpl = 0.f;
+        {
+            const float* hr = hl + 6 * min(hlane, 5);
+#pragma unroll
+            for (int c = 0; c < 6; c++) dpl = __builtin_fmaf(hr[c], bnum[c], dpl);
+        }
+        float dp[6];
+#pragma unroll
+        for (int a = 0; a < 6; a++) dp[a] = lane_fetch(dpl, hb + 4 * a);
+        // convergence norm (:343-352) in FP64, against the launcher's threshold
+        double cn = (double)dp[0] * dp[0];
+        cn += (double)dp[1] * dp[1] * rx2;
+        cn += (double)dp[2] * dp[2] * ry2;
+        cn += (double)dp[3] * dp[3];
+        cn += (double)dp[4] * dp[4] * rx2;
+        cn += (double)dp[5] * dp[5] * ry2;
+
+        // W <- W * W(dp)^-1 (:333-340), as in icgn_entry
+        const float D00 = 1.f + dp[1], D01 = dp[2], D02 = dp[0];
+        const float D10 = dp[4], D11 = 1.f + dp[5], D12 = dp[3];
+        if (cn <= __DBL_MAX__) {
+            const float c20 = D01 * D12 - D02 * D11, c21 = D02 * D10 - D00 * D12;
+            const float det = D11 * D00 + -(D01 * D10);
+            const float invdet = __builtin_amdgcn_rcpf(det);
+            const float I00 = D11 * invdet, I01 = -D01 * invdet, I02 = c20 * invdet;
+            const float I10 = -D10 * invdet, I11 = D00 * invdet, I12 = c21 * invdet;
+            const float I22 = (D00 * D11 - D01 * D10) * invdet;
+            const float N00 = W00 * I00 + W01 * I10, N01 = W00 * I01 + W01 * I11;
+            const float N02 = W00 * I02 + (W01 * I12 + W02 * I22);
+            const float N10 = W10 * I00 + W11 * I10, N11 = W10 * I01 + W11 * I11;
+            const float N12 = W10 * I02 + (W11 * I12 + W12 * I22);
+            W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12;
+            W22 = W22 * I22;
+        } else {
+            const float D20 = 0.f, D21 = 0.f, D22 = 1.f;
+            const float c00 = D11 * D22 - D12 * D21, c01 = D12 * D20 - D10 * D22, c02 = D10 * D21 - D11 * D20;
+            const float c10 = D21 * D02 - D22 * D01, c11 = D22 * D00 - D20 * D02, c12 = D20 * D01 - D21 * D00;
+            const float c20 = D01 * D12 - D02 * D11, c21 = D02 * D10 - D00 * D12, c22 = D00 * D11 - D01 * D10;
+            const float det = c00 * D00 + (c10 * D10 + c20 * D20);
+            const float invdet = __builtin_amdgcn_rcpf(det);
+            const float I00 = c00 * invdet, I01 = c10 * invdet, I02 = c20 * invdet;
+            const float I10 = c01 * invdet, I11 = c11 * invdet, I12 = c21 * invdet;
+            const float I20 = c02 * invdet, I21 = c12 * invdet, I22 = c22 * invdet;
+            const float N00 = W00 * I00 + (W01 * I10 + W02 * I20), N01 = W00 * I01 + (W01 * I11 + W02 * I21);
+            const float N02 = W00 * I02 + (W01 * I12 + W02 * I22), N10 = W10 * I00 + (W11 * I10 + W12 * I20);
+            const float N11 = W10 * I01 + (W11 * I11 + W12 * I21), N12 = W10 * I02 + (W11 * I12 + W12 * I22);
+            const float N20 = W20 * I00 + (W21 * I10 + W22 * I20), N21 = W20 * I01 + (W21 * I11 + W22 * I21);
+            const float N22 = W20 * I02 + (W21 * I12 + W22 * I22);
+            W00 = N00; W01 = N01; W02 = N02; W10 = N10; W11 = N11; W12 = N12; W20 = N20; W21 = N21; W22 = N22;
+        }
+        const bool fin = live && !((float)it < p.stop && cn >= p.conv2);
+        if (__builtin_amdgcn_ballot_w64(fin) != 0) {
+            if (fin && hlane == 0) {
+                const float conv = (float)sqrt(cn);
+                if (p.iters) p.iters[b] = (uint32_t)it;
+                float u = W02, ux = W00 - 1.f, uy = W01, v = W12, vx = W10, vy = W11 - 1.f;
+                float z = zncc;
+                if (conv >= p.conv && (float)it >= p.stop) z = -4.f;
+                if (isnan(z) || isnan(u) || isnan(v)) {
+                    u = u_in;
+                    v = v_in;
+                    z = -5.f;
+                }
+                poi->deformation[0] = u;
+                poi->deformation[1] = ux;
+                poi->deformation[2] = uy;
+                poi->deformation[6] = v;
+                poi->deformation[7] = vx;
+                poi->deformation[8] = vy;
+                poi->result[0] = u_in;
+                poi->result[1] = v_in;
+                poi->result[2] = z;
+                poi->result[3] = (float)it;
+                poi->result[4] = conv;
+                poi->subset_radius[0] = (float)rx;
+                poi->subset_radius[1] = (float)ry;
+            }
+            live = live && !fin;
+        }
+    }
+}
+
+// kPairEntries consecutive entries per block, two per wave, sharing one
+// coefficient tile (icgn_pair_entry).
+template <int SPL>
+__global__ __launch_bounds__(64 * kPairEntries / 2) __attribute__((amdgpu_waves_per_eu(2)))
+void k_icgn_pair(IcgnParams p)
+{
+    __shared__ int gbox[kPairEntries][4];
+    __shared__ float hl[kPairEntries * 36];
+    extern __shared__ float4 ctile[];
+    const int e = 2 * (int)(threadIdx.x >> 6) + (int)((threadIdx.x & 63) >> 5);
+    const int b = xcd_remap(blockIdx.x, gridDim.x) * kPairEntries + e;
+    icgn_pair_entry<SPL>(p, b, e, hl + 36 * e, ctile, gbox);
+}
+
+template <int SPL>
+void launch_icgn_pair_t(const IcgnParams& p, hipStream_t s)
+{
+    const size_t lds = (size_t)p.tile_h * kPairStride * sizeof(float4);
+    hipLaunchKernelGGL((k_icgn_pair<SPL>), dim3((p.n + kPairEntries - 1) / kPairEntries),
+                       dim3(64 * kPairEntries / 2), lds, s, p);
+}
+
+// Instantiated pair shapes (rows per band of a 32-lane entry).
+#define FI_ICGN_PAIR(X) X(31)
 
 // Instantiated lane shapes (waves, rows per band).  EXACT_A/B are compiled in
 // k_icgn_xa.hip / k_icgn_xb.hip, GENERIC in k_icgn.hip.

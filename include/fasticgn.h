@@ -210,6 +210,13 @@ int fi_icgn_set_shape(fi_icgn* h, int waves, int rows_per_band);
  * rows instead of weights x pixels); an iteration that leaves it reads the
  * target in device memory.  One-wave lane shapes only; others run as AUTO. */
 #define FI_SAMPLING_GROUP 2
+/* FI_SAMPLING_PAIR: two POIs per wave (lanes 0-31 and 32-63, each with the
+ * band layout of a 32-lane subset), eight neighbouring POIs per block sharing
+ * one coefficient tile as in FI_SAMPLING_GROUP: the per-iteration sums,
+ * solve and warp update serve two POIs.  Uniform subsets whose 32-lane layout
+ * the build instantiates (radius 15); others run as FI_SAMPLING_GROUP.  Results
+ * differ from the other modes only in FP32 summation order. */
+#define FI_SAMPLING_PAIR 3
 int fi_icgn_set_sampling(fi_icgn* h, int mode);
 /* setRefImage / setTarImage.  The IC-GN kernels address the f32 images with
  * 32-bit offsets: a reference whose width * height * 4 bytes exceeds 2 GiB

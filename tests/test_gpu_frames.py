@@ -307,22 +307,25 @@ def test_frame_large_strain_samples_target_without_table():
         assert cmp["gpu_valid"] > 0.5 * cmp["n"]
 
 
-def test_frame_group_sampling_meets_the_gate(seq, targets):
+@pytest.mark.parametrize("mode", ["group", "pair"])
+def test_frame_group_sampling_meets_the_gate(seq, targets, mode):
     """Group sampling (four neighbouring POIs per block sharing a coefficient
     tile) through the mixed-radius driver: radii 10 and 15 run one-wave shapes
-    in group mode, radius 20 runs as 'auto'; the frame meets the parity gate
-    against the default sampling, and graph replay is byte-identical."""
+    in group mode, radius 20 runs as 'auto'; pair sampling runs radius 15 two
+    POIs per wave (the other radii as group sampling).  The frame meets the
+    parity gate against the default sampling, and graph replay is
+    byte-identical."""
     k = STEPS[2]
     with mixed_driver(seq) as d:
         base, _ = d.process(img(targets[k]), policy=0)
     with mixed_driver(seq) as d:
-        d.set_sampling("group")
+        d.set_sampling(mode)
         grp, st = d.process(img(targets[k]), policy=0)
     cmp = compare(base, grp)
-    say("group vs auto", {x: cmp[x] for x in ("n", "gpu_valid", "status_mismatch", "p99_du", "max_du")})
+    say(f"{mode} vs auto", {x: cmp[x] for x in ("n", "gpu_valid", "status_mismatch", "p99_du", "max_du")})
     assert cmp["pass"], cmp
     with mixed_driver(seq, graph=True) as d:
-        d.set_sampling("group")
+        d.set_sampling(mode)
         a, _ = d.process(img(targets[k]), policy=0)
         d.reset_sequence()
         b, _ = d.process(img(targets[k]), policy=0)

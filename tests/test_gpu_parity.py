@@ -150,6 +150,37 @@ def test_icgn_group_sampling_parity(oracle, grid):
     assert finite_valid(g).mean() > 0.9
 
 
+@pytest.mark.parametrize("grid,odd", [(5, True), (10, False)])
+def test_icgn_pair_sampling_parity(oracle, grid, odd):
+    """Two POIs per wave (lanes 0-31 and 32-63), eight neighbouring POIs per
+    block sharing one coefficient tile.  On a 5 px grid a block's
+    neighbourhoods fit the tile except where its eight POIs wrap to the next
+    grid row; on a 10 px grid none fit and every sample reads the target in
+    device memory.  An odd count leaves the last block's last half without a
+    POI.  Parity gate against the CPU oracle, validity flags identical."""
+    sc = synth.affine_scene("pair", 384, 384, 3400, 17, t=(1.4, -1.1), grad=(2e-3, 1e-3, -1e-3, 2e-3),
+                            radius=15, grid=grid)
+    n = len(sc.pois)
+    q = sc.pois[:n if n % 2 == int(odd) else n - 1].copy()
+    assert len(q) % 2 == int(odd)
+    u, v, *_ = sc.truth(q["x"], q["y"])
+    q["deformation"][:, 0] = np.round(u)
+    q["deformation"][:, 6] = np.round(v)
+    g = run_icgn_gpu(sc.ref, sc.tar, 15, q, sampling="pair")
+    o = q.copy()
+    oracle.icgn(sc.ref.astype(np.float32), sc.tar.astype(np.float32), 15, o)
+    cmp = compare(o, g)
+    report(f"icgn pair sampling grid {grid}", cmp)
+    assert cmp["pass"], cmp
+    assert finite_valid(g).mean() > 0.9
+    # the same POIs through the shared-tile kernel: same statuses, the
+    # displacements within summation-order noise
+    grp = run_icgn_gpu(sc.ref, sc.tar, 15, q, sampling="group")
+    both = compare(grp, g)
+    assert both["status_mismatch"] == 0 and max(both["max_du"], both["max_dv"]) < 1e-3, both
+    assert fields_equal(g, run_icgn_gpu(sc.ref, sc.tar, 15, q, sampling="pair"))
+
+
 def test_icgn_center_offset_parity(oracle):
     """compute(queue, center_offset_queue) (src/oc_icgn.cpp:399-625) on the
     device against the CPU oracle's, offsets U[-3, 3] px on an affine field with
@@ -253,9 +284,12 @@ def test_icgn_rejects_images_beyond_the_offset_range():
             d.set_reference(big)
 
 
-def test_icgn_edge_cases(oracle, c0):
+@pytest.mark.parametrize("sampling", ["auto", "pair"])
+def test_icgn_edge_cases(oracle, c0, sampling):
     """OOB band, zero texture (-5), entry gate: statuses identical, and the
-    fields the reference leaves alone are left alone."""
+    fields the reference leaves alone are left alone.  In pair sampling the two
+    POIs of a wave meet different fates (one rejected at the gate or leaving the
+    image while the other iterates on)."""
     ref = c0.ref.copy()
     ref[100:200, 100:200] = 128
     tar = np.roll(ref, (-2, 3), axis=(0, 1))
@@ -266,11 +300,11 @@ def test_icgn_edge_cases(oracle, c0):
     q["result"][::7, 2] = -4.0
     q["deformation"][3::11, 0] = np.nan
     q["result"][5::13, 2] = np.nan
-    g = run_icgn_gpu(ref, tar, 15, q)
+    g = run_icgn_gpu(ref, tar, 15, q, sampling=sampling)
     o = q.copy()
     oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o)
     cmp = compare(o, g)
-    report("icgn edge cases", cmp)
+    report(f"icgn edge cases ({sampling})", cmp)
     assert cmp["status_mismatch"] == 0
     assert cmp["pass"], cmp
     zc = o["result"][:, 2]
@@ -296,12 +330,13 @@ def test_icgn_self_adaptive_mixed_radii(oracle, c0):
     np.testing.assert_array_equal(g["subset_radius"], o["subset_radius"])
 
 
-def test_icgn_iteration_cap_and_conv(oracle, c0):
+@pytest.mark.parametrize("sampling", ["auto", "pair"])
+def test_icgn_iteration_cap_and_conv(oracle, c0, sampling):
     """A tight criterion with a small cap produces -4 the same way."""
     ref, tar = c0.ref, c0.tar
     q = c0.pois[:600].copy()
     oracle.fftcc(ref.astype(np.float32), tar.astype(np.float32), 15, q)
-    g = run_icgn_gpu(ref, tar, 15, q, conv=1e-6, stop=2)
+    g = run_icgn_gpu(ref, tar, 15, q, conv=1e-6, stop=2, sampling=sampling)
     o = q.copy()
     oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o, conv=1e-6, stop=2)
     cmp = compare(o, g)
@@ -329,9 +364,11 @@ def test_icgn_table_sampling_parity(oracle, c0):
     assert both["status_mismatch"] == 0 and max(both["max_du"], both["max_dv"]) < 1e-3
 
 
-def test_icgn_large_strain_uses_table_fallback(oracle):
+@pytest.mark.parametrize("sampling", ["auto", "pair"])
+def test_icgn_large_strain_uses_table_fallback(oracle, sampling):
     """20 % strain: the warped subset outgrows the LDS tile, so iterations fall
-    back to the table; parity holds."""
+    back to the table (pair sampling: to the target in device memory); parity
+    holds."""
     sc = synth.affine_scene("strain", 384, 384, 3400, 7, t=(1.2, -0.7), grad=(0.2, 0.0, 0.0, 0.2),
                             radius=15, grid=16)
     q = sc.pois.copy()
@@ -340,7 +377,7 @@ def test_icgn_large_strain_uses_table_fallback(oracle):
     q["deformation"][:, 6] = np.round(v)
     q["deformation"][:, 1] = 0.2
     q["deformation"][:, 8] = 0.2
-    g = run_icgn_gpu(sc.ref, sc.tar, 15, q)
+    g = run_icgn_gpu(sc.ref, sc.tar, 15, q, sampling=sampling)
     o = q.copy()
     oracle.icgn(sc.ref.astype(np.float32), sc.tar.astype(np.float32), 15, o)
     cmp = compare(o, g)
@@ -349,15 +386,17 @@ def test_icgn_large_strain_uses_table_fallback(oracle):
     assert finite_valid(g).mean() > 0.5
 
 
-def test_icgn_far_seed_restages_tile(oracle, c0):
+@pytest.mark.parametrize("sampling", ["auto", "pair"])
+def test_icgn_far_seed_restages_tile(oracle, c0, sampling):
     """Seeds 2.4 px from the solution move the subset past the tile's margin
-    during the iterations, so the tile is staged again; parity holds."""
+    during the iterations, so the tile is staged again (pair sampling: the
+    iteration reads the target in device memory); parity holds."""
     ref, tar = c0.ref, c0.tar
     q = c0.pois.copy()
     u, v, *_ = c0.truth(q["x"], q["y"])
     q["deformation"][:, 0] = u + 2.4
     q["deformation"][:, 6] = v - 2.4
-    g = run_icgn_gpu(ref, tar, 15, q, stop=20)
+    g = run_icgn_gpu(ref, tar, 15, q, stop=20, sampling=sampling)
     o = q.copy()
     oracle.icgn(ref.astype(np.float32), tar.astype(np.float32), 15, o, stop=20)
     cmp = compare(o, g)
