@@ -46,6 +46,7 @@ METRIC = "converged POIs/sec (31x31 subset), learned vs FFTCC seed at p99 <=0.01
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_SAMPLE = 76  # 64 B B-spline texel + 4 B reference + 8 B gradients
+SEED_CONSENSUS_TAU = 0.5  # the frame driver's default (fi_frame_set_seed_consensus)
 FLOP_PER_SAMPLE = 75   # warp 8, fractions 8, bicubic 40, mean/norm/error 7, SD 4, numerator 12
 
 
@@ -546,10 +547,11 @@ def main() -> int:
             # threshold): iteration counts comparable, unlike against the
             # FFTCC-seeded oracle above
             from tests.policy import PolicyOracle
-            seeds = sc.pois[pick].copy()
-            ok = seeder.seed(ref32, tar32, seeds)
+            # (every POI is seeded: the neighbourhood consensus needs the neighbours)
+            seeds = sc.pois.copy()
+            ok = seeder.seed(ref32, tar32, seeds, consensus=(float(sc.radius), SEED_CONSENSUS_TAU))
             po = PolicyOracle(pyoracle, sc.ref, sc.pois[pick], sc.radius)
-            o_same, _ = po.learned_frame(sc.tar, seeds, ok, args.seed_zncc)
+            o_same, _ = po.learned_frame(sc.tar, seeds[pick], ok[pick], args.seed_zncc)
             par = compare(o_same, runs["learned"]["res"][pick])
             parity["learned_same_seeds"] = {k: par.get(k) for k in keys}
             log(f"[bench] parity learned, oracle from the same seeds: {parity['learned_same_seeds']}")

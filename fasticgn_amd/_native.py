@@ -185,6 +185,7 @@ SIGNATURES = {
     "fi_frame_set_sampling": [_P, _I],
     "fi_frame_set_seeder": [_P, _P, _F],
     "fi_frame_set_seed_passes": [_P, _I],
+    "fi_frame_set_seed_consensus": [_P, _I, _F],
     "fi_frame_set_self_adaptive": [_P, _I],
     "fi_frame_set_graph": [_P, _I],
     "fi_frame_set_persistent": [_P, _I],
@@ -205,6 +206,7 @@ SIGNATURES = {
     "fi_seeder_forward": [_P, _P, _S, _P],
     "fi_seeder_gradient": [_P, _P, _P, _S, _P, ctypes.POINTER(_F)],
     "fi_seeder_seed": [_P, _P, _P, _I, _I, _P, _S, _P],
+    "fi_seeder_seed_consensus": [_P, _P, _P, _I, _I, _P, _S, _P, _F, _F, _I],
     "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
     "fi_write_raw": [_P, _S, ctypes.c_char_p, ctypes.POINTER(_S)],
 }

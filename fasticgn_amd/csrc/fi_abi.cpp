@@ -445,6 +445,18 @@ int fi_frame_set_seed_passes(fi_frame* h, int passes)
     });
 }
 
+int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau)
+{
+    return guarded([&] {
+        need(h);
+        if (!(tau > 0.f)) fail(FI_E_INVALID_ARGUMENT, "seed consensus tau must be > 0");
+        h->consensus = on != 0;
+        h->consensus_tau = tau;
+        h->cells.view.tau = tau;
+        h->invalidate_graphs();
+    });
+}
+
 int fi_frame_set_graph(fi_frame* h, int on)
 {
     return guarded([&] {
@@ -544,6 +556,17 @@ int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, 
                    uint32_t* ok)
 {
     return guarded([&] { need(h); h->seed(ref, tar, width, height, pois, n, ok); });
+}
+
+int fi_seeder_seed_consensus(fi_seeder* h, const float* ref, const float* tar, int width, int height,
+                             fi_poi* pois, size_t n, uint32_t* ok, float radius, float tau, int passes)
+{
+    return guarded([&] {
+        need(h);
+        if (!(radius > 0.f) || !(tau > 0.f)) fail(FI_E_INVALID_ARGUMENT, "seed consensus radius and tau must be > 0");
+        if (passes < 1 || passes > 4) fail(FI_E_INVALID_ARGUMENT, "seeder passes must be 1..4");
+        h->seed(ref, tar, width, height, pois, n, ok, radius, tau, passes);
+    });
 }
 
 // ---- raw record, opencorr_cli_cuda_mvp.cpp:2498-2536 ----

@@ -778,6 +778,7 @@ void fi_frame::prepare()
 {
     if (!ref_set) fail(FI_E_LOGIC, "frame driver needs a reference image");
     if (!pois_set) fail(FI_E_LOGIC, "frame driver needs a POI template");
+    cells_ready = false;  // the seed-consensus cells follow the layout
     const int w = ref_raw.view.width, h = ref_raw.view.height;
     const size_t px = (size_t)w * h;
 
@@ -1180,6 +1181,18 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     // records: the slot is chosen (and its previous copy waited for) before
     // the segment, packed at the segment's end and copied after it
     const int rs = records_on ? records_slot() : -1;
+    if (mode == 0 && consensus && !cells_ready) {
+        // seed-consensus cells over the layout's positions, once per layout
+        std::vector<float> xs(n), ys(n);
+        for (size_t i = 0; i < n; i++) {
+            const fi_poi& q = tmpl_host.p[order.empty() ? i : order[i]];
+            xs[i] = q.x;
+            ys[i] = q.y;
+        }
+        cells.upload(build_seed_cells(xs.data(), ys.data(), n, (float)r), (float)r, consensus_tau, stream);
+        seed_raw.reserve(2 * std::max<size_t>(n, 1));
+        cells_ready = true;
+    }
 
     auto segment = [&](bool timed) {
         // a fill kernel, not memsets: this runs inside captured frame graphs
@@ -1205,8 +1218,11 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             // POI a later pass cannot crop keeps the earlier seed, and only the
             // first pass's flag decides the FFTCC fallback for an unseeded POI
             seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, seed_ok.p, stream);
-            for (int pass = 1; pass < seed_passes; pass++)
+            if (consensus) launch_seed_consensus(cur.p, seed_ok.p, seed_raw.p, cnt, cells.view, stream);
+            for (int pass = 1; pass < seed_passes; pass++) {
                 seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, nullptr, stream);
+                if (consensus) launch_seed_consensus(cur.p, seed_ok.p, seed_raw.p, cnt, cells.view, stream);
+            }
         } else if (mode == 1) {
             fftcc_all(cur.p);
         } else {

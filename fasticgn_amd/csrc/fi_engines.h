@@ -126,6 +126,15 @@ struct fi_frame {
     fi_seeder* seeder = nullptr;   // learned seeder (FI_POLICY_LEARNED_SEED)
     float seed_threshold = 0.8f;   // ZNCC below which a learned seed falls back to FFTCC
     int seed_passes = 1;           // seeder passes, each re-centred on the previous one's rounded u, v
+    // Neighbourhood consensus of the seeder's predictions after each pass
+    // (launch_seed_consensus) within +-r, neighbours agreeing within
+    // consensus_tau px; its cells are built on the host the first time a
+    // learned frame runs on a layout.
+    bool consensus = true;
+    float consensus_tau = 0.5f;
+    bool cells_ready = false;
+    fi::DevSeedCells cells;
+    fi::DevArray<float4> seed_raw;
     hipStream_t stream = nullptr;
     fi::DeviceImage ref_raw;
     // Targets: two slots filled by a staging thread (host copy into pinned

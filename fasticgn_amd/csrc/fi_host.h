@@ -199,6 +199,29 @@ void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* id
 // executed iteration counts)
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s, const uint32_t* iters = nullptr,
                size_t n_iters = 0);
+// Neighbourhood consensus of learned seeds (see k_policy.hip): raw (2 float4
+// per POI) receives the seeds as they stand, then each seeded POI (ok) takes
+// the mean of its seeded neighbours' predictions transported to it, over the
+// neighbours that agree within c.tau.
+void launch_seed_consensus(fi_poi* pois, const uint32_t* ok, float4* raw, int n, const SeedCells& c,
+                           hipStream_t s);
+// The cells of launch_seed_consensus on the host: side max(radius / 3,
+// sqrt(area / m)) over the bounding box of the m finite positions (FP64),
+// searched ceil(radius / side) cells to each side; POIs of a cell in position
+// order; a position with a non-finite coordinate has cell -1.
+struct HostSeedCells {
+    std::vector<uint32_t> start, idx;
+    std::vector<int32_t> cell;
+    int ncx = 0, ncy = 0, k = 1;
+};
+HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float radius);
+// Device copy of a HostSeedCells (uploaded on s, which is synchronised).
+struct DevSeedCells {
+    DevArray<uint32_t> start, idx;
+    DevArray<int32_t> cell;
+    SeedCells view{};
+    void upload(const HostSeedCells& hc, float radius, float tau, hipStream_t s);
+};
 // learned-seed fallback candidates (see k_policy.hip)
 void launch_seed_scan(const fi_poi* cur, const uint32_t* ok, int n, float threshold, FrameCounters* counters,
                       uint32_t* candidates, hipStream_t s);

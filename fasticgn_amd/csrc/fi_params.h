@@ -106,4 +106,18 @@ struct IcgnParams {
                               // squared norm is >= conv2, i.e. (float)sqrt(norm^2) >= conv
 };
 
+// Neighbourhood consensus of learned seeds: POIs binned into square cells
+// (row-major, ncx x ncy) of side >= radius / k, so the POIs within +-radius of
+// a POI lie in the (2k+1) x (2k+1) cells around its own.
+struct SeedCells {
+    const uint32_t* start;  // ncx * ncy + 1 offsets into idx
+    const uint32_t* idx;    // POI positions, grouped by cell
+    const int32_t* cell;    // per POI position: its cell, -1 = none (non-finite position)
+    int ncx, ncy;
+    int k;                  // cells searched on each side
+    int m;                  // length of idx (the POIs with a cell)
+    float radius;           // neighbours within +-radius in x and in y
+    float tau;              // largest |u, v| disagreement of a neighbour's transported seed
+};
+
 }  // namespace fi

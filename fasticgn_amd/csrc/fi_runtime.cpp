@@ -1,4 +1,5 @@
 // Host runtime: last-error storage, image upload, DFT twiddles.
+#include <algorithm>
 #include <cstring>
 #include <cmath>
 #include <string>
@@ -291,6 +292,76 @@ void dft_twiddles(int n, float* out)
         out[2 * k] = (float)std::cos(a);
         out[2 * k + 1] = (float)(-std::sin(a));
     }
+}
+
+HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float radius)
+{
+    HostSeedCells hc;
+    hc.cell.assign(n, -1);
+    double x0 = HUGE_VAL, x1 = -HUGE_VAL, y0 = HUGE_VAL, y1 = -HUGE_VAL;
+    size_t m = 0;
+    for (size_t i = 0; i < n; i++) {
+        const double x = xs[i], y = ys[i];
+        if (!std::isfinite(x) || !std::isfinite(y)) continue;
+        x0 = std::min(x0, x);
+        x1 = std::max(x1, x);
+        y0 = std::min(y0, y);
+        y1 = std::max(y1, y);
+        m++;
+    }
+    if (m == 0) {
+        hc.start.assign(1, 0);
+        return hc;
+    }
+    // side >= radius / 3: the neighbours within +-radius lie within
+    // k = ceil(radius / side) <= 3 cells of a POI's own (7 x 7 cells, a close
+    // fit to the +-radius window); >= sqrt(area / m): about one POI per cell
+    // at most
+    const double side = std::max((double)radius / 3.0, std::sqrt((x1 - x0 + 1.0) * (y1 - y0 + 1.0) / (double)m));
+    hc.k = std::max(1, (int)std::ceil((double)radius / side));
+    const double nx = std::floor((x1 - x0) / side) + 1.0, ny = std::floor((y1 - y0) / side) + 1.0;
+    if (nx * ny > 4.0 * (double)m + 65536.0)
+        fail(FI_E_UNSUPPORTED, "seed consensus: POI layout too sparse for its cell grid");
+    hc.ncx = (int)nx;
+    hc.ncy = (int)ny;
+    const size_t cells = (size_t)hc.ncx * hc.ncy;
+    hc.start.assign(cells + 1, 0);
+    for (size_t i = 0; i < n; i++) {
+        const double x = xs[i], y = ys[i];
+        if (!std::isfinite(x) || !std::isfinite(y)) continue;
+        const int cx = std::min((int)std::floor((x - x0) / side), hc.ncx - 1);
+        const int cy = std::min((int)std::floor((y - y0) / side), hc.ncy - 1);
+        hc.cell[i] = cy * hc.ncx + cx;
+        hc.start[(size_t)hc.cell[i] + 1]++;
+    }
+    for (size_t c = 0; c < cells; c++) hc.start[c + 1] += hc.start[c];
+    hc.idx.resize(m);
+    std::vector<uint32_t> fill(hc.start.begin(), hc.start.end() - 1);
+    for (size_t i = 0; i < n; i++)
+        if (hc.cell[i] >= 0) hc.idx[fill[(size_t)hc.cell[i]]++] = (uint32_t)i;
+    return hc;
+}
+
+void DevSeedCells::upload(const HostSeedCells& hc, float radius, float tau, hipStream_t s)
+{
+    start.reserve(hc.start.size());
+    idx.reserve(std::max<size_t>(hc.idx.size(), 1));
+    cell.reserve(std::max<size_t>(hc.cell.size(), 1));
+    FI_HIP(hipMemcpyAsync(start.p, hc.start.data(), hc.start.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!hc.idx.empty())
+        FI_HIP(hipMemcpyAsync(idx.p, hc.idx.data(), hc.idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    if (!hc.cell.empty())
+        FI_HIP(hipMemcpyAsync(cell.p, hc.cell.data(), hc.cell.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    FI_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope with the caller
+    view.start = start.p;
+    view.idx = idx.p;
+    view.cell = cell.p;
+    view.ncx = hc.ncx;
+    view.ncy = hc.ncy;
+    view.k = hc.k;
+    view.m = (int)hc.idx.size();
+    view.radius = radius;
+    view.tau = tau;
 }
 
 }  // namespace fi

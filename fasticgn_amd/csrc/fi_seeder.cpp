@@ -304,7 +304,8 @@ void fi_seeder::forward(const uint16_t* x1, size_t n, float* out)
     }
 }
 
-void fi_seeder::seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok)
+void fi_seeder::seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok,
+                     float radius, float tau, int passes)
 {
     if (n == 0) return;
     if (!ref || !tar || !pois) fail(FI_E_INVALID_ARGUMENT, "null buffer");
@@ -323,6 +324,23 @@ void fi_seeder::seed(const float* ref, const float* tar, int w, int h, fi_poi* p
     FI_HIP(hipMemcpyAsync(d_tar.p, tar, px * sizeof(float), hipMemcpyHostToDevice, stream));
     FI_HIP(hipMemcpyAsync(d_pois.p, pois, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
     run_seed(d_ref.p, d_tar.p, w, h, d_pois.p, nullptr, (int)n, d_ok.p, stream);
+    DevSeedCells cells;
+    DevArray<float4> raw;
+    if (radius > 0.f) {
+        std::vector<float> xs(n), ys(n);
+        for (size_t i = 0; i < n; i++) {
+            xs[i] = pois[i].x;
+            ys[i] = pois[i].y;
+        }
+        cells.upload(build_seed_cells(xs.data(), ys.data(), n, radius), radius, tau, stream);
+        raw.reserve(2 * n);
+        launch_seed_consensus(d_pois.p, d_ok.p, raw.p, (int)n, cells.view, stream);
+        for (int pass = 1; pass < passes; pass++) {
+            run_seed(d_ref.p, d_tar.p, w, h, d_pois.p, nullptr, (int)n, nullptr, stream);
+            launch_seed_consensus(d_pois.p, d_ok.p, raw.p, (int)n, cells.view, stream);
+        }
+        FI_HIP(hipGetLastError());
+    }
     FI_HIP(hipMemcpyAsync(pois, d_pois.p, n * sizeof(fi_poi), hipMemcpyDeviceToHost, stream));
     std::vector<uint32_t> h_ok(n);
     FI_HIP(hipMemcpyAsync(h_ok.data(), d_ok.p, n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));

@@ -33,7 +33,13 @@ struct fi_seeder {
     void forward(const uint16_t* x1, size_t n, float* out);
     void gradient(const uint16_t* x1, const float* targets, size_t n, float* g, float* loss);
     void loss_and_grad(int bsz, float* loss_parts_out);
-    void seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok);
+    // radius > 0: the neighbourhood consensus of the predictions follows
+    // (launch_seed_consensus, neighbours within +-radius agreeing within tau);
+    // passes > 1: further passes, each cropping the target at the previous
+    // seeds and followed by the consensus over the first pass's ok, as the
+    // frame driver runs them
+    void seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok,
+              float radius = 0.f, float tau = 0.f, int passes = 1);
     // device-side seeding for the frame driver (stream-ordered, no sync)
     void run_seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, const uint32_t* index, int n,
                   uint32_t* ok, hipStream_t s) const;

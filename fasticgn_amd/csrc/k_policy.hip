@@ -6,6 +6,7 @@
 // they run where the POIs already live so a frame needs no host scan.
 #include <hip/hip_runtime.h>
 
+#include "fi_device.h"
 #include "fi_host.h"
 
 namespace fi {
@@ -473,5 +474,99 @@ void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s,
                        dim3(kPolicyThreads), 0, s, cur, n, counters, iters, n_iters);
 }
 
+
+// ---------------------------------------------------------------------------
+// Neighbourhood consensus of learned seeds.  The network predicts each POI's
+// (u, v) and gradients from its own crops, and its errors differ from POI to
+// POI, while the displacement field varies little over a subset's footprint
+// (the first-order warp IC-GN fits assumes as much).  Each seeded POI takes
+// the mean of its seeded neighbours' predictions within +-radius, each
+// neighbour's (u, v) transported to the POI with that neighbour's gradients,
+// u_j - (ux_j dx + uy_j dy) with dx = x_j - x_i, over the neighbours whose
+// transported (u, v) agree with the POI's own prediction within tau (a
+// neighbour across a discontinuity does not pull the seed).  The mean
+// gradients then carry the mean (u, v) from the neighbours' centroid to the
+// POI, so a linear field is reproduced whatever the neighbourhood's shape.
+// Plain FP32 in a fixed order (cells row by row, a cell's POIs in position
+// order), no fused multiply-adds: a host restatement reproduces it exactly.
+// ---------------------------------------------------------------------------
+// raw[k] = the prediction of POI c.idx[k] (positions grouped by cell), with x
+// at 3e38 (and the rest zero) when the POI is not seeded or a value of its
+// prediction is not finite: every neighbour test rejects it and every term it
+// adds (w = 0 times a finite value) is an exact zero
+__global__ void k_seed_raw(const fi_poi* __restrict__ pois, const uint32_t* __restrict__ ok, int m,
+                           SeedCells c, float4* __restrict__ raw)
+{
+    // XCD-aware: each XCD packs a contiguous run of cells, the run its own
+    // consensus blocks read (k_seed_consensus), so the reads hit its L2
+    const int k = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t i = c.idx[k];
+    const fi_poi& q = pois[i];
+    const float* d = q.deformation;
+    const bool use = ok[i] && finite_f(d[0]) && finite_f(d[6]) && finite_f(d[1]) && finite_f(d[2]) &&
+                     finite_f(d[7]) && finite_f(d[8]);
+    raw[2 * k] = use ? make_float4(q.x, q.y, d[0], d[6]) : make_float4(3e38f, 0.f, 0.f, 0.f);
+    raw[2 * k + 1] = use ? make_float4(d[1], d[2], d[7], d[8]) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// One thread per POI: its (2K+1)^2 cells are 2K+1 runs of raw (a cell row's
+// cells are consecutive), read without indirection.  An entry adds w * value
+// with w = 1 when it passes the tests, else 0: a rejected entry adds +-0,
+// which leaves each sum's bits as they are (the sums start at +0 and never
+// become -0), the loads of a run are independent and each sum is one FMA.
+__global__ void k_seed_consensus(fi_poi* __restrict__ pois, const uint32_t* __restrict__ ok,
+                                 const float4* __restrict__ raw, int n, SeedCells c)
+{
+    const int i = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n || !ok[i]) return;
+    const int ci = c.cell[i];
+    if (ci < 0) return;
+    fi_poi& q = pois[i];
+    const float ax = q.x, ay = q.y, au = q.deformation[0], av = q.deformation[6];
+    const int cx = ci % c.ncx, cy = ci / c.ncx;
+    const int xa = max(cx - c.k, 0), xb = min(cx + c.k, c.ncx - 1);
+    float cnt = 0.f, su = 0.f, sv = 0.f, sdx = 0.f, sdy = 0.f, sux = 0.f, suy = 0.f, svx = 0.f, svy = 0.f;
+    for (int yy = max(cy - c.k, 0); yy <= min(cy + c.k, c.ncy - 1); yy++) {
+        const uint32_t k0 = c.start[yy * c.ncx + xa], k1 = c.start[yy * c.ncx + xb + 1];
+#pragma unroll 8
+        for (uint32_t k = k0; k < k1; k++) {
+            const float4 b = raw[2 * k], g = raw[2 * k + 1];  // x, y, u, v; ux, uy, vx, vy
+            const float dx = b.x - ax, dy = b.y - ay;
+            const float tu = b.z - (g.x * dx + g.y * dy);
+            const float tv = b.w - (g.z * dx + g.w * dy);
+            const float w = (fabsf(dx) <= c.radius && fabsf(dy) <= c.radius && fabsf(tu - au) <= c.tau &&
+                             fabsf(tv - av) <= c.tau) ? 1.f : 0.f;
+            cnt += w;
+            su = __builtin_fmaf(w, b.z, su);
+            sv = __builtin_fmaf(w, b.w, sv);
+            sdx = __builtin_fmaf(w, dx, sdx);
+            sdy = __builtin_fmaf(w, dy, sdy);
+            sux = __builtin_fmaf(w, g.x, sux);
+            suy = __builtin_fmaf(w, g.y, suy);
+            svx = __builtin_fmaf(w, g.z, svx);
+            svy = __builtin_fmaf(w, g.w, svy);
+        }
+    }
+    if (cnt == 0.f) return;  // the POI's own prediction is not finite
+    const float inv = 1.f / cnt;
+    const float ux = sux * inv, uy = suy * inv, vx = svx * inv, vy = svy * inv;
+    const float mdx = sdx * inv, mdy = sdy * inv;
+    q.deformation[0] = su * inv - (ux * mdx + uy * mdy);
+    q.deformation[6] = sv * inv - (vx * mdx + vy * mdy);
+    q.deformation[1] = ux;
+    q.deformation[2] = uy;
+    q.deformation[7] = vx;
+    q.deformation[8] = vy;
+}
+
+void launch_seed_consensus(fi_poi* pois, const uint32_t* ok, float4* raw, int n, const SeedCells& c,
+                           hipStream_t s)
+{
+    if (n <= 0) return;
+    // every position with a cell is in idx once: its length is at most n
+    hipLaunchKernelGGL(k_seed_raw, grid_for(c.m), dim3(kPolicyThreads), 0, s, pois, ok, c.m, c, raw);
+    hipLaunchKernelGGL(k_seed_consensus, grid_for(n), dim3(kPolicyThreads), 0, s, pois, ok, raw, n, c);
+}
 
 }  // namespace fi

@@ -124,6 +124,13 @@ class FrameDriver(_Handle):
         previous pass's rounded u, v."""
         nat.call("fi_frame_set_seed_passes", self._h, int(passes))
 
+    def set_seed_consensus(self, on: bool = True, tau: float = 0.5) -> None:
+        """Neighbourhood consensus of the seeder's predictions after each pass
+        (default on, tau 0.5 px): each seeded POI takes the mean of its seeded
+        neighbours' predictions within +-radius, carried to it with their
+        gradients, over the neighbours that agree within tau px."""
+        nat.call("fi_frame_set_seed_consensus", self._h, int(bool(on)), float(tau))
+
     def set_self_adaptive(self, on: bool = True) -> None:
         """Mixed subset radii: every template POI's (square) subset_radius is
         its FFTCC and IC-GN radius (DIC::setSelfAdaptive); call before

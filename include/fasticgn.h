@@ -426,6 +426,17 @@ int fi_seeder_forward(fi_seeder* h, const uint16_t* x1, size_t n, float* out);
  * where both crops lie inside the images (others untouched).  ok may be NULL. */
 int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, int height, fi_poi* pois,
                    size_t n, uint32_t* ok);
+/* fi_seeder_seed, then the neighbourhood consensus of the predictions: each
+ * seeded POI takes the mean of its seeded neighbours' predictions within
+ * +-radius px in x and y, each neighbour's u, v carried to the POI with that
+ * neighbour's predicted gradients, over the neighbours whose carried u, v
+ * agree with the POI's own within tau px; the mean gradients carry the mean
+ * u, v from the neighbours' centroid to the POI.  passes (1..4): further seeder
+ * passes as in fi_frame_set_seed_passes, each followed by the consensus over
+ * the first pass's ok (the frame driver's learned seeding, run alone).  No
+ * reference counterpart. */
+int fi_seeder_seed_consensus(fi_seeder* h, const float* ref, const float* tar, int width, int height,
+                             fi_poi* pois, size_t n, uint32_t* ok, float radius, float tau, int passes);
 
 /* Frame driver with the learned seeder (policy FI_POLICY_LEARNED_SEED): the
  * FFTCC seeding of a frame is replaced by the seeder; POIs whose IC-GN result
@@ -437,6 +448,13 @@ int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold);
  * rounded u, v of pass k-1, so a POI that starts far from its displacement
  * (the seeder's +-8 px range) is seeded from a near crop by the second pass. */
 int fi_frame_set_seed_passes(fi_frame* h, int passes);
+/* Neighbourhood consensus of the seeder's predictions after each pass (as
+ * fi_seeder_seed_consensus, radius = the driver's subset radius; default on,
+ * tau 0.5 px).  The seeds of a smooth field then lose most of the network's
+ * per-POI error, and IC-GN needs fewer iterations; a neighbour across a
+ * discontinuity disagrees by more than tau and is left out.  No reference
+ * counterpart. */
+int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau);
 
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */

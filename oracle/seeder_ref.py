@@ -10,6 +10,13 @@ pinned by the tests against closed forms (linearity of the head, identity
 layers, the patch permutation applied by hand) and is the checker of the
 device kernels, not part of the product.
 
+consensus restates the neighbourhood consensus of the seeds
+(launch_seed_consensus in fasticgn_amd/csrc/k_policy.hip): the same cells and
+the same FP32 operations in the same order, so the device's seeds are
+reproduced bit for bit; it is pinned by closed forms in
+tests/test_seed_consensus.py (a linear field carried exactly over any
+neighbourhood, the plain window mean of a symmetric one, a step excluded).
+
 loss_and_grad restates one training step's loss and parameter gradient with
 the device's rounding points (BF16 inputs of every GEMM, FP32 elsewhere); with
 bf16=False it is the plain float64 backward pass, pinned by finite
@@ -122,3 +129,81 @@ def loss_and_grad(params: np.ndarray, x1_bits: np.ndarray, targets: np.ndarray, 
         grads += [gw.ravel(), gb]
     grads += [g5w.ravel(), g5b]
     return loss, np.concatenate(grads).astype(dt)
+
+
+def seed_cells(x: np.ndarray, y: np.ndarray, radius: float):
+    """Cells of the consensus (build_seed_cells in fasticgn_amd/csrc/fi_runtime.cpp):
+    side max(radius / 3, sqrt(area / m)) over the finite positions' bounding
+    box in FP64, searched k = ceil(radius / side) cells to each side; returns
+    (cell per position or -1, ncx, ncy, k, lists of positions per cell in
+    position order)."""
+    x = np.asarray(x, dtype=np.float32).astype(np.float64)
+    y = np.asarray(y, dtype=np.float32).astype(np.float64)
+    fin = np.isfinite(x) & np.isfinite(y)
+    cell = np.full(len(x), -1, dtype=np.int64)
+    m = int(fin.sum())
+    if m == 0:
+        return cell, 0, 0, 1, []
+    x0, x1, y0, y1 = x[fin].min(), x[fin].max(), y[fin].min(), y[fin].max()
+    r = float(np.float32(radius))
+    side = max(r / 3.0, float(np.sqrt((x1 - x0 + 1.0) * (y1 - y0 + 1.0) / m)))
+    k = max(1, int(np.ceil(r / side)))
+    ncx = int(np.floor((x1 - x0) / side)) + 1
+    ncy = int(np.floor((y1 - y0) / side)) + 1
+    cx = np.minimum(np.floor((x[fin] - x0) / side).astype(np.int64), ncx - 1)
+    cy = np.minimum(np.floor((y[fin] - y0) / side).astype(np.int64), ncy - 1)
+    cell[fin] = cy * ncx + cx
+    members = [[] for _ in range(ncx * ncy)]
+    for i in np.nonzero(fin)[0]:
+        members[cell[i]].append(int(i))
+    return cell, ncx, ncy, k, [np.array(mb, dtype=np.int64) for mb in members]
+
+
+def consensus(x: np.ndarray, y: np.ndarray, seeds: np.ndarray, ok: np.ndarray, radius: float, tau: float):
+    """Consensus seeds (n x 6: u, v, ux, uy, vx, vy) from the network's seeds:
+    each POI with ok takes the mean of its ok neighbours within +-radius (x and
+    y) whose predictions are finite, each neighbour's u, v carried to it with
+    that neighbour's gradients,
+    over the neighbours whose carried u, v agree with its own within tau; the
+    mean gradients carry the mean u, v from the neighbours' centroid to it.
+    FP32, sums in cell order (row by row around the POI's cell), then position
+    order."""
+    f = np.float32
+    x = np.asarray(x, dtype=f)
+    y = np.asarray(y, dtype=f)
+    s = np.asarray(seeds, dtype=f)
+    ok = np.asarray(ok, dtype=bool)
+    out = s.copy()
+    cell, ncx, ncy, K, members = seed_cells(x, y, radius)
+    R, T = f(radius), f(tau)
+    for i in range(len(x)):
+        if not ok[i] or cell[i] < 0:
+            continue
+        cx, cy = int(cell[i] % ncx), int(cell[i] // ncx)
+        cand = [members[yy * ncx + xx] for yy in range(max(cy - K, 0), min(cy + K, ncy - 1) + 1)
+                for xx in range(max(cx - K, 0), min(cx + K, ncx - 1) + 1)]
+        j = np.concatenate(cand) if cand else np.zeros(0, dtype=np.int64)
+        j = j[ok[j] & np.all(np.isfinite(s[j]), axis=1)]
+        dx = (x[j] - x[i]).astype(f)
+        dy = (y[j] - y[i]).astype(f)
+        with np.errstate(invalid="ignore"):
+            keep = (np.abs(dx) <= R) & (np.abs(dy) <= R)
+            j, dx, dy = j[keep], dx[keep], dy[keep]
+            sj = s[j]
+            tu = (sj[:, 0] - ((sj[:, 2] * dx).astype(f) + (sj[:, 3] * dy).astype(f)).astype(f)).astype(f)
+            tv = (sj[:, 1] - ((sj[:, 4] * dx).astype(f) + (sj[:, 5] * dy).astype(f)).astype(f)).astype(f)
+            keep = (np.abs((tu - s[i, 0]).astype(f)) <= T) & (np.abs((tv - s[i, 1]).astype(f)) <= T)
+        sj, dx, dy = sj[keep], dx[keep], dy[keep]
+        if len(sj) == 0:
+            continue
+        # sequential FP32 sums (cumsum is a running sum in the array's type)
+        tot = lambda a: np.cumsum(a.astype(f), dtype=f)[-1]
+        cnt = f(len(sj))
+        inv = f(f(1.0) / cnt)
+        su, sv, sdx, sdy = tot(sj[:, 0]), tot(sj[:, 1]), tot(dx), tot(dy)
+        g = [f(tot(sj[:, k]) * inv) for k in (2, 3, 4, 5)]
+        mdx, mdy = f(sdx * inv), f(sdy * inv)
+        out[i, 0] = f(f(su * inv) - f(f(g[0] * mdx) + f(g[1] * mdy)))
+        out[i, 1] = f(f(sv * inv) - f(f(g[2] * mdx) + f(g[3] * mdy)))
+        out[i, 2:] = g
+    return out

@@ -127,11 +127,12 @@ def test_config4_crack_field_parity(oracle):
         say("  near the crack only", {x: cn.get(x) for x in GATE_KEYS})
         assert cmp["pass"] and cn["pass"], (f64, cmp, cn)
     # learned frame: the oracle's learned frame from the seeder's seeds
-    seeds = sc.pois[pick].copy()
-    ok = seeder.seed(ref32, tar32, seeds)
-    seeder.seed(ref32, tar32, seeds)  # second pass: crops at the first pass's rounded u, v
+    # the driver's seeding: two passes (the second crops at the first's rounded
+    # u, v), each followed by the neighbourhood consensus over every POI
+    seeds = sc.pois.copy()
+    ok = seeder.seed(ref32, tar32, seeds, consensus=(float(R), 0.5), passes=2)
     po = PolicyOracle(oracle, sc.ref, sc.pois[pick], R)
-    o, ost = po.learned_frame(sc.tar, seeds, ok, 0.8)
+    o, ost = po.learned_frame(sc.tar, seeds[pick], ok[pick], 0.8)
     cmp = compare(o, res_l[pick], max_iteration_mismatch=0.01)
     say("config4 learned frame vs FP32 oracle", {x: cmp.get(x) for x in GATE_KEYS},
         "fallback on the sample: oracle", ost["seed_fallback_count"], "device", int(redo[pick].sum()))
