@@ -46,7 +46,8 @@ METRIC = "converged POIs/sec (31x31 subset), learned vs FFTCC seed at p99 <=0.01
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_SAMPLE = 76  # 64 B B-spline texel + 4 B reference + 8 B gradients
-SEED_CONSENSUS_TAU = 0.5  # the frame driver's default (fi_frame_set_seed_consensus)
+SEED_CONSENSUS_TAU = 0.5  # the frame driver's defaults (fi_frame_set_seed_consensus)
+SEED_SAMPLE = "checker"
 FLOP_PER_SAMPLE = 75   # warp 8, fractions 8, bicubic 40, mean/norm/error 7, SD 4, numerator 12
 
 
@@ -329,7 +330,7 @@ def bench_config4(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--cpu-sample-every", type=int, default=4)
@@ -341,6 +342,9 @@ def main() -> int:
                          "shared by four neighbouring POIs)")
     ap.add_argument("--seeder", default="auto",
                     help="learned seeder weights (.npy), 'auto' = the shipped weights, 'off' = FFTCC path only")
+    ap.add_argument("--seed-sample", default=None, choices=["all", "checker", "quarter"],
+                    help="POIs the seeder's network runs on under the neighbourhood consensus (the others take "
+                         "their neighbours' consensus); default: the frame driver's")
     ap.add_argument("--seed-zncc", type=float, default=0.8,
                     help="ZNCC below which a learned-seed POI is redone from FFTCC")
     ap.add_argument("--workload", default="config1", choices=["config1", "config2", "config4"],
@@ -503,6 +507,8 @@ def main() -> int:
         if wpath.exists():
             seeder = sd.LearnedSeeder(np.load(wpath))
             drv.set_seeder(seeder, args.seed_zncc)
+            if args.seed_sample:
+                drv.set_seed_consensus(True, SEED_CONSENSUS_TAU, args.seed_sample)
             runs["learned"] = timed(fa.driver.LEARNED_SEED, "learned seed")
         else:
             log(f"[bench] no seeder weights at {wpath}: FFTCC path only")
@@ -549,7 +555,8 @@ def main() -> int:
             from tests.policy import PolicyOracle
             # (every POI is seeded: the neighbourhood consensus needs the neighbours)
             seeds = sc.pois.copy()
-            ok = seeder.seed(ref32, tar32, seeds, consensus=(float(sc.radius), SEED_CONSENSUS_TAU))
+            ok = seeder.seed(ref32, tar32, seeds, consensus=(float(sc.radius), SEED_CONSENSUS_TAU),
+                             sample=args.seed_sample or SEED_SAMPLE)
             po = PolicyOracle(pyoracle, sc.ref, sc.pois[pick], sc.radius)
             o_same, _ = po.learned_frame(sc.tar, seeds[pick], ok[pick], args.seed_zncc)
             par = compare(o_same, runs["learned"]["res"][pick])

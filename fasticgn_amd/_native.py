@@ -29,6 +29,8 @@ FI_POLICY_SKIP_FFTCC = 1
 FI_POLICY_POI_FALLBACK = 2
 FI_POLICY_DEFAULT = FI_POLICY_SKIP_FFTCC | FI_POLICY_POI_FALLBACK
 FI_SAMPLING_AUTO, FI_SAMPLING_TABLE, FI_SAMPLING_GROUP, FI_SAMPLING_PAIR = 0, 1, 2, 3
+# FI_SEED_SAMPLE_*: the POIs the learned seeder's network seeds under the consensus
+SEED_SAMPLES = {"all": 0, "checker": 1, "quarter": 2}
 SAMPLING_MODES = {"auto": FI_SAMPLING_AUTO, "table": FI_SAMPLING_TABLE, "group": FI_SAMPLING_GROUP,
                   "pair": FI_SAMPLING_PAIR}
 FI_POLICY_LEARNED_SEED = 4
@@ -185,7 +187,7 @@ SIGNATURES = {
     "fi_frame_set_sampling": [_P, _I],
     "fi_frame_set_seeder": [_P, _P, _F],
     "fi_frame_set_seed_passes": [_P, _I],
-    "fi_frame_set_seed_consensus": [_P, _I, _F],
+    "fi_frame_set_seed_consensus": [_P, _I, _F, _I],
     "fi_frame_set_self_adaptive": [_P, _I],
     "fi_frame_set_graph": [_P, _I],
     "fi_frame_set_persistent": [_P, _I],
@@ -206,7 +208,7 @@ SIGNATURES = {
     "fi_seeder_forward": [_P, _P, _S, _P],
     "fi_seeder_gradient": [_P, _P, _P, _S, _P, ctypes.POINTER(_F)],
     "fi_seeder_seed": [_P, _P, _P, _I, _I, _P, _S, _P],
-    "fi_seeder_seed_consensus": [_P, _P, _P, _I, _I, _P, _S, _P, _F, _F, _I],
+    "fi_seeder_seed_consensus": [_P, _P, _P, _I, _I, _P, _S, _P, _F, _F, _I, _I],
     "fi_format_raw": [_P, _S, _P, _S, ctypes.POINTER(_S)],
     "fi_write_raw": [_P, _S, ctypes.c_char_p, ctypes.POINTER(_S)],
 }

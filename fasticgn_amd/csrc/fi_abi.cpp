@@ -445,14 +445,23 @@ int fi_frame_set_seed_passes(fi_frame* h, int passes)
     });
 }
 
-int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau)
+static void check_seed_sample(int sample)
+{
+    if (sample != FI_SEED_SAMPLE_ALL && sample != FI_SEED_SAMPLE_CHECKER && sample != FI_SEED_SAMPLE_QUARTER)
+        fail(FI_E_INVALID_ARGUMENT, "unknown seed sample " + std::to_string(sample));
+}
+
+int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau, int sample)
 {
     return guarded([&] {
         need(h);
         if (!(tau > 0.f)) fail(FI_E_INVALID_ARGUMENT, "seed consensus tau must be > 0");
+        check_seed_sample(sample);
         h->consensus = on != 0;
         h->consensus_tau = tau;
         h->cells.view.tau = tau;
+        if (sample != h->seed_sample) h->cells_ready = false;
+        h->seed_sample = sample;
         h->invalidate_graphs();
     });
 }
@@ -559,13 +568,15 @@ int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, 
 }
 
 int fi_seeder_seed_consensus(fi_seeder* h, const float* ref, const float* tar, int width, int height,
-                             fi_poi* pois, size_t n, uint32_t* ok, float radius, float tau, int passes)
+                             fi_poi* pois, size_t n, uint32_t* ok, float radius, float tau, int passes,
+                             int sample)
 {
     return guarded([&] {
         need(h);
         if (!(radius > 0.f) || !(tau > 0.f)) fail(FI_E_INVALID_ARGUMENT, "seed consensus radius and tau must be > 0");
         if (passes < 1 || passes > 4) fail(FI_E_INVALID_ARGUMENT, "seeder passes must be 1..4");
-        h->seed(ref, tar, width, height, pois, n, ok, radius, tau, passes);
+        check_seed_sample(sample);
+        h->seed(ref, tar, width, height, pois, n, ok, radius, tau, passes, sample);
     });
 }
 

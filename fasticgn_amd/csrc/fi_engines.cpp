@@ -1189,7 +1189,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             xs[i] = q.x;
             ys[i] = q.y;
         }
-        cells.upload(build_seed_cells(xs.data(), ys.data(), n, (float)r), (float)r, consensus_tau, stream);
+        cells.upload(build_seed_cells(xs.data(), ys.data(), n, (float)r, seed_sample), (float)r, consensus_tau,
+                     stream);
         seed_raw.reserve(2 * std::max<size_t>(n, 1));
         cells_ready = true;
     }
@@ -1217,10 +1218,20 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             // each pass crops the target at the previous pass's rounded u, v; a
             // POI a later pass cannot crop keeps the earlier seed, and only the
             // first pass's flag decides the FFTCC fallback for an unseeded POI
-            seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, seed_ok.p, stream);
+            // with a sample, the network seeds the sampled POIs and the
+            // consensus seeds (and flags) the rest
+            const bool sampled = consensus && cells.view.sample != nullptr;
+            auto network = [&](uint32_t* okp) {
+                if (!sampled)
+                    seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, okp, stream);
+                else if (cells.sampled > 0)
+                    seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, cells.sample_idx.p, (int)cells.sampled, okp,
+                                     stream);
+            };
+            network(seed_ok.p);
             if (consensus) launch_seed_consensus(cur.p, seed_ok.p, seed_raw.p, cnt, cells.view, stream);
             for (int pass = 1; pass < seed_passes; pass++) {
-                seeder->run_seed(ref_f.p, tar_f.p, w, h, cur.p, nullptr, cnt, nullptr, stream);
+                network(nullptr);
                 if (consensus) launch_seed_consensus(cur.p, seed_ok.p, seed_raw.p, cnt, cells.view, stream);
             }
         } else if (mode == 1) {

@@ -132,6 +132,7 @@ struct fi_frame {
     // learned frame runs on a layout.
     bool consensus = true;
     float consensus_tau = 0.5f;
+    int seed_sample = FI_SEED_SAMPLE_CHECKER;  // the POIs the network seeds (FI_SEED_SAMPLE_*)
     bool cells_ready = false;
     fi::DevSeedCells cells;
     fi::DevArray<float4> seed_raw;

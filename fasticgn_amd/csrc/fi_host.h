@@ -202,23 +202,31 @@ void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s,
 // Neighbourhood consensus of learned seeds (see k_policy.hip): raw (2 float4
 // per POI) receives the seeds as they stand, then each seeded POI (ok) takes
 // the mean of its seeded neighbours' predictions transported to it, over the
-// neighbours that agree within c.tau.
-void launch_seed_consensus(fi_poi* pois, const uint32_t* ok, float4* raw, int n, const SeedCells& c,
+// neighbours that agree within c.tau; a POI outside c.sample is seeded from its
+// neighbours alone and gets its ok here.
+void launch_seed_consensus(fi_poi* pois, uint32_t* ok, float4* raw, int n, const SeedCells& c,
                            hipStream_t s);
 // The cells of launch_seed_consensus on the host: side max(radius / 3,
 // sqrt(area / m)) over the bounding box of the m finite positions (FP64),
 // searched ceil(radius / side) cells to each side; POIs of a cell in position
-// order; a position with a non-finite coordinate has cell -1.
+// order; a position with a non-finite coordinate has cell -1.  sample_mode
+// picks the POIs the network seeds (FI_SEED_SAMPLE_*): every POI, those in
+// the cells (cx, cy) with cx + cy even (a checkerboard), or with cx and cy
+// even (a quarter); sample / sample_idx are empty for every POI.
 struct HostSeedCells {
     std::vector<uint32_t> start, idx;
     std::vector<int32_t> cell;
+    std::vector<uint8_t> sample;
+    std::vector<uint32_t> sample_idx;
     int ncx = 0, ncy = 0, k = 1;
 };
-HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float radius);
+HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float radius, int sample_mode = 0);
 // Device copy of a HostSeedCells (uploaded on s, which is synchronised).
 struct DevSeedCells {
-    DevArray<uint32_t> start, idx;
+    DevArray<uint32_t> start, idx, sample_idx;
     DevArray<int32_t> cell;
+    DevArray<uint8_t> sample;
+    size_t sampled = 0;  // entries of sample_idx (0: every POI, no list)
     SeedCells view{};
     void upload(const HostSeedCells& hc, float radius, float tau, hipStream_t s);
 };

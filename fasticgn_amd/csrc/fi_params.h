@@ -116,6 +116,8 @@ struct SeedCells {
     int ncx, ncy;
     int k;                  // cells searched on each side
     int m;                  // length of idx (the POIs with a cell)
+    const uint8_t* sample;  // per POI position: 1 = seeded by the network, 0 = by its neighbours
+                            // alone; nullptr = every POI by the network
     float radius;           // neighbours within +-radius in x and in y
     float tau;              // largest |u, v| disagreement of a neighbour's transported seed
 };

@@ -294,7 +294,7 @@ void dft_twiddles(int n, float* out)
     }
 }
 
-HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float radius)
+HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float radius, int sample_mode)
 {
     HostSeedCells hc;
     hc.cell.assign(n, -1);
@@ -339,6 +339,20 @@ HostSeedCells build_seed_cells(const float* xs, const float* ys, size_t n, float
     std::vector<uint32_t> fill(hc.start.begin(), hc.start.end() - 1);
     for (size_t i = 0; i < n; i++)
         if (hc.cell[i] >= 0) hc.idx[fill[(size_t)hc.cell[i]]++] = (uint32_t)i;
+    if (sample_mode != FI_SEED_SAMPLE_ALL) {
+        hc.sample.assign(n, 0);
+        for (size_t i = 0; i < n; i++) {
+            const int c = hc.cell[i];
+            if (c < 0) continue;
+            const int cx = c % hc.ncx, cy = c / hc.ncx;
+            const bool pick = sample_mode == FI_SEED_SAMPLE_CHECKER ? ((cx + cy) & 1) == 0
+                                                                     : ((cx | cy) & 1) == 0;
+            if (pick) {
+                hc.sample[i] = 1;
+                hc.sample_idx.push_back((uint32_t)i);
+            }
+        }
+    }
     return hc;
 }
 
@@ -352,7 +366,17 @@ void DevSeedCells::upload(const HostSeedCells& hc, float radius, float tau, hipS
         FI_HIP(hipMemcpyAsync(idx.p, hc.idx.data(), hc.idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     if (!hc.cell.empty())
         FI_HIP(hipMemcpyAsync(cell.p, hc.cell.data(), hc.cell.size() * sizeof(int32_t), hipMemcpyHostToDevice, s));
+    sampled = hc.sample_idx.size();
+    if (!hc.sample.empty()) {
+        sample.reserve(hc.sample.size());
+        FI_HIP(hipMemcpyAsync(sample.p, hc.sample.data(), hc.sample.size(), hipMemcpyHostToDevice, s));
+        sample_idx.reserve(std::max<size_t>(sampled, 1));
+        if (sampled)
+            FI_HIP(hipMemcpyAsync(sample_idx.p, hc.sample_idx.data(), sampled * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, s));
+    }
     FI_HIP(hipStreamSynchronize(s));  // the host vectors go out of scope with the caller
+    view.sample = hc.sample.empty() ? nullptr : sample.p;
     view.start = start.p;
     view.idx = idx.p;
     view.cell = cell.p;

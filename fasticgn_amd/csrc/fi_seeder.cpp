@@ -305,7 +305,7 @@ void fi_seeder::forward(const uint16_t* x1, size_t n, float* out)
 }
 
 void fi_seeder::seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok,
-                     float radius, float tau, int passes)
+                     float radius, float tau, int passes, int sample)
 {
     if (n == 0) return;
     if (!ref || !tar || !pois) fail(FI_E_INVALID_ARGUMENT, "null buffer");
@@ -323,7 +323,6 @@ void fi_seeder::seed(const float* ref, const float* tar, int w, int h, fi_poi* p
     FI_HIP(hipMemcpyAsync(d_ref.p, ref, px * sizeof(float), hipMemcpyHostToDevice, stream));
     FI_HIP(hipMemcpyAsync(d_tar.p, tar, px * sizeof(float), hipMemcpyHostToDevice, stream));
     FI_HIP(hipMemcpyAsync(d_pois.p, pois, n * sizeof(fi_poi), hipMemcpyHostToDevice, stream));
-    run_seed(d_ref.p, d_tar.p, w, h, d_pois.p, nullptr, (int)n, d_ok.p, stream);
     DevSeedCells cells;
     DevArray<float4> raw;
     if (radius > 0.f) {
@@ -332,11 +331,21 @@ void fi_seeder::seed(const float* ref, const float* tar, int w, int h, fi_poi* p
             xs[i] = pois[i].x;
             ys[i] = pois[i].y;
         }
-        cells.upload(build_seed_cells(xs.data(), ys.data(), n, radius), radius, tau, stream);
+        cells.upload(build_seed_cells(xs.data(), ys.data(), n, radius, sample), radius, tau, stream);
         raw.reserve(2 * n);
+    }
+    // the network on every POI, or on the sample (the consensus seeds the rest)
+    auto network = [&](uint32_t* okp) {
+        if (!cells.view.sample)
+            run_seed(d_ref.p, d_tar.p, w, h, d_pois.p, nullptr, (int)n, okp, stream);
+        else if (cells.sampled > 0)
+            run_seed(d_ref.p, d_tar.p, w, h, d_pois.p, cells.sample_idx.p, (int)cells.sampled, okp, stream);
+    };
+    network(d_ok.p);
+    if (radius > 0.f) {
         launch_seed_consensus(d_pois.p, d_ok.p, raw.p, (int)n, cells.view, stream);
         for (int pass = 1; pass < passes; pass++) {
-            run_seed(d_ref.p, d_tar.p, w, h, d_pois.p, nullptr, (int)n, nullptr, stream);
+            network(nullptr);
             launch_seed_consensus(d_pois.p, d_ok.p, raw.p, (int)n, cells.view, stream);
         }
         FI_HIP(hipGetLastError());

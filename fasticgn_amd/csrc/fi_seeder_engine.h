@@ -38,8 +38,10 @@ struct fi_seeder {
     // passes > 1: further passes, each cropping the target at the previous
     // seeds and followed by the consensus over the first pass's ok, as the
     // frame driver runs them
+    // sample (FI_SEED_SAMPLE_*): with the consensus, the POIs the network
+    // seeds; the others take the consensus of their neighbours
     void seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, size_t n, uint32_t* ok,
-              float radius = 0.f, float tau = 0.f, int passes = 1);
+              float radius = 0.f, float tau = 0.f, int passes = 1, int sample = 0);
     // device-side seeding for the frame driver (stream-ordered, no sync)
     void run_seed(const float* ref, const float* tar, int w, int h, fi_poi* pois, const uint32_t* index, int n,
                   uint32_t* ok, hipStream_t s) const;

@@ -504,8 +504,8 @@ __global__ void k_seed_raw(const fi_poi* __restrict__ pois, const uint32_t* __re
     const uint32_t i = c.idx[k];
     const fi_poi& q = pois[i];
     const float* d = q.deformation;
-    const bool use = ok[i] && finite_f(d[0]) && finite_f(d[6]) && finite_f(d[1]) && finite_f(d[2]) &&
-                     finite_f(d[7]) && finite_f(d[8]);
+    const bool use = (!c.sample || c.sample[i]) && ok[i] && finite_f(d[0]) && finite_f(d[6]) && finite_f(d[1]) &&
+                     finite_f(d[2]) && finite_f(d[7]) && finite_f(d[8]);
     raw[2 * k] = use ? make_float4(q.x, q.y, d[0], d[6]) : make_float4(3e38f, 0.f, 0.f, 0.f);
     raw[2 * k + 1] = use ? make_float4(d[1], d[2], d[7], d[8]) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
@@ -515,19 +515,53 @@ __global__ void k_seed_raw(const fi_poi* __restrict__ pois, const uint32_t* __re
 // with w = 1 when it passes the tests, else 0: a rejected entry adds +-0,
 // which leaves each sum's bits as they are (the sums start at +0 and never
 // become -0), the loads of a run are independent and each sum is one FMA.
-__global__ void k_seed_consensus(fi_poi* __restrict__ pois, const uint32_t* __restrict__ ok,
+// A POI the network did not seed (c.sample) has no prediction of its own:
+// the tau test is measured from the mean of the carried predictions within
+// +-radius (a first pass), and its ok is set by whether any survive.
+__global__ void k_seed_consensus(fi_poi* __restrict__ pois, uint32_t* __restrict__ ok,
                                  const float4* __restrict__ raw, int n, SeedCells c)
 {
     const int i = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n || !ok[i]) return;
+    if (i >= n) return;
+    const bool own = !c.sample || c.sample[i];
+    if (own && !ok[i]) return;
     const int ci = c.cell[i];
-    if (ci < 0) return;
+    if (ci < 0) {
+        if (!own) ok[i] = 0;
+        return;
+    }
     fi_poi& q = pois[i];
-    const float ax = q.x, ay = q.y, au = q.deformation[0], av = q.deformation[6];
+    const float ax = q.x, ay = q.y;
+    float au = q.deformation[0], av = q.deformation[6];
     const int cx = ci % c.ncx, cy = ci / c.ncx;
     const int xa = max(cx - c.k, 0), xb = min(cx + c.k, c.ncx - 1);
+    const int ya = max(cy - c.k, 0), yb = min(cy + c.k, c.ncy - 1);
+    if (!own) {
+        float n1 = 0.f, s1u = 0.f, s1v = 0.f;
+        for (int yy = ya; yy <= yb; yy++) {
+            const uint32_t k0 = c.start[yy * c.ncx + xa], k1 = c.start[yy * c.ncx + xb + 1];
+#pragma unroll 8
+            for (uint32_t k = k0; k < k1; k++) {
+                const float4 b = raw[2 * k], g = raw[2 * k + 1];
+                const float dx = b.x - ax, dy = b.y - ay;
+                const float tu = b.z - (g.x * dx + g.y * dy);
+                const float tv = b.w - (g.z * dx + g.w * dy);
+                const float w = (fabsf(dx) <= c.radius && fabsf(dy) <= c.radius) ? 1.f : 0.f;
+                n1 += w;
+                s1u = __builtin_fmaf(w, tu, s1u);
+                s1v = __builtin_fmaf(w, tv, s1v);
+            }
+        }
+        if (n1 == 0.f) {
+            ok[i] = 0;
+            return;
+        }
+        const float inv1 = 1.f / n1;
+        au = s1u * inv1;
+        av = s1v * inv1;
+    }
     float cnt = 0.f, su = 0.f, sv = 0.f, sdx = 0.f, sdy = 0.f, sux = 0.f, suy = 0.f, svx = 0.f, svy = 0.f;
-    for (int yy = max(cy - c.k, 0); yy <= min(cy + c.k, c.ncy - 1); yy++) {
+    for (int yy = ya; yy <= yb; yy++) {
         const uint32_t k0 = c.start[yy * c.ncx + xa], k1 = c.start[yy * c.ncx + xb + 1];
 #pragma unroll 8
         for (uint32_t k = k0; k < k1; k++) {
@@ -548,7 +582,10 @@ __global__ void k_seed_consensus(fi_poi* __restrict__ pois, const uint32_t* __re
             svy = __builtin_fmaf(w, g.w, svy);
         }
     }
-    if (cnt == 0.f) return;  // the POI's own prediction is not finite
+    if (cnt == 0.f) {  // own prediction not finite, or no neighbour agrees with the mean
+        if (!own) ok[i] = 0;
+        return;
+    }
     const float inv = 1.f / cnt;
     const float ux = sux * inv, uy = suy * inv, vx = svx * inv, vy = svy * inv;
     const float mdx = sdx * inv, mdy = sdy * inv;
@@ -558,9 +595,10 @@ __global__ void k_seed_consensus(fi_poi* __restrict__ pois, const uint32_t* __re
     q.deformation[2] = uy;
     q.deformation[7] = vx;
     q.deformation[8] = vy;
+    if (!own) ok[i] = 1;
 }
 
-void launch_seed_consensus(fi_poi* pois, const uint32_t* ok, float4* raw, int n, const SeedCells& c,
+void launch_seed_consensus(fi_poi* pois, uint32_t* ok, float4* raw, int n, const SeedCells& c,
                            hipStream_t s)
 {
     if (n <= 0) return;

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kInferThreads, 2) void k_seed_infer(SeedInferParams
     if (p.w < kSeedCrop || p.h < kSeedCrop) {
         // no crop fits: nothing is seeded
         for (int b = (blockIdx.x * kInferWaves + wave) * 64 + lane; p.ok && b < p.n; b += gridDim.x * kInferThreads)
-            p.ok[b] = 0;
+            p.ok[p.index ? p.index[b] : (uint32_t)b] = 0;
         return;
     }
 
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kInferThreads, 2) void k_seed_infer(SeedInferParams
         }
         // lane (0, s) holds outputs 0..3 of POI s, lane (1, s) outputs 4..7
         if (b < p.n) {
-            if (g == 0 && p.ok) p.ok[b] = (uint32_t)ok;
+            if (g == 0 && p.ok) p.ok[qi] = (uint32_t)ok;
             if (ok && g < 2) {
                 fi_poi& q = p.pois[qi];
                 const float* b5 = p.wts.b5;

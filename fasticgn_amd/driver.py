@@ -124,12 +124,17 @@ class FrameDriver(_Handle):
         previous pass's rounded u, v."""
         nat.call("fi_frame_set_seed_passes", self._h, int(passes))
 
-    def set_seed_consensus(self, on: bool = True, tau: float = 0.5) -> None:
+    def set_seed_consensus(self, on: bool = True, tau: float = 0.5, sample: str = "checker") -> None:
         """Neighbourhood consensus of the seeder's predictions after each pass
-        (default on, tau 0.5 px): each seeded POI takes the mean of its seeded
+        (default on, tau 0.5 px, sample 'checker'): each seeded POI takes the mean of its seeded
         neighbours' predictions within +-radius, carried to it with their
-        gradients, over the neighbours that agree within tau px."""
-        nat.call("fi_frame_set_seed_consensus", self._h, int(bool(on)), float(tau))
+        gradients, over the neighbours that agree within tau px.  sample: the
+        POIs the network seeds ('all', 'checker' = a checkerboard of the
+        consensus cells, 'quarter'); the others take their neighbours'
+        consensus."""
+        if sample not in nat.SEED_SAMPLES:
+            raise ValueError(f"sample must be one of {sorted(nat.SEED_SAMPLES)}")
+        nat.call("fi_frame_set_seed_consensus", self._h, int(bool(on)), float(tau), nat.SEED_SAMPLES[sample])
 
     def set_self_adaptive(self, on: bool = True) -> None:
         """Mixed subset radii: every template POI's (square) subset_radius is

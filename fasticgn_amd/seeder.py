@@ -181,25 +181,26 @@ class LearnedSeeder:
         return out
 
     def seed(self, ref: np.ndarray, tar: np.ndarray, pois: np.ndarray,
-             consensus: tuple[float, float] | None = None, passes: int = 1) -> np.ndarray:
+             consensus: tuple[float, float] | None = None, passes: int = 1, sample: str = "all") -> np.ndarray:
         """Seeds POIs in place (fused inference kernel); returns the ok mask.
         consensus = (radius, tau): then the neighbourhood consensus of the
         predictions (fi_seeder_seed_consensus), and passes - 1 further passes
-        each followed by it, as the frame driver seeds with its subset radius."""
+        each followed by it, as the frame driver seeds with its subset radius;
+        sample: the POIs the network seeds (FrameDriver.set_seed_consensus)."""
         ref = np.ascontiguousarray(ref, dtype=np.float32)
         tar = np.ascontiguousarray(tar, dtype=np.float32)
         assert ref.shape == tar.shape and pois.dtype == POI_DTYPE and pois.flags.c_contiguous
         ok = np.zeros(len(pois), dtype=np.uint32)
         h, w = ref.shape
         if consensus is None:
-            if passes != 1:
-                raise ValueError("further seeder passes go with the consensus (the frame driver's seeding)")
+            if passes != 1 or sample != "all":
+                raise ValueError("further seeder passes and samples go with the consensus (the frame driver's seeding)")
             nat.call("fi_seeder_seed", self._h, nat.ptr(ref), nat.ptr(tar), w, h, nat.ptr(pois), len(pois),
                      nat.ptr(ok))
         else:
             radius, tau = consensus
             nat.call("fi_seeder_seed_consensus", self._h, nat.ptr(ref), nat.ptr(tar), w, h, nat.ptr(pois),
-                     len(pois), nat.ptr(ok), float(radius), float(tau), int(passes))
+                     len(pois), nat.ptr(ok), float(radius), float(tau), int(passes), nat.SEED_SAMPLES[sample])
         return ok.astype(bool)
 
 

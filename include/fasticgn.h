@@ -433,10 +433,11 @@ int fi_seeder_seed(fi_seeder* h, const float* ref, const float* tar, int width, 
  * agree with the POI's own within tau px; the mean gradients carry the mean
  * u, v from the neighbours' centroid to the POI.  passes (1..4): further seeder
  * passes as in fi_frame_set_seed_passes, each followed by the consensus over
- * the first pass's ok (the frame driver's learned seeding, run alone).  No
- * reference counterpart. */
+ * the first pass's ok (the frame driver's learned seeding, run alone).
+ * sample: FI_SEED_SAMPLE_* (below).  No reference counterpart. */
 int fi_seeder_seed_consensus(fi_seeder* h, const float* ref, const float* tar, int width, int height,
-                             fi_poi* pois, size_t n, uint32_t* ok, float radius, float tau, int passes);
+                             fi_poi* pois, size_t n, uint32_t* ok, float radius, float tau, int passes,
+                             int sample);
 
 /* Frame driver with the learned seeder (policy FI_POLICY_LEARNED_SEED): the
  * FFTCC seeding of a frame is replaced by the seeder; POIs whose IC-GN result
@@ -450,11 +451,19 @@ int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold);
 int fi_frame_set_seed_passes(fi_frame* h, int passes);
 /* Neighbourhood consensus of the seeder's predictions after each pass (as
  * fi_seeder_seed_consensus, radius = the driver's subset radius; default on,
- * tau 0.5 px).  The seeds of a smooth field then lose most of the network's
+ * tau 0.5 px, the network on a checkerboard of the cells).  The seeds of a smooth field then lose most of the network's
  * per-POI error, and IC-GN needs fewer iterations; a neighbour across a
  * discontinuity disagrees by more than tau and is left out.  No reference
  * counterpart. */
-int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau);
+int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau, int sample);
+/* Which POIs the network seeds when the consensus is on: every POI, a
+ * checkerboard of the consensus cells (about half), or the cells with both
+ * coordinates even (about a quarter); every other POI takes the consensus of
+ * its network-seeded neighbours, the tau test then measured from their mean
+ * (none left within tau: not seeded, FFTCC fallback). */
+#define FI_SEED_SAMPLE_ALL 0
+#define FI_SEED_SAMPLE_CHECKER 1
+#define FI_SEED_SAMPLE_QUARTER 2
 
 /* ---- raw output record (opencorr_cli_cuda_mvp.cpp:2498-2536) --------------
  * 36-byte little-endian record x,y (int32), u,v,ux,uy,vx,vy,zncc (float32). */

@@ -131,6 +131,16 @@ def loss_and_grad(params: np.ndarray, x1_bits: np.ndarray, targets: np.ndarray, 
     return loss, np.concatenate(grads).astype(dt)
 
 
+def seed_sample(cell: np.ndarray, ncx: int, mode: int) -> np.ndarray:
+    """The POIs the network seeds under the consensus: every one (mode 0),
+    those in cells (cx, cy) with cx + cy even (1), or cx and cy even (2)."""
+    if mode == 0:
+        return np.ones(len(cell), dtype=bool)
+    cx, cy = cell % max(ncx, 1), cell // max(ncx, 1)
+    pick = ((cx + cy) % 2 == 0) if mode == 1 else ((cx % 2 == 0) & (cy % 2 == 0))
+    return (cell >= 0) & pick
+
+
 def seed_cells(x: np.ndarray, y: np.ndarray, radius: float):
     """Cells of the consensus (build_seed_cells in fasticgn_amd/csrc/fi_runtime.cpp):
     side max(radius / 3, sqrt(area / m)) over the finite positions' bounding
@@ -159,7 +169,8 @@ def seed_cells(x: np.ndarray, y: np.ndarray, radius: float):
     return cell, ncx, ncy, k, [np.array(mb, dtype=np.int64) for mb in members]
 
 
-def consensus(x: np.ndarray, y: np.ndarray, seeds: np.ndarray, ok: np.ndarray, radius: float, tau: float):
+def consensus(x: np.ndarray, y: np.ndarray, seeds: np.ndarray, ok: np.ndarray, radius: float, tau: float,
+              sample_mode: int = 0, return_ok: bool = False):
     """Consensus seeds (n x 6: u, v, ux, uy, vx, vy) from the network's seeds:
     each POI with ok takes the mean of its ok neighbours within +-radius (x and
     y) whose predictions are finite, each neighbour's u, v carried to it with
@@ -167,23 +178,37 @@ def consensus(x: np.ndarray, y: np.ndarray, seeds: np.ndarray, ok: np.ndarray, r
     over the neighbours whose carried u, v agree with its own within tau; the
     mean gradients carry the mean u, v from the neighbours' centroid to it.
     FP32, sums in cell order (row by row around the POI's cell), then position
-    order."""
+    order.  sample_mode (seed_sample): only the sampled POIs' predictions are
+    used; a POI outside the sample (its own seeds and ok ignored) measures the
+    tau test from the mean of the carried predictions within +-radius and is
+    seeded (ok) when any neighbour then survives.  return_ok: also the ok of
+    every POI after the consensus."""
     f = np.float32
     x = np.asarray(x, dtype=f)
     y = np.asarray(y, dtype=f)
     s = np.asarray(seeds, dtype=f)
     ok = np.asarray(ok, dtype=bool)
     out = s.copy()
+    ok_out = ok.copy()
     cell, ncx, ncy, K, members = seed_cells(x, y, radius)
+    sampled = seed_sample(cell, ncx, sample_mode)
+    usable = sampled & ok & np.all(np.isfinite(s), axis=1)
     R, T = f(radius), f(tau)
+    # sequential FP32 sums (cumsum is a running sum in the array's type)
+    tot = lambda a: np.cumsum(a.astype(f), dtype=f)[-1] if len(a) else f(0)
     for i in range(len(x)):
-        if not ok[i] or cell[i] < 0:
+        own = bool(sampled[i])
+        if own and not ok[i]:
+            continue
+        if cell[i] < 0:
+            if not own:
+                ok_out[i] = False
             continue
         cx, cy = int(cell[i] % ncx), int(cell[i] // ncx)
         cand = [members[yy * ncx + xx] for yy in range(max(cy - K, 0), min(cy + K, ncy - 1) + 1)
                 for xx in range(max(cx - K, 0), min(cx + K, ncx - 1) + 1)]
         j = np.concatenate(cand) if cand else np.zeros(0, dtype=np.int64)
-        j = j[ok[j] & np.all(np.isfinite(s[j]), axis=1)]
+        j = j[usable[j]]
         dx = (x[j] - x[i]).astype(f)
         dy = (y[j] - y[i]).astype(f)
         with np.errstate(invalid="ignore"):
@@ -192,12 +217,22 @@ def consensus(x: np.ndarray, y: np.ndarray, seeds: np.ndarray, ok: np.ndarray, r
             sj = s[j]
             tu = (sj[:, 0] - ((sj[:, 2] * dx).astype(f) + (sj[:, 3] * dy).astype(f)).astype(f)).astype(f)
             tv = (sj[:, 1] - ((sj[:, 4] * dx).astype(f) + (sj[:, 5] * dy).astype(f)).astype(f)).astype(f)
-            keep = (np.abs((tu - s[i, 0]).astype(f)) <= T) & (np.abs((tv - s[i, 1]).astype(f)) <= T)
+            if own:
+                au, av = s[i, 0], s[i, 1]
+            else:
+                if len(j) == 0:
+                    ok_out[i] = False
+                    continue
+                inv1 = f(f(1.0) / f(len(j)))
+                au, av = f(tot(tu) * inv1), f(tot(tv) * inv1)
+            keep = (np.abs((tu - au).astype(f)) <= T) & (np.abs((tv - av).astype(f)) <= T)
         sj, dx, dy = sj[keep], dx[keep], dy[keep]
         if len(sj) == 0:
+            if not own:
+                ok_out[i] = False
             continue
-        # sequential FP32 sums (cumsum is a running sum in the array's type)
-        tot = lambda a: np.cumsum(a.astype(f), dtype=f)[-1]
+        if not own:
+            ok_out[i] = True
         cnt = f(len(sj))
         inv = f(f(1.0) / cnt)
         su, sv, sdx, sdy = tot(sj[:, 0]), tot(sj[:, 1]), tot(dx), tot(dy)
@@ -206,4 +241,4 @@ def consensus(x: np.ndarray, y: np.ndarray, seeds: np.ndarray, ok: np.ndarray, r
         out[i, 0] = f(f(su * inv) - f(f(g[0] * mdx) + f(g[1] * mdy)))
         out[i, 1] = f(f(sv * inv) - f(f(g[2] * mdx) + f(g[3] * mdy)))
         out[i, 2:] = g
-    return out
+    return (out, ok_out) if return_ok else out

@@ -128,9 +128,10 @@ def test_config4_crack_field_parity(oracle):
         assert cmp["pass"] and cn["pass"], (f64, cmp, cn)
     # learned frame: the oracle's learned frame from the seeder's seeds
     # the driver's seeding: two passes (the second crops at the first's rounded
-    # u, v), each followed by the neighbourhood consensus over every POI
+    # u, v), the network on a checkerboard of the POIs, each pass followed by
+    # the neighbourhood consensus over every POI
     seeds = sc.pois.copy()
-    ok = seeder.seed(ref32, tar32, seeds, consensus=(float(R), 0.5), passes=2)
+    ok = seeder.seed(ref32, tar32, seeds, consensus=(float(R), 0.5), passes=2, sample="checker")
     po = PolicyOracle(oracle, sc.ref, sc.pois[pick], R)
     o, ost = po.learned_frame(sc.tar, seeds[pick], ok[pick], 0.8)
     cmp = compare(o, res_l[pick], max_iteration_mismatch=0.01)

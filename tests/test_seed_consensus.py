@@ -111,12 +111,53 @@ def test_unseeded_and_non_finite_are_left_out():
     np.testing.assert_allclose(o[good, 0], 0.5, atol=1e-6)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
+def test_sampled_seeding_linear_field(mode):
+    """The network seeds a checkerboard (1) or a quarter (2) of the cells; the
+    other POIs take their neighbours' consensus, carried to them: a linear
+    field comes back everywhere and every POI is seeded.  The sample is the
+    cell pattern (cells of side radius / 3 = one grid step here)."""
+    x, y = grid(20, 220, 30, 200, 5)
+    a = (1.25, 0.013, -0.021, -3.5, 0.034, 0.0045)
+    s = linear_seeds(x, y, a)
+    cell, ncx, *_ = S.seed_cells(x, y, 15.0)
+    picked = S.seed_sample(cell, ncx, mode)
+    assert abs(picked.mean() - (0.5 if mode == 1 else 0.25)) < 0.03
+    seeds = np.where(picked[:, None], s, np.float32(0))  # the others have no prediction
+    ok = picked.copy()
+    o, okk = S.consensus(x, y, seeds, ok, 15.0, 0.5, sample_mode=mode, return_ok=True)
+    assert okk.all()
+    np.testing.assert_allclose(o[:, :2], s[:, :2], atol=3e-6)
+    np.testing.assert_allclose(o[:, 2:], s[:, 2:], atol=1e-7)
+
+
+def test_sampled_seeding_at_a_step():
+    """A POI outside the sample measures tau from the mean of its neighbours'
+    predictions: next to a 1.5 px step it takes its majority side's value
+    exactly, or, with a neighbourhood split near half and half, no seed at all
+    (FFTCC fallback) - never a blend of the two sides."""
+    x, y = grid(0, 200, 0, 100, 5)
+    s = np.zeros((len(x), 6), np.float32)
+    s[:, 0] = np.where(x < 100, 2.0, 3.5)
+    s[:, 1] = -1.0
+    cell, ncx, *_ = S.seed_cells(x, y, 15.0)
+    picked = S.seed_sample(cell, ncx, 1)
+    o, okk = S.consensus(x, y, s, picked.copy(), 15.0, 0.5, sample_mode=1, return_ok=True)
+    other = ~picked
+    side = np.where(x < 100, 2.0, 3.5)
+    assert np.all(np.abs(o[other & okk, 0] - side[other & okk]) < 1e-6)
+    assert (other & ~okk).any() and (other & okk & (np.abs(x - 100) <= 15)).any()
+    assert np.all(okk[np.abs(x - 100) > 15])
+
+
 # ------------------------------------------------------------------ GPU --
 
 @pytest.mark.gpu
-def test_device_consensus_matches_restatement():
+@pytest.mark.parametrize("sample", ["all", "checker", "quarter"])
+def test_device_consensus_matches_restatement(sample):
     """fi_seeder_seed_consensus on a config-1 slice against the restatement
-    applied to the raw device seeds: identical bits."""
+    applied to the raw device seeds: identical bits and flags, with the network
+    on every POI or on a sample of them."""
     from fasticgn_amd import seeder as sd
     sc = synth.config1(size=512)
     ref32, tar32 = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
@@ -126,23 +167,25 @@ def test_device_consensus_matches_restatement():
         raw = q.copy()
         ok = s.seed(ref32, tar32, raw)
         dev = q.copy()
-        ok2 = s.seed(ref32, tar32, dev, consensus=(15.0, 0.5))
-    assert np.array_equal(ok, ok2)
+        ok2 = s.seed(ref32, tar32, dev, consensus=(15.0, 0.5), sample=sample)
     cols = [0, 6, 1, 2, 7, 8]
     d = raw["deformation"]
     seeds = np.stack([d[:, c] for c in cols], 1)
-    want = S.consensus(raw["x"], raw["y"], seeds, ok, 15.0, 0.5)
+    want, want_ok = S.consensus(raw["x"], raw["y"], seeds, ok, 15.0, 0.5,
+                                sample_mode=fa._native.SEED_SAMPLES[sample], return_ok=True)
     got = np.stack([dev["deformation"][:, c] for c in cols], 1)
-    assert got.tobytes() == want.tobytes()
-    assert np.abs(got - seeds)[ok].max() > 0  # the consensus changed the seeds
+    assert np.array_equal(ok2, want_ok)
+    m = want_ok  # a POI left unseeded keeps whatever it held
+    assert got[m].tobytes() == want[m].tobytes()
+    assert np.abs(got - seeds)[ok & m].max() > 0  # the consensus changed the seeds
 
 
 @pytest.mark.gpu
 def test_frame_consensus_parity_and_iterations():
-    """The learned frame with consensus (the driver's default) against the
-    oracle run from the same consensus seeds on a POI sample: the parity gate
-    and identical iteration counts up to summation-order noise; fewer IC-GN
-    iterations than the raw seeds."""
+    """The learned frame with consensus, the network on a checkerboard of the
+    POIs, against the oracle run from the same seeds on a POI sample: the
+    parity gate and identical iteration counts up to summation-order noise;
+    fewer IC-GN iterations than the raw seeds, with or without the sample."""
     from fasticgn_amd import seeder as sd
     from oracle import pyoracle
     from tests.parity import compare
@@ -150,21 +193,24 @@ def test_frame_consensus_parity_and_iterations():
     sc = synth.config1(size=1024)
     stats = {}
     with sd.LearnedSeeder() as s:
-        for on in (False, True):
+        for on, sample in ((False, "all"), (True, "all"), (True, "checker")):
             with fa.FrameDriver(sc.radius) as d:
                 d.set_reference(fa.Image2D(sc.ref))
                 d.set_pois(sc.pois)
                 d.set_seeder(s, 0.8)
-                d.set_seed_consensus(on)
+                d.set_seed_consensus(on, 0.5, sample)
                 d.upload_target(fa.Image2D(sc.tar))
-                stats[on] = d.run(fa.driver.LEARNED_SEED)
-                res = d.download()
-        print("iterations per POI: raw", stats[False]["iteration_mean"], "consensus",
-              stats[True]["iteration_mean"], flush=True)
-        assert stats[True]["iteration_mean"] < stats[False]["iteration_mean"] - 0.15
+                stats[on, sample] = d.run(fa.driver.LEARNED_SEED)
+                if sample == "checker":
+                    res = d.download()
+        print("iterations per POI: raw", stats[False, "all"]["iteration_mean"], "consensus",
+              stats[True, "all"]["iteration_mean"], "checkerboard", stats[True, "checker"]["iteration_mean"],
+              flush=True)
+        assert stats[True, "all"]["iteration_mean"] < stats[False, "all"]["iteration_mean"] - 0.15
+        assert stats[True, "checker"]["iteration_mean"] < stats[False, "all"]["iteration_mean"] - 0.15
         ref32, tar32 = sc.ref.astype(np.float32), sc.tar.astype(np.float32)
         seeds = sc.pois.copy()
-        ok = s.seed(ref32, tar32, seeds, consensus=(float(sc.radius), 0.5))
+        ok = s.seed(ref32, tar32, seeds, consensus=(float(sc.radius), 0.5), sample="checker")
     pick = np.arange(0, len(sc.pois), 9)
     po = PolicyOracle(pyoracle, sc.ref, sc.pois[pick], sc.radius)
     o, _ = po.learned_frame(sc.tar, seeds[pick], ok[pick], 0.8)
