@@ -255,7 +255,7 @@ def bench_config4(args) -> int:
     tune = drv.autotune()
     chosen = [e for e in tune if e["chosen"]]
     log(f"[bench] autotune chosen {chosen}")
-    seeder = sd.LearnedSeeder()
+    seeder = sd.LearnedSeeder(np.load(args.seeder) if args.seeder not in ("auto", "off") else None)
     drv.set_seeder(seeder, args.seed_zncc)
     # POIs start at guess 0 with u up to ~6 px: a second seeder pass crops the
     # target at the first pass's rounded u, v (the seeder's near regime)

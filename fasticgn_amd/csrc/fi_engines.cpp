@@ -1080,7 +1080,7 @@ void fi_frame::fftcc_all(fi_poi* base)
 // wins), redo flags.  With count_dev (lo = 0) the candidate count is read on
 // the device and m is only the capacity the launches are sized for, so the
 // repair can sit in a captured frame graph.
-void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* count_dev)
+void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* count_dev, const uint32_t* seeded)
 {
     const int k = m - lo;
     if (k <= 0) return;
@@ -1091,7 +1091,8 @@ void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* 
     launch_gather(tmpl.p + c.off, idx, k, rep, stream, count_dev, redo.p + c.off);
     fftcc_pass(c, rep, k, idx, count_dev);
     icgn_pass(c, rep, k, idx, count_dev, 1);
-    launch_merge_repair(cur.p + c.off, rep, idx, k, counters.p + ci, stream, count_dev);
+    launch_merge_repair(cur.p + c.off, rep, idx, k, counters.p + ci, stream, count_dev,
+                        seeded ? seeded + c.off : nullptr);
     FI_HIP(hipGetLastError());
 }
 
@@ -1267,7 +1268,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             for (int ci = 0; ci < C; ci++)
                 if (classes[ci]->n > 0)
                     repair_class(*classes[ci], repair_cap(*classes[ci]), 0,
-                                 mode == 0 ? &counters.p[ci].candidates : &counters.p[ci].repair_n);
+                                 mode == 0 ? &counters.p[ci].candidates : &counters.p[ci].repair_n,
+                                 mode == 0 ? seed_ok.p : nullptr);
         }
         if (mode == 2) {
             // whole-frame fallback with per-POI merge (:3115-3152), gated on the
@@ -1344,7 +1346,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         for (int ci = 0; ci < C; ci++) {
             const int m = (int)pc[ci].candidates, cap = repair_cap(*classes[ci]);
             if (m > cap) {
-                repair_class(*classes[ci], m, cap);
+                repair_class(*classes[ci], m, cap, nullptr, seed_ok.p);
                 redo_qa = true;
             }
         }

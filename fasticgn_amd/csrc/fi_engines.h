@@ -254,7 +254,10 @@ private:
     bool fused_shape(int& waves, int& spl) const;
     void icgn_all(fi_poi* base, int region = 0);
     void fftcc_all(fi_poi* base);
-    void repair_class(const FrameClass& c, int m, int lo = 0, const unsigned* count_dev = nullptr);
+    // seeded: the learned seeder's flags (null for temporal repairs), see
+    // launch_merge_repair
+    void repair_class(const FrameClass& c, int m, int lo = 0, const unsigned* count_dev = nullptr,
+                      const uint32_t* seeded = nullptr);
     int repair_cap(const FrameClass& c) const;
     void prep_target(bool table);
     bool table_needed() const;

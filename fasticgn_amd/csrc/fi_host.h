@@ -193,8 +193,12 @@ void launch_gather(const fi_poi* src, const uint32_t* idx, int n, fi_poi* dst, h
 // flags[0..n) = 1 when merged
 void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int n, hipStream_t s,
                                  const unsigned* gate = nullptr, uint8_t* flags = nullptr);
+// seeded (optional, indexed like cur): the learned seeder's flags; a POI with
+// flag 0 takes its repair unconditionally, the others under
+// shouldReplacePoiWithFallback
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
-                         FrameCounters* counters, hipStream_t s, const unsigned* count = nullptr);
+                         FrameCounters* counters, hipStream_t s, const unsigned* count = nullptr,
+                         const uint32_t* seeded = nullptr);
 // counters->iterations += sum of iters[0..n_iters) (the frame's per-entry
 // executed iteration counts)
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s, const uint32_t* iters = nullptr,

@@ -16,11 +16,9 @@ constexpr float kGoodZncc = 0.8f;  // kSkipFftccZnccThreshold, :479
 __device__ __forceinline__ bool finite_f(float v) { return isfinite(v); }
 
 // isBadPoiForFallback (:493-497)
-__device__ __forceinline__ bool is_bad(const fi_poi& p)
-{
-    const float z = p.result[2];
-    return z != -3.f && (!finite_f(z) || z < kGoodZncc);
-}
+__device__ __forceinline__ bool is_bad_zncc(float z) { return z != -3.f && (!finite_f(z) || z < kGoodZncc); }
+
+__device__ __forceinline__ bool is_bad(const fi_poi& p) { return is_bad_zncc(p.result[2]); }
 
 // shouldReplacePoiWithFallback (:505-517)
 __device__ __forceinline__ bool should_replace(const fi_poi& cur, const fi_poi& fb)
@@ -55,10 +53,10 @@ __device__ __forceinline__ bool is_candidate(const fi_poi& cur, const fi_poi* pr
 // of one per POI (same-address atomics serialise in L2).
 constexpr int kPolicyThreads = 256;
 
-template <int N>
+template <int N, int THREADS = kPolicyThreads>
 __device__ __forceinline__ void block_reduce(unsigned (&v)[N], unsigned (&mx)[1])
 {
-    __shared__ unsigned red[N + 1][kPolicyThreads / 64];
+    __shared__ unsigned red[N + 1][THREADS / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int q = 0; q < N; q++) {
@@ -80,11 +78,11 @@ __device__ __forceinline__ void block_reduce(unsigned (&v)[N], unsigned (&mx)[1]
 #pragma unroll
         for (int q = 0; q < N; q++) {
             unsigned t = 0;
-            for (int u = 0; u < kPolicyThreads / 64; u++) t += red[q][u];
+            for (int u = 0; u < THREADS / 64; u++) t += red[q][u];
             v[q] = t;
         }
         unsigned t = 0;
-        for (int u = 0; u < kPolicyThreads / 64; u++) t = max(t, red[N][u]);
+        for (int u = 0; u < THREADS / 64; u++) t = max(t, red[N][u]);
         mx[0] = t;
     }
 }
@@ -246,14 +244,17 @@ __global__ void k_merge_frame(const fi_poi* __restrict__ skip, fi_poi* __restric
     if (!take_frame_fallback(skip[i], scratch[i])) scratch[i] = skip[i];
 }
 
+// seeded (optional): the learned seeder's per-POI flags; a POI it did not
+// seed takes the repair unconditionally, so its record is the FFTCC-seeded
+// path's whatever the IC-GN from its stale guess reached
 __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restrict__ rep,
                                const uint32_t* __restrict__ idx, int n, FrameCounters* counters,
-                               const unsigned* count)
+                               const unsigned* count, const uint32_t* __restrict__ seeded)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= bounded(n, count)) return;
     const uint32_t i = idx[j];
-    if (should_replace(cur[i], rep[j])) {
+    if ((seeded && seeded[i] == 0u) || should_replace(cur[i], rep[j])) {
         const bool recovered = !is_bad(rep[j]);
         cur[i] = rep[j];
         if (recovered) atomicAdd(&counters->recovered, 1ull);
@@ -264,28 +265,46 @@ __global__ void k_merge_repair(fi_poi* __restrict__ cur, const fi_poi* __restric
 // over finite iteration counts; bad per isBadPoiForFallback.  Also the frame's
 // executed IC-GN iterations: the sum of iters[0..n_iters), the per-entry
 // counts every IC-GN launch of the frame wrote (zeroed at the frame's start).
-__global__ __launch_bounds__(kPolicyThreads) void k_qa(const fi_poi* __restrict__ cur, int n,
-                                                        FrameCounters* counters,
-                                                        const uint32_t* __restrict__ iters, size_t n_iters)
+constexpr int kQaThreads = 1024;
+
+__global__ __launch_bounds__(kQaThreads) void k_qa(const fi_poi* __restrict__ cur, int n,
+                                                   FrameCounters* counters,
+                                                   const uint32_t* __restrict__ iters, size_t n_iters)
 {
-    // grid-stride: a few hundred blocks, so the six counters see one atomic
-    // per block (same-address atomics serialise)
+    // grid-stride over a few hundred blocks, so the six counters see one
+    // atomic per block (same-address atomics serialise); blocks of 1024
+    // threads and loads batched four deep keep enough reads in flight
     unsigned v[5] = { 0u, 0u, 0u, 0u, 0u }, mx[1] = { 0u };
-    for (size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_iters; j += (size_t)gridDim.x * blockDim.x)
-        v[4] += iters[j];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const fi_poi& p = cur[i];
-        const float z = p.result[2];
-        v[0] += (finite_f(z) && z >= 0.f) ? 1u : 0u;
-        v[1] += is_bad(p) ? 1u : 0u;
-        const float it = p.result[3];
-        if (finite_f(it)) {
-            v[2] += 1u;
-            v[3] += (unsigned)(long long)it;
-            mx[0] = max(mx[0], (unsigned)(int)(it + 0.5f));
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t j = t; j < n_iters; j += 4 * stride) {
+        uint32_t a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) a[u] = j + u * stride < n_iters ? iters[j + u * stride] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[4] += a[u];
+    }
+    for (size_t i0 = t; i0 < (size_t)n; i0 += 4 * stride) {
+        float2 r[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t i = i0 + u * stride;
+            // result[2] and result[3]: zncc and the iteration count
+            r[u] = i < (size_t)n ? make_float2(cur[i].result[2], cur[i].result[3]) : make_float2(-1.f, NAN);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            if (i0 + u * stride >= (size_t)n) continue;
+            const float z = r[u].x, it = r[u].y;
+            v[0] += (finite_f(z) && z >= 0.f) ? 1u : 0u;
+            v[1] += is_bad_zncc(z) ? 1u : 0u;
+            if (finite_f(it)) {
+                v[2] += 1u;
+                v[3] += (unsigned)(long long)it;
+                mx[0] = max(mx[0], (unsigned)(int)(it + 0.5f));
+            }
         }
     }
-    block_reduce<5>(v, mx);
+    block_reduce<5, kQaThreads>(v, mx);
     if (threadIdx.x == 0) {
         if (v[4]) atomicAdd(&counters->iterations, (unsigned long long)v[4]);
         if (v[0]) atomicAdd(&counters->valid, (unsigned long long)v[0]);
@@ -458,20 +477,20 @@ void launch_merge_frame_fallback(const fi_poi* skip, fi_poi* scratch_inout, int 
 }
 
 void launch_merge_repair(fi_poi* cur, const fi_poi* repaired, const uint32_t* idx, int n,
-                         FrameCounters* counters, hipStream_t s, const unsigned* count)
+                         FrameCounters* counters, hipStream_t s, const unsigned* count, const uint32_t* seeded)
 {
     if (n > 0)
         hipLaunchKernelGGL(k_merge_repair, grid_for(n), dim3(256), 0, s, cur, repaired, idx, n,
-                           counters, count);
+                           counters, count, seeded);
 }
 
 void launch_qa(const fi_poi* cur, int n, FrameCounters* counters, hipStream_t s, const uint32_t* iters,
                size_t n_iters)
 {
     const size_t m = std::max((size_t)n, n_iters);
-    hipLaunchKernelGGL(k_qa, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + kPolicyThreads - 1) / kPolicyThreads,
+    hipLaunchKernelGGL(k_qa, dim3((unsigned)std::max<size_t>(1, std::min<size_t>((m + kQaThreads - 1) / kQaThreads,
                                                                                     kReduceBlocks))),
-                       dim3(kPolicyThreads), 0, s, cur, n, counters, iters, n_iters);
+                       dim3(kQaThreads), 0, s, cur, n, counters, iters, n_iters);
 }
 
 

@@ -6,9 +6,11 @@
 Data: block-mosaic speckle pairs (one affine map per 64x64 block, u, v ~
 U[-8, 8] px, gradients ~ U[-0.05, 0.05]), generated by a pool of CPU workers
 and labelled by ICGN2D1 on the device (as seeder.make_training_set); the
-target crop sits at guess 0 for half of the samples and at the rounded truth
-moved by U[-1, 1] px for the other half.  5 % of the samples are held out and
-the prediction error on them is reported for both halves.  Training
+target crop sits at the rounded truth moved by U[-1, 1] px for a fraction
+--near of the samples (default 0.9: the regime of the second seeding pass and
+of every POI a neighbourhood consensus has already pulled close) and at guess
+0 for the rest.  5 % of the samples are held out and the prediction error on
+them is reported for both groups.  Training
 runs entirely in the BF16-MFMA kernels (fi_seeder_train_epoch).
 """
 from __future__ import annotations
@@ -83,7 +85,7 @@ def main() -> int:
     ap.add_argument("--workers", type=int, default=min(12, os.cpu_count() or 1))
     ap.add_argument("--out", default=str(Path(__file__).resolve().parent / "seeder_weights.npy"))
     ap.add_argument("--report", default="")
-    ap.add_argument("--near", type=float, default=0.5, help="fraction of crops near the truth")
+    ap.add_argument("--near", type=float, default=0.9, help="fraction of crops near the truth")
     args = ap.parse_args()
 
     def log(msg):
@@ -120,7 +122,8 @@ def main() -> int:
               "holdout_rmse_u_v_px": [float(rmse[0]), float(rmse[1])],
               "holdout_rmse_grad": [float(v / sd.GRAD_SCALE) for v in rmse[2:]],
               "holdout_p99_abs_u_px": float(np.percentile(np.abs(err[:, 0]), 99)),
-              "holdout_near": split(hn), "holdout_far": split(~hn)}
+              "holdout_near": split(hn), "holdout_far": split(~hn),
+              "command": "python -m fasticgn_amd.seeder_train " + " ".join(sys.argv[1:])}
     log("[seeder] " + json.dumps(report))
     np.save(args.out, params)
     if args.report:

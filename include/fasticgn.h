@@ -441,8 +441,10 @@ int fi_seeder_seed_consensus(fi_seeder* h, const float* ref, const float* tar, i
 
 /* Frame driver with the learned seeder (policy FI_POLICY_LEARNED_SEED): the
  * FFTCC seeding of a frame is replaced by the seeder; POIs whose IC-GN result
- * has ZNCC below zncc_threshold (or failed, or could not be seeded) are re-run
- * from FFTCC and merged (better ZNCC wins).  The seeder must outlive its use. */
+ * has ZNCC below zncc_threshold (or failed) are re-run from FFTCC and merged
+ * (better ZNCC wins); a POI the seeder could not seed takes the FFTCC re-run
+ * as it is, so its record is the FFTCC-seeded frame's.  The seeder must
+ * outlive its use. */
 #define FI_POLICY_LEARNED_SEED 4u
 int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold);
 /* Seeder passes per frame (1..4, default 1): pass k crops the target at the

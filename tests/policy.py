@@ -157,7 +157,8 @@ class PolicyOracle:
         """The learned-seed frame (FI_POLICY_LEARNED_SEED) from given seeds:
         IC-GN from the seeds; POIs that could not be seeded (ok false) or
         whose result is not a finite zncc >= threshold are redone from the
-        template by FFTCC + IC-GN and replaced under
+        template by FFTCC + IC-GN.  An unseeded POI takes the redo as it is
+        (its record is the FFTCC-seeded path's); the others are replaced under
         shouldReplacePoiWithFallback (better zncc wins, cli:505-517)."""
         tar = self.o.as_f32(tar_img, self.depth12)
         q = seeds.copy()
@@ -167,7 +168,7 @@ class PolicyOracle:
         if len(cand):
             fb = self.template[cand].copy()
             self._fftcc_icgn(tar, fb, cand)
-            rep = should_replace(q[cand], fb)
+            rep = should_replace(q[cand], fb) | ~ok[cand]
             q[cand[rep]] = fb[rep]
         return q, {"seed_fallback_count": len(cand)}
 

@@ -546,7 +546,9 @@ def test_learned_seed_frame_parity(oracle):
 def test_learned_seed_bad_seeds_fall_back_to_fftcc(oracle, c0):
     """A seeder whose u output is off by 40 px: every IC-GN from its seeds
     fails or lands on a low ZNCC, FFTCC redoes those POIs, and the frame equals
-    the plain FFTCC-seeded frame wherever the fallback result won."""
+    the plain FFTCC-seeded frame except where the redo is out of bounds (ZNCC
+    -3, which never replaces: shouldReplacePoiWithFallback).  The POIs the
+    consensus leaves unseeded take the redo as it is."""
     from fasticgn_amd import seeder as sd
     p = np.load(sd.WEIGHTS_PATH).copy()
     p[-6] += 40.0  # b5[0]: u seed + 40 px
@@ -559,6 +561,7 @@ def test_learned_seed_bad_seeds_fall_back_to_fftcc(oracle, c0):
     assert st["seed_fallback_count"] >= 0.95 * len(c0.pois)
     same = np.all(res["deformation"] == plain["deformation"], axis=1) & np.all(res["result"] == plain["result"], axis=1)
     assert same.mean() >= 0.95
+    assert np.all(plain["result"][~same, 2] == -3) and np.all(res["result"][~same, 2] == -3)
     cmp = compare(plain, res)
     assert cmp["pass"], cmp
 
