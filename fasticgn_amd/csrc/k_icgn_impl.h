@@ -458,9 +458,13 @@ __device__ __forceinline__ void wave_sum(float (&v)[6]) { wave_sum6(v); }
 
 // Block totals: wave totals combined through LDS in wave order with TwoSum,
 // so every wave holds the same bits.
+// lds holds WAVES * kBlockSumMax floats.
+constexpr int kBlockSumMax = 6;
+
 template <int WAVES, int N>
 __device__ __forceinline__ void block_sum(float (&v)[N], float* lds, int wave, int lane)
 {
+    static_assert(N <= kBlockSumMax, "the block's reduction buffer holds kBlockSumMax values per wave");
     wave_sum(v);
     if (WAVES == 1) return;
     if (lane == 0) {
@@ -667,6 +671,24 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
     const __amdgpu_buffer_rsrc_t table = table_rsrc(p.coef, (uint32_t)w * (uint32_t)h * 64u);
     const uint32_t row_bytes = (uint32_t)w * 64u, plane_bytes = (uint32_t)w * 16u;
 
+    // Shared tile (GROUP): an iteration samples it when its neighbourhood,
+    // columns max((int)(cx - ex - kTileSlack) - 1, 0) .. min((int)(cx + ex +
+    // kTileSlack) + 2, w - 1) and the rows alike, lies in the tile's columns
+    // tx0 .. tx0 + tile_w - 1 and rows.  With c = cx - ex >= 0 (samples in
+    // range) the truncation is the floor and c - kTileSlack is exact (c <
+    // 2^20), so the first column test is c >= tx0 + 1 + kTileSlack, or always
+    // for tx0 = 0; the last, c' = cx + ex < tx0 + tile_w - 2 - kTileSlack, or
+    // always for a tile reaching the image's last column.
+    float g_lox = INFINITY, g_hix = -INFINITY, g_loy = INFINITY, g_hiy = -INFINITY;
+    if constexpr (GROUP) {
+        if (tile_w > 0) {
+            g_lox = tx0 > 0 ? (float)tx0 + (1.f + kTileSlack) : -INFINITY;
+            g_loy = ty0 > 0 ? (float)ty0 + (1.f + kTileSlack) : -INFINITY;
+            g_hix = tx0 + tile_w >= w ? INFINITY : (float)(tx0 + tile_w) - (2.f + kTileSlack);
+            g_hiy = ty0 + tile_h >= h ? INFINITY : (float)(ty0 + tile_h) - (2.f + kTileSlack);
+        }
+    }
+
     int it = 0;
     float zncc = 0.f;
     double cn;
@@ -710,16 +732,18 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
             }
         }
 
-        // neighbourhood columns/rows this iteration needs (within the image:
-        // every sample passed the range test)
-        const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
-        const int nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
-        const int ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
-        const int ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
         bool use_tile;
         if constexpr (GROUP) {
-            use_tile = tile_w > 0 && nx0 >= tx0 && nx1 < tx0 + tile_w && ny0 >= ty0 && ny1 < ty0 + tile_h;
+            // the neighbourhood test below against the fixed shared tile, as
+            // four compares with the bounds formed before the loop
+            use_tile = (cx - ex >= g_lox) & (cx + ex < g_hix) & (cy - ey >= g_loy) & (cy + ey < g_hiy);
         } else {
+            // neighbourhood columns/rows this iteration needs (within the
+            // image: every sample passed the range test)
+            const int nx0 = max((int)(cx - ex - kTileSlack) - 1, 0);
+            const int nx1 = min((int)(cx + ex + kTileSlack) + 2, w - 1);
+            const int ny0 = max((int)(cy - ey - kTileSlack) - 1, 0);
+            const int ny1 = min((int)(cy + ey + kTileSlack) + 2, h - 1);
             use_tile = tile_w > 0 && nx1 - nx0 < tile_w && ny1 - ny0 < tile_h;
             if (use_tile && !(nx0 >= tx0 && nx1 < tx0 + tile_w && ny0 >= ty0 && ny1 < ty0 + tile_h)) {
                 // centre the tile on the neighbourhood, inside the image
@@ -747,7 +771,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
                     const float tp = sample(gxp, gyp, k);
                     const float tpm = (k < kMaxOverlap) ? tp * ((k < ovl) ? m_dup : 1.f) : tp;
                     tpv[k] = tp;
-                    st += tpm;
+                    // (a sum's first term is assigned: 0 + x is not folded)
+                    st = (EXACT && k == 0) ? tpm : st + tpm;
                     stt = __builtin_fmaf(tpm, tp, stt);
                     str = __builtin_fmaf(tp, rz[k], str);
                 }
@@ -789,8 +814,8 @@ __device__ __forceinline__ void icgn_entry(const IcgnParams& p, const int b, flo
         for (int k = SPL - 1; k >= 0; k--) {
             if (EXACT || k < R) {
                 const float e = __builtin_fmaf(scale, tpv[k] - tm, -rz[k]);
-                AK += A;
-                BK += B;
+                AK = (EXACT && k == SPL - 2) ? A : AK + A;
+                BK = (EXACT && k == SPL - 2) ? B : BK + B;
                 A = __builtin_fmaf(gxv[k], e, A);
                 B = __builtin_fmaf(gyv[k], e, B);
             }
@@ -896,7 +921,7 @@ template <int WAVES, int SPL, bool EXACT, bool OFFSET = false>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
 void k_icgn(IcgnParams p)
 {
-    __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
+    __shared__ float red[WAVES > 1 ? WAVES * kBlockSumMax : 1];
     __shared__ int flags[WAVES];
     __shared__ float hl[WAVES * 36];
     extern __shared__ float tile[];
@@ -910,7 +935,7 @@ template <int WAVES, int SPL, bool EXACT>
 __global__ __launch_bounds__(64 * WAVES) __attribute__((amdgpu_waves_per_eu(FI_ICGN_MIN_WAVES)))
 void k_icgn_queue(IcgnParams p)
 {
-    __shared__ float red[WAVES > 1 ? WAVES * 6 : 1];
+    __shared__ float red[WAVES > 1 ? WAVES * kBlockSumMax : 1];
     __shared__ int flags[WAVES];
     __shared__ float hl[WAVES * 36];
     // claim[0]: the entry the block runs next, -1 once every range is
