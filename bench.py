@@ -11,7 +11,10 @@ previous frame runs) to the frame's output records (36 B per POI, packed on
 the device) resident in host memory, their copy overlapping the next frame;
 the seeding (FFTCC, or the learned seeder with its FFTCC fallback), IC-GN, the
 fallback repair and the frame QA run in between, replayed as a captured
-hipGraph.  Both seeding modes are timed.
+hipGraph, each frame launched behind the one before (fi_frame_run_async) so
+the device runs frames back to back while the host waits for one frame's
+statistics and records and queues the next target.  Both seeding modes are
+timed.
 
 Prints one JSON line.  value = converged POIs per second (zncc finite >= 0)
 of the learned-seed path when it meets the parity gate against the CPU oracle
@@ -450,27 +453,31 @@ def main() -> int:
 
     def t_frame(policy: int, label: str) -> dict:
         """Frames timed from target upload to records in host memory, graph
-        replay; the records of frame k are read (host memory) after frame k+1
-        is launched, their copy having overlapped it."""
+        replay.  Frame k is launched with run_async, queued on the device
+        behind frame k-1; then frame k-1's statistics are taken (wait) and
+        its records read in host memory (their copy overlapped frame k), and
+        target k+1 is queued (staged while frame k runs)."""
         drv.set_graph(True)
         drv.set_records(True)
-        drv.upload_target(tar_img)
-        for _ in range(args.warmup):
+
+        def frames(steps: int, stats: list, checks: list):
             drv.upload_target(tar_img)
-            drv.run(policy=policy)
-        drv.run(policy=policy)  # drain: no target queued
-        drv.records(0, copy=False)
+            for k in range(steps):
+                drv.run_async(policy=policy)
+                if k > 0:
+                    stats.append(drv.wait())
+                    checks.append(int(drv.records(1, copy=False)["zncc"][::4099].view(np.uint32).sum()))
+                if k + 1 < steps:
+                    drv.upload_target(tar_img)
+            stats.append(drv.wait())
+            return drv.records(0, copy=False)
+
+        # warm-up: the second buffer set, its graphs and the staging path
+        frames(max(args.warmup, 3), [], [])
         sync()
         stats, checks = [], []
         t0 = time.perf_counter()
-        drv.upload_target(tar_img)
-        for k in range(args.steps):
-            if k + 1 < args.steps:
-                drv.upload_target(tar_img)  # the next frame's target, staged while this one runs
-            stats.append(drv.run(policy=policy))
-            if k > 0:
-                checks.append(int(drv.records(1, copy=False)["zncc"][::4099].view(np.uint32).sum()))
-        last = drv.records(0, copy=False)
+        last = frames(args.steps, stats, checks)
         wall = time.perf_counter() - t0
         ms = wall * 1e3 / args.steps
         conv = float(np.mean([s["valid_count"] for s in stats]))

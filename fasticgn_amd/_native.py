@@ -181,6 +181,8 @@ SIGNATURES = {
     "fi_frame_set_pois": [_P, _P, _S],
     "fi_frame_upload_target": [_P, _P, _I, _I, _S, _I, _I],
     "fi_frame_run": [_P, _U, ctypes.POINTER(FrameStats)],
+    "fi_frame_run_async": [_P, _U],
+    "fi_frame_wait": [_P, ctypes.POINTER(FrameStats)],
     "fi_frame_download": [_P, _P, _S],
     "fi_frame_reset_sequence": [_P],
     "fi_frame_set_shape": [_P, _I, _I],

@@ -312,12 +312,14 @@ int fi_frame_destroy(fi_frame* h)
 
 int fi_frame_set_ref(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
 {
-    return guarded([&] { need(h); h->set_ref(px, w, hh, stride, pixel, mode); });
+    return guarded([&] { need(h);
+        h->drain(); h->set_ref(px, w, hh, stride, pixel, mode); });
 }
 
 int fi_frame_set_pois(fi_frame* h, const fi_poi* pois, size_t n)
 {
-    return guarded([&] { need(h); h->set_pois(pois, n); });
+    return guarded([&] { need(h);
+        h->drain(); h->set_pois(pois, n); });
 }
 
 int fi_frame_upload_target(fi_frame* h, const void* px, int w, int hh, size_t stride, int pixel, int mode)
@@ -330,14 +332,26 @@ int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats)
     return guarded([&] { need(h); h->run(policy, stats); });
 }
 
+int fi_frame_run_async(fi_frame* h, unsigned policy)
+{
+    return guarded([&] { need(h); h->run_async(policy); });
+}
+
+int fi_frame_wait(fi_frame* h, fi_frame_stats* stats)
+{
+    return guarded([&] { need(h); h->wait(stats); });
+}
+
 int fi_frame_download(fi_frame* h, fi_poi* out, size_t n)
 {
-    return guarded([&] { need(h); h->download(out, n); });
+    return guarded([&] { need(h);
+        h->drain(); h->download(out, n); });
 }
 
 int fi_frame_download_redo(fi_frame* h, uint8_t* out, size_t n)
 {
-    return guarded([&] { need(h); h->download_redo(out, n); });
+    return guarded([&] { need(h);
+        h->drain(); h->download_redo(out, n); });
 }
 
 int fi_frame_records(fi_frame* h, int age, const void** records, size_t* n_records)
@@ -347,13 +361,15 @@ int fi_frame_records(fi_frame* h, int age, const void** records, size_t* n_recor
 
 int fi_frame_set_records(fi_frame* h, int on)
 {
-    return guarded([&] { need(h); h->set_records(on != 0); });
+    return guarded([&] { need(h);
+        h->drain(); h->set_records(on != 0); });
 }
 
 int fi_frame_set_temporal_extrapolation(fi_frame* h, int on, float max_delta)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         if (!(max_delta >= 0.f)) fail(FI_E_INVALID_ARGUMENT, "temporal max_delta must be >= 0 (0 = the radius)");
         h->temporal = on != 0;
         h->temporal_max_delta = max_delta;
@@ -365,6 +381,7 @@ int fi_frame_prepare(fi_frame* h, fi_prepare_stats* stats)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         const fi_prepare_stats ps = h->prepare_now();
         if (stats) *stats = ps;
     });
@@ -372,13 +389,15 @@ int fi_frame_prepare(fi_frame* h, fi_prepare_stats* stats)
 
 int fi_frame_reset_sequence(fi_frame* h)
 {
-    return guarded([&] { need(h); h->reset_sequence(); });
+    return guarded([&] { need(h);
+        h->drain(); h->reset_sequence(); });
 }
 
 int fi_frame_set_shape(fi_frame* h, int waves, int spl)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         check_shape(waves, spl);
         if (waves && !icgn_shape_covers(waves, spl, h->r, h->r))
             fail(FI_E_INVALID_ARGUMENT, "IC-GN shape " + std::to_string(waves) + "x" + std::to_string(spl) +
@@ -397,6 +416,7 @@ int fi_frame_set_sampling(fi_frame* h, int mode)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         check_sampling(mode);
         h->sampling = mode;
         for (auto& c : h->classes) c->sampling = -1;
@@ -408,6 +428,7 @@ int fi_frame_set_self_adaptive(fi_frame* h, int on)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         h->set_self_adaptive(on != 0);
     });
 }
@@ -416,6 +437,7 @@ int fi_frame_set_persistent(fi_frame* h, int blocks)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         if (blocks < 0) fail(FI_E_INVALID_ARGUMENT, "persistent block count must be >= 0");
         h->persistent_blocks = blocks;
         for (auto& c : h->classes) c->queue_blocks = -1;
@@ -427,6 +449,7 @@ int fi_frame_set_fused(fi_frame* h, int on)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         h->fuse = on != 0;
         h->fused.waves = h->fused.spl = 0;
         h->fused.sampling = -1;
@@ -439,6 +462,7 @@ int fi_frame_set_seed_passes(fi_frame* h, int passes)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         if (passes < 1 || passes > 4) fail(FI_E_INVALID_ARGUMENT, "seeder passes must be 1..4");
         h->seed_passes = passes;
         h->invalidate_graphs();
@@ -455,6 +479,7 @@ int fi_frame_set_seed_consensus(fi_frame* h, int on, float tau, int sample)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         if (!(tau > 0.f)) fail(FI_E_INVALID_ARGUMENT, "seed consensus tau must be > 0");
         check_seed_sample(sample);
         h->consensus = on != 0;
@@ -470,6 +495,7 @@ int fi_frame_set_graph(fi_frame* h, int on)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         h->use_graph = on != 0;
         h->invalidate_graphs();
     });
@@ -479,6 +505,7 @@ int fi_frame_autotune(fi_frame* h, fi_tune_entry* report, size_t report_len, siz
 {
     return guarded([&] {
         need(h);
+        h->drain();
         if (report == nullptr && report_len > 0) fail(FI_E_INVALID_ARGUMENT, "report pointer is null");
         size_t m = 0;
         h->autotune(report, report_len, &m);
@@ -490,6 +517,7 @@ int fi_frame_set_seeder(fi_frame* h, fi_seeder* seeder, float zncc_threshold)
 {
     return guarded([&] {
         need(h);
+        h->drain();
         if (!(zncc_threshold <= 1.f)) fail(FI_E_INVALID_ARGUMENT, "seed ZNCC threshold must be <= 1");
         if (seeder && seeder->device != h->device)
             fail(FI_E_INVALID_ARGUMENT, "seeder and frame driver are on different devices");

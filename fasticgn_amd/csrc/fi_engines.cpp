@@ -581,7 +581,12 @@ fi_frame::fi_frame(int radius, float conv_, float stop_, int dev)
     FI_HIP(hipEventCreateWithFlags(&tmpl_ev, hipEventDisableTiming));
     FI_HIP(hipEventCreate(&join_ev));
     for (auto& e : ev) FI_HIP(hipEventCreate(&e));
-    for (auto& t : tslot) FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+    for (auto& t : tslot) {
+        FI_HIP(hipEventCreateWithFlags(&t.ready, hipEventDisableTiming));
+        FI_HIP(hipEventCreateWithFlags(&t.freed, hipEventDisableTiming));
+    }
+    for (auto& pair : fev)
+        for (auto& e : pair) FI_HIP(hipEventCreate(&e));
     rec_stream = pool_stream();
     for (auto& e : rec_ev) FI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     FI_HIP(hipEventCreateWithFlags(&pack_ev, hipEventDisableTiming));
@@ -600,8 +605,13 @@ fi_frame::~fi_frame()
     for (auto& gr : graphs) (void)hipGraphExecDestroy(gr.exec);
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
-    for (auto& t : tslot)
+    for (auto& t : tslot) {
         if (t.ready) (void)hipEventDestroy(t.ready);
+        if (t.freed) (void)hipEventDestroy(t.freed);
+    }
+    for (auto& pair : fev)
+        for (auto& e : pair)
+            if (e) (void)hipEventDestroy(e);
     for (auto& e : rec_ev)
         if (e) (void)hipEventDestroy(e);
     if (pack_ev) (void)hipEventDestroy(pack_ev);
@@ -625,10 +635,11 @@ fi_frame::~fi_frame()
 }
 
 // Staging thread: packs a queued target into its slot's pinned buffer and
-// starts the DMA on copy_stream.  The frame that last read the slot has
-// finished: run() and autotune() return only after their frame completes, and
-// at most two targets are queued, so a slot is refilled only after the run
-// that took it returned.
+// starts the DMA on copy_stream.  upload_target queues a slot only after the
+// last frame that took it has finished (its `freed` event: frames launched by
+// run_async may still be running).  The wait is the caller's, not this
+// thread's: a host wait on a compute-stream event here could meet a frame
+// graph being captured on that stream.
 void fi_frame::stage_loop()
 {
     for (;;) {
@@ -745,6 +756,8 @@ void fi_frame::upload_target(const void* px, int w, int h, size_t stride, int pi
     // (reported by the run that takes this target)
     if (pending.size() >= 2) fail(FI_E_LOGIC, "two targets are already queued; run a frame first");
     const int s = next_slot;
+    // the slot's last frame (possibly still in flight) has read it
+    FI_HIP(hipEventSynchronize(tslot[s].freed));
     next_slot ^= 1;
     {
         std::unique_lock<std::mutex> lk(stage_mu);
@@ -967,7 +980,7 @@ void fi_frame::fftcc_pass(const FrameClass& c, fi_poi* queue_base, int count, co
 }
 
 void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, const uint32_t* prep_index,
-                         const unsigned* count_dev, int region)
+                         const unsigned* count_dev, int region, int first)
 {
     if (count <= 0) return;
     const int mode = c.sampling >= 0 ? c.sampling : sampling;
@@ -990,7 +1003,7 @@ void fi_frame::icgn_pass(const FrameClass& c, fi_poi* queue_base, int count, con
     ip.ry = c.r;
     ip.conv = conv;
     ip.stop = stop;
-    ip.iters = iters.p + (size_t)region * n + c.off;
+    ip.iters = iters.p + (size_t)region * n + c.off + first;
     int waves = c.waves, spl = c.spl;
     if (waves == 0 && shape_waves && icgn_shape_covers(shape_waves, shape_spl, c.r, c.r)) {
         waves = shape_waves;
@@ -1080,7 +1093,8 @@ void fi_frame::fftcc_all(fi_poi* base)
 // wins), redo flags.  With count_dev (lo = 0) the candidate count is read on
 // the device and m is only the capacity the launches are sized for, so the
 // repair can sit in a captured frame graph.
-void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* count_dev, const uint32_t* seeded)
+void fi_frame::repair_class(fi_poi* out, const FrameClass& c, int m, int lo, const unsigned* count_dev,
+                            const uint32_t* seeded)
 {
     const int k = m - lo;
     if (k <= 0) return;
@@ -1090,8 +1104,8 @@ void fi_frame::repair_class(const FrameClass& c, int m, int lo, const unsigned* 
     fi_poi* rep = repair.p + c.off + lo;
     launch_gather(tmpl.p + c.off, idx, k, rep, stream, count_dev, redo.p + c.off);
     fftcc_pass(c, rep, k, idx, count_dev);
-    icgn_pass(c, rep, k, idx, count_dev, 1);
-    launch_merge_repair(cur.p + c.off, rep, idx, k, counters.p + ci, stream, count_dev,
+    icgn_pass(c, rep, k, idx, count_dev, 1, lo);
+    launch_merge_repair(out + c.off, rep, idx, k, counters.p + ci, stream, count_dev,
                         seeded ? seeded + c.off : nullptr);
     FI_HIP(hipGetLastError());
 }
@@ -1147,7 +1161,66 @@ bool fi_frame::table_needed() const
 void fi_frame::run(unsigned policy, fi_frame_stats* stats)
 {
     DeviceGuard g(device);
-    if (!prepared) prepare();
+    if (!inflight.empty() || !done.empty())
+        fail(FI_E_LOGIC, "frames launched by fi_frame_run_async are waiting for fi_frame_wait");
+    launch(policy);
+    complete_oldest();
+    if (stats) *stats = done.front();
+    done.pop_front();
+}
+
+void fi_frame::run_async(unsigned policy)
+{
+    DeviceGuard g(device);
+    launch(policy);
+}
+
+void fi_frame::wait(fi_frame_stats* stats)
+{
+    DeviceGuard g(device);
+    if (done.empty()) {
+        if (inflight.empty()) fail(FI_E_LOGIC, "no frame launched by fi_frame_run_async is waiting");
+        complete_oldest();
+    }
+    if (stats) *stats = done.front();
+    done.pop_front();
+}
+
+void fi_frame::drain()
+{
+    DeviceGuard g(device);
+    while (!inflight.empty()) complete_oldest();
+}
+
+// The per-frame buffers and their twins trade places (set 0 <-> set 1).
+void fi_frame::swap_scratch()
+{
+    repair.swap(repair_alt);
+    cand.swap(cand_alt);
+    seed_ok.swap(seed_ok_alt);
+    iters.swap(iters_alt);
+    redo.swap(redo_alt);
+    counters.swap(counters_alt);
+    h_counters.swap(h_counters_alt);
+    tar_f.swap(tar_f_alt);
+    scratch_set ^= 1;
+}
+
+// Launches one frame's device work (processFrame, below) and returns without
+// waiting for it.  A frame is launched behind one still running only when
+// neither needs the other's completion on the host: both replayed as graphs,
+// not a temporal sequence (whose seeds and fallback baseline come from the
+// previous frame's completion), no B-spline table (one table buffer) and no
+// host-built seed cells pending.  Such a frame runs on the other set of
+// per-frame buffers, so the earlier frame's late repairs (candidates past
+// repair_cap, run on the host's reading of its counters) still find its own
+// candidates, flags, iteration counts and converted target.
+void fi_frame::launch(unsigned policy)
+{
+    if (!prepared) {
+        drain();
+        prepare();
+    }
     if (!tar_ready) fail(FI_E_LOGIC, "frame driver needs an uploaded target");
     const int w = ref_raw.view.width, h = ref_raw.view.height;
     const int cnt = (int)n;
@@ -1158,6 +1231,29 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     st.fallback_threshold = NAN;
     const bool table = table_needed();
     prep_target(table);
+    const bool seeds_learned = (policy & FI_POLICY_LEARNED_SEED) && seeder != nullptr;
+    const bool overlap = use_graph && !table && !(policy & FI_POLICY_SKIP_FFTCC) &&
+                         !(seeds_learned && consensus && !cells_ready);
+    if (overlap && !(repair_alt.cap >= repair.cap && cand_alt.cap >= cand.cap && seed_ok_alt.cap >= seed_ok.cap &&
+                     iters_alt.cap >= iters.cap && redo_alt.cap >= redo.cap && counters_alt.cap >= counters.cap &&
+                     h_counters_alt.cap >= h_counters.cap && tar_f_alt.cap >= tar_f.cap)) {
+        // the twin set sized like the first: with nothing in flight, and the
+        // graphs captured on the old blocks dropped
+        drain();
+        repair_alt.reserve(repair.cap);
+        cand_alt.reserve(cand.cap);
+        seed_ok_alt.reserve(seed_ok.cap);
+        iters_alt.reserve(iters.cap);
+        redo_alt.reserve(redo.cap);
+        counters_alt.reserve(counters.cap);
+        h_counters_alt.reserve(h_counters.cap);
+        tar_f_alt.reserve(tar_f.cap);
+        invalidate_graphs();
+    }
+    while (!inflight.empty() && (!overlap || inflight.size() >= 2 || !inflight.front().graph)) complete_oldest();
+    // the set no frame in flight uses
+    const int set = inflight.empty() ? scratch_set : (inflight.back().set ^ 1);
+    if (set != scratch_set) swap_scratch();
     const int ts = take_target(true);
     const DevImage tview = target(ts);
     FI_HIP(hipStreamWaitEvent(stream, tslot[ts].ready, 0));
@@ -1267,7 +1363,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         if (mode == 0 || (mode == 2 && collect)) {
             for (int ci = 0; ci < C; ci++)
                 if (classes[ci]->n > 0)
-                    repair_class(*classes[ci], repair_cap(*classes[ci]), 0,
+                    repair_class(cur.p, *classes[ci], repair_cap(*classes[ci]), 0,
                                  mode == 0 ? &counters.p[ci].candidates : &counters.p[ci].repair_n,
                                  mode == 0 ? seed_ok.p : nullptr);
         }
@@ -1296,7 +1392,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     };
 
     if (use_graph) {
-        const uint64_t key[11] = { (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
+        const uint64_t key[12] = { (uint64_t)(uintptr_t)cand.p, (uint64_t)mode, (uint64_t)collect, (uint64_t)(uintptr_t)cur.p,
                                    (uint64_t)(uintptr_t)prev.p, (uint64_t)(uintptr_t)tview.data,
                                    (uint64_t)(uintptr_t)(table ? coef.p : nullptr), (uint64_t)(uintptr_t)tar_f.p,
                                    (uint64_t)(uintptr_t)seeder, (uint64_t)(uintptr_t)hist2,
@@ -1306,6 +1402,8 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         for (auto& gr : graphs)
             if (std::memcmp(gr.key, key, sizeof(key)) == 0) exec = gr.exec;
         if (!exec) {
+            // a new capture behind an idle stream
+            drain();
             FI_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
             try {
                 segment(false);
@@ -1325,14 +1423,70 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             gr.exec = exec;
             graphs.push_back(gr);
         }
-        FI_HIP(hipEventRecord(ev[0], stream));
+        FI_HIP(hipEventRecord(fev[scratch_set][0], stream));
         FI_HIP(hipGraphLaunch(exec, stream));
+        FI_HIP(hipEventRecord(fev[scratch_set][1], stream));
     } else {
         segment(true);
+        FI_HIP(hipEventRecord(ev[4], stream));
     }
-    FI_HIP(hipEventRecord(ev[4], stream));
-    if (rs >= 0) copy_records(rs);
-    FI_HIP(hipStreamSynchronize(stream));
+    if (rs >= 0) copy_records(rs, frame_index);
+    // the staging thread refills this target slot only after this frame
+    FI_HIP(hipEventRecord(tslot[ts].freed, stream));
+    InFlight f;
+    f.st = st;
+    f.policy = policy;
+    f.mode = mode;
+    f.rs = rs;
+    f.ts = ts;
+    f.set = scratch_set;
+    f.index = frame_index;
+    f.collect = collect != 0;
+    f.skip = skip;
+    f.table = table;
+    f.graph = use_graph;
+    f.out = cur.p;
+    inflight.push_back(f);
+    last_slot = ts;
+
+    // history: prev2 <- prev, prev <- this frame, cur <- the oldest buffer
+    std::swap(prev2.p, prev.p);
+    std::swap(prev2.cap, prev.cap);
+    std::swap(cur.p, prev.p);
+    std::swap(cur.cap, prev.cap);
+    have_prev2 = have_prev;
+    have_prev = true;
+    frame_index++;
+}
+
+// The host's part of the oldest frame in flight once its device work has
+// finished: statistics from the counters it copied back, repairs of the
+// candidates past repair_cap (with the frame's buffer set as the members),
+// QA and records again after them.
+void fi_frame::complete_oldest()
+{
+    const InFlight f = inflight.front();
+    inflight.pop_front();
+    const bool other = f.set != scratch_set;
+    if (other) swap_scratch();
+    try {
+        finish(f);
+    } catch (...) {
+        if (other) swap_scratch();
+        throw;
+    }
+    if (other) swap_scratch();
+}
+
+void fi_frame::finish(const InFlight& f)
+{
+    FI_HIP(hipEventSynchronize(f.graph ? fev[f.set][1] : ev[4]));
+    const int cnt = (int)n;
+    const int C = (int)nclass();
+    const int mode = f.mode, rs = f.rs, collect = f.collect ? 1 : 0;
+    const bool skip = f.skip, table = f.table;
+    const unsigned policy = f.policy;
+    fi_frame_stats st = f.st;
     std::vector<FrameCounters> pc(h_counters.p, h_counters.p + C + 1);
     FrameCounters qa = pc[C];
     bool redo_qa = false;
@@ -1346,7 +1500,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         for (int ci = 0; ci < C; ci++) {
             const int m = (int)pc[ci].candidates, cap = repair_cap(*classes[ci]);
             if (m > cap) {
-                repair_class(*classes[ci], m, cap, nullptr, seed_ok.p);
+                repair_class(f.out, *classes[ci], m, cap, nullptr, seed_ok.p);
                 redo_qa = true;
             }
         }
@@ -1376,7 +1530,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
             for (int ci = 0; ci < C; ci++) {
                 const int m = (int)pc[ci].candidates, cap = repair_cap(*classes[ci]);
                 if (m > cap) {
-                    repair_class(*classes[ci], m, cap);
+                    repair_class(f.out, *classes[ci], m, cap);
                     redo_qa = true;
                 }
             }
@@ -1390,17 +1544,16 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     }
     if (redo_qa) {
         launch_zero_words(reinterpret_cast<uint32_t*>(counters.p + C), sizeof(FrameCounters) / 4, stream);
-        launch_qa(cur.p, cnt, counters.p + C, stream, iters.p, 3 * n);
+        launch_qa(f.out, cnt, counters.p + C, stream, iters.p, 3 * n);
         FI_HIP(hipGetLastError());
         qa = read_counters(nullptr);
         if (rs >= 0) {
             // repairs past the cap changed the frame: its records again, same slot
             FI_HIP(hipStreamWaitEvent(stream, rec_ev[rs], 0));
-            pack_records(cur.p, rs);
-            copy_records(rs);
+            pack_records(f.out, rs);
+            copy_records(rs, f.index);
         }
     }
-    last_slot = ts;
 
     const float bad_ratio = cnt ? (float)qa.bad / (float)cnt : 0.f;
     if (!skip) st.pre_repair_bad_ratio = bad_ratio;
@@ -1419,9 +1572,9 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
     st.iteration_mean = qa.iter_finite ? (float)((double)qa.iter_sum / (double)qa.iter_finite) : 0.f;
     st.iteration_max = (int32_t)qa.iter_max;
     float ms = 0.f;
-    FI_HIP(hipEventElapsedTime(&ms, ev[0], ev[4]));
+    FI_HIP(hipEventElapsedTime(&ms, f.graph ? fev[f.set][0] : ev[0], f.graph ? fev[f.set][1] : ev[4]));
     st.device_ms = ms;
-    if (use_graph) {
+    if (f.graph) {
         // stages inside a graph are not timed separately
         st.bspline_ms = st.fftcc_ms = st.icgn_ms = NAN;
         st.seed_ms = mode == 0 ? NAN : 0.0;
@@ -1437,16 +1590,7 @@ void fi_frame::run(unsigned policy, fi_frame_stats* stats)
         FI_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
         st.icgn_ms = ms;
     }
-
-    // history: prev2 <- prev, prev <- this frame, cur <- the oldest buffer
-    std::swap(prev2.p, prev.p);
-    std::swap(prev2.cap, prev.cap);
-    std::swap(cur.p, prev.p);
-    std::swap(cur.cap, prev.cap);
-    have_prev2 = have_prev;
-    have_prev = true;
-    frame_index++;
-    if (stats) *stats = st;
+    done.push_back(st);
 }
 
 // Packs the raw records of src (the frame just run, layout order) on the
@@ -1457,7 +1601,7 @@ void fi_frame::queue_records(const fi_poi* src)
 {
     const int s = records_slot();
     pack_records(src, s);
-    copy_records(s);
+    copy_records(s, frame_index);
 }
 
 // The next record slot, its buffers sized, the compute stream waiting for the
@@ -1481,7 +1625,7 @@ void fi_frame::pack_records(const fi_poi* src, int s)
 
 // the copy of slot s to host memory on rec_stream, after the work queued on
 // the compute stream so far
-void fi_frame::copy_records(int s)
+void fi_frame::copy_records(int s, int index)
 {
     const size_t bytes = n * (size_t)36;
     FI_HIP(hipGetLastError());
@@ -1489,7 +1633,7 @@ void fi_frame::copy_records(int s)
     FI_HIP(hipStreamWaitEvent(rec_stream, pack_ev, 0));
     if (bytes) FI_HIP(hipMemcpyAsync(h_rec[s].p, d_rec[s].p, bytes, hipMemcpyDeviceToHost, rec_stream));
     FI_HIP(hipEventRecord(rec_ev[s], rec_stream));
-    rec_held[s] = frame_index;
+    rec_held[s] = index;
 }
 
 void fi_frame::set_records(bool on)
@@ -1508,7 +1652,9 @@ void fi_frame::records(int age, const void** out, size_t* count)
     if (s < 0) {
         if (age != 0 || want < 0)
             fail(FI_E_LOGIC, "the records of that frame are not held (fi_frame_set_records keeps two frames)");
-        // not packed by its run: pack the latest results (prev after the rotation)
+        // not packed by its run: pack the latest results (prev after the
+        // rotation) once every frame in flight has completed
+        drain();
         const int keep = frame_index;
         frame_index = want;
         try {

@@ -144,6 +144,7 @@ struct fi_frame {
     struct TargetSlot {
         fi::DeviceImage img;
         hipEvent_t ready = nullptr;  // DMA done (copy_stream)
+        hipEvent_t freed = nullptr;  // the last frame that took the slot finished (compute stream)
         int state = 0;               // 0 idle, 1 staging, 2 staged
         int err_code = 0;
         std::string err;
@@ -207,8 +208,30 @@ struct fi_frame {
     fi_prepare_stats last_prepare{}; // timing of the last reference-side preparation
     int frame_index = 0;
     hipEvent_t ev[8] = {};
+    // Frames in flight (fi_frame_run_async): at most two, each with the set
+    // of per-frame buffers it was launched on (scratch_set, the members
+    // below or their *_alt twins, exchanged by swap_scratch), so a frame
+    // launched behind another leaves the first one's candidates, counters,
+    // iteration counts and converted target intact for its completion.
+    struct InFlight {
+        fi_frame_stats st;
+        unsigned policy;
+        int mode, rs, ts, set, index;
+        bool collect, skip, table, graph;
+        fi_poi* out;  // the frame's results
+    };
+    std::deque<InFlight> inflight;
+    std::deque<fi_frame_stats> done;  // completed, not yet returned by wait()
+    int scratch_set = 0;
+    hipEvent_t fev[2][2] = {};  // per buffer set: graph launch start, end
+    fi::DevArray<fi_poi> repair_alt;
+    fi::DevArray<uint32_t> cand_alt, seed_ok_alt, iters_alt;
+    fi::DevArray<uint8_t> redo_alt;
+    fi::DevArray<fi::FrameCounters> counters_alt;
+    fi::PinnedArray<fi::FrameCounters> h_counters_alt;
+    fi::DevArray<float> tar_f_alt;
     struct Graph {
-        uint64_t key[11];
+        uint64_t key[12];
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;     // instantiated frame graphs (cleared on any change)
@@ -220,6 +243,13 @@ struct fi_frame {
     void set_self_adaptive(bool on);
     void upload_target(const void* px, int w, int h, size_t stride, int pixel, int mode);
     void run(unsigned policy, fi_frame_stats* stats);
+    // run() split in two: launch a frame (behind one still running when both
+    // are independent frames replayed as graphs) and return; wait() returns
+    // the oldest launched frame's statistics once it has finished.
+    void run_async(unsigned policy);
+    void wait(fi_frame_stats* stats);
+    // completes every frame in flight (their statistics stay for wait())
+    void drain();
     void download(fi_poi* out, size_t n);
     // Raw records of the frame `age` runs back (0: the last) in pinned host
     // memory, valid until the second run after that frame; the last frame's
@@ -239,24 +269,30 @@ struct fi_frame {
 
 private:
     void prepare();
+    void launch(unsigned policy);
+    void complete_oldest();
+    void finish(const InFlight& f);
+    void swap_scratch();
     void queue_records(const fi_poi* src);
     int records_slot();
     void pack_records(const fi_poi* src, int s);
-    void copy_records(int s);
+    void copy_records(int s, int index);
     void fftcc_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* slot_dev,
                     const unsigned* count_dev = nullptr);
     // region: which third of `iters` the launch writes its per-entry
     // iteration counts to (0 first pass, 1 POI-level repair, 2 whole-frame
-    // fallback); k_qa sums all three
+    // fallback), from entry `first` of the class on (a repair's candidates
+    // past repair_cap follow the ones repaired inside the frame); k_qa sums
+    // all three
     void icgn_pass(const FrameClass& c, fi_poi* queue, int count, const uint32_t* prep_index,
-                   const unsigned* count_dev = nullptr, int region = 0);
+                   const unsigned* count_dev = nullptr, int region = 0, int first = 0);
     void icgn_fused(fi_poi* base, int region = 0);
     bool fused_shape(int& waves, int& spl) const;
     void icgn_all(fi_poi* base, int region = 0);
     void fftcc_all(fi_poi* base);
     // seeded: the learned seeder's flags (null for temporal repairs), see
     // launch_merge_repair
-    void repair_class(const FrameClass& c, int m, int lo = 0, const unsigned* count_dev = nullptr,
+    void repair_class(fi_poi* out, const FrameClass& c, int m, int lo = 0, const unsigned* count_dev = nullptr,
                       const uint32_t* seeded = nullptr);
     int repair_cap(const FrameClass& c) const;
     void prep_target(bool table);

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "fi_launch.h"
@@ -95,6 +96,14 @@ struct DevArray {
         cap = 0;
         bytes = 0;
     }
+    // exchanges the two blocks (ownership included)
+    void swap(DevArray& o) noexcept
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        std::swap(bytes, o.bytes);
+        std::swap(dev, o.dev);
+    }
 };
 
 // Pinned host staging buffer that only grows.
@@ -122,6 +131,13 @@ struct PinnedArray {
         p = nullptr;
         cap = 0;
         bytes = 0;
+    }
+    void swap(PinnedArray& o) noexcept
+    {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        std::swap(bytes, o.bytes);
+        std::swap(dev, o.dev);
     }
 };
 

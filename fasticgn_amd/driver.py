@@ -60,6 +60,21 @@ class FrameDriver(_Handle):
         nat.call("fi_frame_run", self._h, int(policy), ctypes.byref(st))
         return st.as_dict()
 
+    def run_async(self, policy: int = DEFAULT_POLICY) -> None:
+        """Launch a frame and return; wait() returns its statistics.  With
+        graphs on, a frame that is not seeded from the previous one is queued
+        on the device behind the frame still running (at most two in flight),
+        so the device does not idle while the host waits, reads records and
+        queues the next target."""
+        nat.call("fi_frame_run_async", self._h, int(policy))
+
+    def wait(self) -> dict:
+        """Statistics of the oldest frame launched by run_async, once it has
+        finished."""
+        st = FrameStats()
+        nat.call("fi_frame_wait", self._h, ctypes.byref(st))
+        return st.as_dict()
+
     def download(self) -> np.ndarray:
         out = np.zeros(self.n, dtype=POI_DTYPE)
         nat.call("fi_frame_download", self._h, nat.ptr(out), len(out))

@@ -303,6 +303,23 @@ int fi_frame_upload_target(fi_frame* h, const void* px, int width, int height,
  * shouldTakeFrameFallbackResult, cli:3086-3152) is decided and run on the
  * device, inside the frame's graph when graphs are on. */
 int fi_frame_run(fi_frame* h, unsigned policy, fi_frame_stats* stats);
+/* fi_frame_run in two halves, for a caller that keeps the device busy across
+ * the frame loop (processFrame per target, opencorr_cli_cuda_mvp.cpp:3024-3210,
+ * which the reference runs one frame after another): fi_frame_run_async
+ * launches the frame and returns; fi_frame_wait returns the oldest launched
+ * frame's statistics once it has finished (FI_E_LOGIC when none is waiting).
+ * A frame is queued on the device behind one still running when both are
+ * replayed as graphs (fi_frame_set_graph), are not seeded from the previous
+ * frame (no FI_POLICY_SKIP_FFTCC) and need no B-spline table; otherwise the
+ * launch first completes the frames in flight (their statistics wait for
+ * fi_frame_wait).  At most two frames are in flight.  Results, records and
+ * statistics are those of fi_frame_run frame for frame.  Every other frame
+ * call but fi_frame_upload_target and fi_frame_records completes the frames
+ * in flight first; fi_frame_run fails with FI_E_LOGIC while any launched frame
+ * has not been waited for.  Typical loop: run_async(k), wait(k - 1) and read
+ * its records (age 1), upload_target(k + 1). */
+int fi_frame_run_async(fi_frame* h, unsigned policy);
+int fi_frame_wait(fi_frame* h, fi_frame_stats* stats);
 int fi_frame_download(fi_frame* h, fi_poi* out, size_t n);
 /* Per POI of the last frame (template order): 1 if it was run through FFTCC
  * (an FFTCC-seeded frame, the whole-frame fallback, POI-level repair or the
