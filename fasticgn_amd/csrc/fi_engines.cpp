@@ -1251,9 +1251,11 @@ void fi_frame::launch(unsigned policy)
         invalidate_graphs();
     }
     while (!inflight.empty() && (!overlap || inflight.size() >= 2 || !inflight.front().graph)) complete_oldest();
-    // the set no frame in flight uses
-    const int set = inflight.empty() ? scratch_set : (inflight.back().set ^ 1);
-    if (set != scratch_set) swap_scratch();
+    // A frame that may overlap takes the other set from the frame launched
+    // before it (that frame's set is the members': a completion's exchange is
+    // undone), in flight or not: the sets then alternate with the record
+    // slots and the staged targets, and the frame graphs' keys repeat.
+    if (overlap) swap_scratch();
     const int ts = take_target(true);
     const DevImage tview = target(ts);
     FI_HIP(hipStreamWaitEvent(stream, tslot[ts].ready, 0));
